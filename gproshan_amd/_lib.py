@@ -1,0 +1,113 @@
+"""ctypes binding of libgproshan_ptp.so (the C ABI in include/gproshan_ptp.h).
+
+The library is loaded from this package directory only.  If it is missing the
+import fails with an explicit error: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgproshan_ptp.so")
+
+NIL = 0xFFFFFFFF
+PTP_TOL = 1e-3
+
+OK, ERR_INVALID, ERR_NOMEM, ERR_DUPLICATE_SOURCE, ERR_HIP, ERR_NO_DEVICE, ERR_CAPACITY, ERR_KERNEL = \
+    0, -1, -2, -3, -4, -5, -6, -7
+PREC_F32, PREC_F64 = 0, 1
+RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
+SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND = 0, 1, 2
+
+# Every symbol include/gproshan_ptp.h declares.
+EXPORTS = [
+    "gp_ptp_opts_default", "gp_last_error", "gp_device_check",
+    "gp_che_create", "gp_che_destroy", "gp_che_sizes", "gp_che_tables", "gp_che_link",
+    "gp_compute_toplesets", "gp_compute_toplesets_gpu",
+    "gp_parallel_toplesets_propagation_gpu", "gp_parallel_toplesets_propagation_coalescence_gpu",
+    "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
+    "gp_normalize_ptp", "gp_ptp",
+    "gp_plan_create", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
+]
+
+
+class GPError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+class PtpOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
+                ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
+                ("block", C.c_uint32), ("grid", C.c_uint32)]
+
+
+class PtpStats(C.Structure):
+    _fields_ = [("sweeps", C.c_uint32), ("n_limits", C.c_uint32), ("relaxations", C.c_uint64),
+                ("cond_checks", C.c_uint64), ("f64_updates", C.c_uint64), ("kernel_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                          " (there is no fallback implementation)")
+    L = C.CDLL(LIB_PATH)
+    P, u32, i32 = C.c_void_p, C.c_uint32, C.c_int
+    L.gp_ptp_opts_default.argtypes = [C.POINTER(PtpOpts)]
+    L.gp_ptp_opts_default.restype = None
+    L.gp_last_error.restype = C.c_char_p
+    L.gp_device_check.argtypes = [C.c_char_p, C.c_size_t]
+    L.gp_che_create.argtypes = [P, u32, P, u32, C.POINTER(P)]
+    L.gp_che_destroy.argtypes = [P]
+    L.gp_che_destroy.restype = None
+    L.gp_che_sizes.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(i32)]
+    L.gp_che_tables.argtypes = [P, P, P, P]
+    L.gp_che_link.argtypes = [P, u32, P, u32, C.POINTER(u32)]
+    L.gp_compute_toplesets.argtypes = [P, P, u32, P, P, P, u32, C.POINTER(u32)]
+    L.gp_compute_toplesets_gpu.argtypes = [P, P, u32, P, P, P, u32, C.POINTER(u32)]
+    drv = [P, P, u32, P, u32, P, P, P, C.POINTER(C.c_double)]
+    L.gp_parallel_toplesets_propagation_gpu.argtypes = drv
+    L.gp_parallel_toplesets_propagation_coalescence_gpu.argtypes = drv
+    L.gp_parallel_toplesets_propagation_cpu.argtypes = [P, P, u32, P, u32, P, P]
+    L.gp_parallel_toplesets_propagation_coalescence_cpu.argtypes = [P, P, u32, P, u32, P, P]
+    L.gp_normalize_ptp.argtypes = [P, u32]
+    L.gp_ptp.argtypes = [P, P, u32, C.POINTER(PtpOpts), P, P, C.POINTER(PtpStats)]
+    L.gp_plan_create.argtypes = [P, P, u32, C.POINTER(PtpOpts), C.POINTER(P)]
+    L.gp_plan_run.argtypes = [P, C.POINTER(PtpStats)]
+    L.gp_plan_result.argtypes = [P, P]
+    L.gp_plan_trace.argtypes = [P, P, u32, C.POINTER(u32)]
+    L.gp_plan_destroy.argtypes = [P]
+    L.gp_plan_destroy.restype = None
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != OK:
+        raise GPError(rc, lib().gp_last_error().decode())
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_opts(**kw) -> PtpOpts:
+    o = PtpOpts()
+    lib().gp_ptp_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o

@@ -1,0 +1,263 @@
+// extern "C" boundary of libgproshan_ptp.  Every function catches all C++
+// exceptions and maps them to a status code; nothing unwinds across the ABI.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "gp_internal.hpp"
+
+struct gp_che {
+	gp::Che m;
+};
+
+#define GP_GUARD(...)                                                                  \
+	try {                                                                              \
+		gp::clear_error();                                                             \
+		__VA_ARGS__                                                                    \
+	} catch (const std::bad_alloc &) {                                                 \
+		return gp::fail(GP_ERR_NOMEM, "host allocation failed");                       \
+	} catch (const std::exception &e) {                                                \
+		return gp::fail(GP_ERR_INVALID, "exception: %s", e.what());                    \
+	} catch (...) {                                                                    \
+		return gp::fail(GP_ERR_INVALID, "unknown exception");                          \
+	}
+
+extern "C" {
+
+void gp_ptp_opts_default(gp_ptp_opts *o)
+{
+	if (!o) return;
+	std::memset(o, 0, sizeof(*o));
+	o->precision = GP_PREC_F32;
+	o->result_buffer = GP_RESULT_LAST_WRITTEN;
+	o->schedule = GP_SCHED_RINGS;
+	o->coalesce = 1;
+	o->tol = GP_PTP_TOL;
+}
+
+const char *gp_last_error(void) { return gp::last_error(); }
+
+int gp_device_check(char *name_out, size_t cap)
+{
+	GP_GUARD(return gp::device_check(name_out, cap);)
+}
+
+int gp_che_create(const double *xyz, uint32_t n_vertices, const uint32_t *faces, uint32_t n_faces, gp_che **out)
+{
+	GP_GUARD(
+		if (!out) return gp::fail(GP_ERR_INVALID, "out is null");
+		*out = nullptr;
+		auto c = std::make_unique<gp_che>();
+		int rc = gp::build_che(xyz, n_vertices, faces, n_faces, c->m);
+		if (rc) return rc;
+		*out = c.release();
+		return GP_OK;
+	)
+}
+
+void gp_che_destroy(gp_che *mesh) { delete mesh; }
+
+int gp_che_sizes(const gp_che *mesh, uint32_t *nv, uint32_t *nf, uint32_t *nhe, uint32_t *ne, int *manifold)
+{
+	if (!mesh) return gp::fail(GP_ERR_INVALID, "mesh is null");
+	if (nv) *nv = mesh->m.nv;
+	if (nf) *nf = mesh->m.nf;
+	if (nhe) *nhe = mesh->m.nhe;
+	if (ne) *ne = mesh->m.ne;
+	if (manifold) *manifold = mesh->m.manifold;
+	return GP_OK;
+}
+
+int gp_che_tables(const gp_che *mesh, uint32_t *vt, uint32_t *ot, uint32_t *evt)
+{
+	if (!mesh) return gp::fail(GP_ERR_INVALID, "mesh is null");
+	const gp::Che &m = mesh->m;
+	if (vt && m.nhe) std::memcpy(vt, m.vt.data(), sizeof(uint32_t) * m.nhe);
+	if (ot && m.nhe) std::memcpy(ot, m.ot.data(), sizeof(uint32_t) * m.nhe);
+	if (evt) std::memcpy(evt, m.evt.data(), sizeof(uint32_t) * m.nv);
+	return GP_OK;
+}
+
+int gp_che_link(const gp_che *mesh, uint32_t v, uint32_t *he_out, uint32_t cap, uint32_t *n)
+{
+	if (!mesh || !n) return gp::fail(GP_ERR_INVALID, "null argument");
+	const gp::Che &m = mesh->m;
+	if (v >= m.nv) return gp::fail(GP_ERR_INVALID, "vertex out of range");
+	// Walk the star as che::link does and report half-edge ids.
+	uint32_t c = 0;
+	const uint32_t stop = m.evt[v];
+	for (uint32_t he = stop, guard = 0; he != gp::NIL && guard <= m.nhe; guard++) {
+		if (c < cap && he_out) he_out[c] = gp::he_next(he);
+		c++;
+		const uint32_t pv = gp::he_prev(he);
+		if (m.ot[pv] == gp::NIL) {
+			if (c < cap && he_out) he_out[c] = pv;
+			c++;
+		}
+		const uint32_t h2 = m.ot[pv];
+		he = (h2 != stop) ? h2 : gp::NIL;
+	}
+	*n = c;
+	return c > cap ? gp::fail(GP_ERR_CAPACITY, "link larger than cap") : GP_OK;
+}
+
+int gp_compute_toplesets(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t *toplesets,
+                         uint32_t *sorted, uint32_t *limits, uint32_t cap_limits, uint32_t *n_limits)
+{
+	GP_GUARD(
+		if (!mesh || !toplesets || !sorted || !limits || !n_limits || (n_sources && !sources))
+			return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<uint32_t> lim;
+		int rc = gp::toplesets_host(mesh->m, sources, n_sources, toplesets, sorted, lim);
+		if (rc) return rc;
+		*n_limits = (uint32_t) lim.size();
+		if (lim.size() > cap_limits) return gp::fail(GP_ERR_CAPACITY, "limits needs %zu entries", lim.size());
+		std::memcpy(limits, lim.data(), sizeof(uint32_t) * lim.size());
+		return GP_OK;
+	)
+}
+
+int gp_compute_toplesets_gpu(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t *toplesets,
+                             uint32_t *sorted, uint32_t *limits, uint32_t cap_limits, uint32_t *n_limits)
+{
+	GP_GUARD(
+		if (!mesh || !toplesets || !sorted || !limits || !n_limits || (n_sources && !sources))
+			return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<uint32_t> lim;
+		// The device tables are cached on the mesh; the mesh itself is not modified.
+		int rc = gp::toplesets_gpu(const_cast<gp::Che &>(mesh->m), sources, n_sources, toplesets, sorted, lim);
+		if (rc) return rc;
+		*n_limits = (uint32_t) lim.size();
+		if (lim.size() > cap_limits) return gp::fail(GP_ERR_CAPACITY, "limits needs %zu entries", lim.size());
+		std::memcpy(limits, lim.data(), sizeof(uint32_t) * lim.size());
+		return GP_OK;
+	)
+}
+
+static int ptp_gpu_driver(gp_che *mesh, const uint32_t *sources, uint32_t ns, const uint32_t *limits, uint32_t nl,
+                          const uint32_t *sorted, double *dist, uint32_t *clusters, double *seconds, int coalesce)
+{
+	if (!mesh || !dist) return gp::fail(GP_ERR_INVALID, "null argument");
+	if (clusters) return gp::fail(GP_ERR_INVALID, "clusters output is not supported");
+	gp_ptp_opts o;
+	gp_ptp_opts_default(&o);
+	o.coalesce = coalesce;
+	gp_plan *p = nullptr;
+	int rc = gp::plan_create_rings(&mesh->m, sources, ns, limits, nl, sorted, o, &p);
+	if (rc) return rc;
+	std::unique_ptr<gp_plan, void (*)(gp_plan *)> guard(p, gp_plan_destroy);
+	gp_ptp_stats st{};
+	rc = gp::plan_run(p, &st);
+	if (rc) return rc;
+	rc = gp::plan_result(p, dist);
+	if (seconds) *seconds = st.kernel_ms * 1e-3;
+	return rc;
+}
+
+int gp_parallel_toplesets_propagation_gpu(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                                          const uint32_t *limits, uint32_t n_limits, const uint32_t *sorted,
+                                          double *dist, uint32_t *clusters, double *seconds)
+{
+	GP_GUARD(return ptp_gpu_driver(mesh, sources, n_sources, limits, n_limits, sorted, dist, clusters, seconds, 0);)
+}
+
+int gp_parallel_toplesets_propagation_coalescence_gpu(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                                                      const uint32_t *limits, uint32_t n_limits,
+                                                      const uint32_t *sorted, double *dist, uint32_t *clusters,
+                                                      double *seconds)
+{
+	GP_GUARD(return ptp_gpu_driver(mesh, sources, n_sources, limits, n_limits, sorted, dist, clusters, seconds, 1);)
+}
+
+int gp_parallel_toplesets_propagation_cpu(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                                          const uint32_t *limits, uint32_t n_limits, const uint32_t *sorted,
+                                          double *dist)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !limits || !sorted || !dist) return gp::fail(GP_ERR_INVALID, "null argument");
+		if (n_limits < 2) return gp::fail(GP_ERR_INVALID, "empty limits");
+		return gp::ptp_host(mesh->m, sources, n_sources, limits, n_limits, sorted, false, GP_PTP_TOL, dist, nullptr);
+	)
+}
+
+int gp_parallel_toplesets_propagation_coalescence_cpu(const gp_che *mesh, const uint32_t *sources,
+                                                      uint32_t n_sources, const uint32_t *limits,
+                                                      uint32_t n_limits, const uint32_t *sorted, double *dist)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !limits || !sorted || !dist) return gp::fail(GP_ERR_INVALID, "null argument");
+		if (n_limits < 2) return gp::fail(GP_ERR_INVALID, "empty limits");
+		const gp::Che &m = mesh->m;
+		gp::Che cm;
+		std::vector<uint32_t> inv;
+		int rc = gp::coalesce(m, sorted, limits[n_limits - 1], cm, inv);
+		if (rc) return rc;
+		std::vector<uint32_t> src(n_sources);
+		for (uint32_t q = 0; q < n_sources; q++) {
+			if (sources[q] >= m.nv || inv[sources[q]] == gp::NIL) return gp::fail(GP_ERR_INVALID, "source not reached");
+			src[q] = inv[sources[q]];
+		}
+		std::vector<double> d(cm.nv);
+		rc = gp::ptp_host(cm, src.data(), n_sources, limits, n_limits, nullptr, false, GP_PTP_TOL, d.data(), nullptr);
+		if (rc) return rc;
+		for (uint32_t v = 0; v < m.nv; v++) dist[v] = inv[v] != gp::NIL ? d[inv[v]] : INFINITY;
+		return GP_OK;
+	)
+}
+
+int gp_normalize_ptp(double *dist, uint32_t n)
+{
+	if (!dist) return gp::fail(GP_ERR_INVALID, "null argument");
+	double mx = 0;
+	for (uint32_t v = 0; v < n; v++)
+		if (dist[v] < INFINITY && dist[v] > mx) mx = dist[v];
+	for (uint32_t v = 0; v < n; v++) dist[v] /= mx;
+	return GP_OK;
+}
+
+int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_ptp_opts *opts, double *dist,
+           uint32_t *sorted_out, gp_ptp_stats *stats)
+{
+	GP_GUARD(
+		if (!mesh || !dist || !sources) return gp::fail(GP_ERR_INVALID, "null argument");
+		gp_ptp_opts o;
+		if (opts) o = *opts; else gp_ptp_opts_default(&o);
+		const gp::Che &m = mesh->m;
+		std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
+		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
+		if (rc) return rc;
+		if (sorted_out) std::memcpy(sorted_out, sorted.data(), sizeof(uint32_t) * m.nv);
+		gp_plan *p = nullptr;
+		rc = gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, &p);
+		if (rc) return rc;
+		std::unique_ptr<gp_plan, void (*)(gp_plan *)> guard(p, gp_plan_destroy);
+		rc = gp::plan_run(p, stats);
+		if (rc) return rc;
+		return gp::plan_result(p, dist);
+	)
+}
+
+int gp_plan_create(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_ptp_opts *opts, gp_plan **out)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !out) return gp::fail(GP_ERR_INVALID, "null argument");
+		gp_ptp_opts o;
+		if (opts) o = *opts; else gp_ptp_opts_default(&o);
+		const gp::Che &m = mesh->m;
+		std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
+		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
+		if (rc) return rc;
+		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
+	)
+}
+
+int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats) { GP_GUARD(return gp::plan_run(plan, stats);) }
+int gp_plan_result(gp_plan *plan, double *dist) { GP_GUARD(return gp::plan_result(plan, dist);) }
+int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap, uint32_t *n)
+{
+	GP_GUARD(return gp::plan_trace(plan, trace_out, cap, n);)
+}
+void gp_plan_destroy(gp_plan *plan);
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// Compact half-edge mesh, triangle fans and host toplesets.
+//
+// The tables follow the reference's construction rules exactly
+// (che::init -> update_evt_ot_et, src/che.cpp:1254-1264,1295-1362): VT is the
+// face array; OT pairs a half-edge with the first unpaired reversed half-edge
+// in the reference's scan order; EVT is the largest half-edge leaving v,
+// replaced by the outgoing border half-edge on a border, NIL on a vertex with
+// two border fans.  The construction here is parallel: a reversed pair that
+// is the only candidate of both half-edges is order-independent and is paired
+// directly; only the few half-edges with several candidates (non-manifold
+// edges) are paired serially in the reference's order.
+#include "gp_internal.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace gp {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const char *fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_last_error = buf;
+	return status;
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+void clear_error() { g_last_error.clear(); }
+
+Che::~Che()
+{
+	if (dev) device_mesh_free(dev);
+}
+
+int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf, Che &m)
+{
+	if (nv == 0 || !xyz) return fail(GP_ERR_INVALID, "che: no vertices");
+	if (nf && !faces) return fail(GP_ERR_INVALID, "che: faces is null");
+	if (3ull * nf >= 0x7FFFFFFFull || nv >= 0x7FFFFFFFu)
+		return fail(GP_ERR_CAPACITY, "che: more than 2^31 half-edges or vertices");
+	const uint32_t nhe = 3u * nf;
+	for (uint32_t he = 0; he < nhe; he++)
+		if (faces[he] >= nv) return fail(GP_ERR_INVALID, "che: face index %u out of range", faces[he]);
+
+	m.nv = nv; m.nf = nf; m.nhe = nhe; m.manifold = true;
+	m.gt.assign(xyz, xyz + 3 * (size_t) nv);
+	m.vt.assign(faces, faces + nhe);
+	m.ot.assign(nhe, NIL);
+	m.evt.assign(nv, NIL);
+
+	// Half-edges leaving each vertex, ascending (counting sort).
+	std::vector<uint32_t> off(nv + 1, 0), lst(nhe);
+	for (uint32_t he = 0; he < nhe; he++) off[m.vt[he] + 1]++;
+	for (uint32_t v = 0; v < nv; v++) off[v + 1] += off[v];
+	{
+		std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+		for (uint32_t he = 0; he < nhe; he++) lst[fill[m.vt[he]]++] = he;
+	}
+
+	// Candidates of he: g = prev(h), h leaving VT[he], with VT[g] == VT[next(he)].
+	// The relation is symmetric (g is a candidate of he iff he is one of g).
+	std::vector<uint8_t> ncand(nhe);
+	std::vector<uint32_t> first(nhe, NIL);
+	#pragma omp parallel for schedule(static)
+	for (int64_t he = 0; he < (int64_t) nhe; he++) {
+		const uint32_t a = m.vt[he], b = m.vt[he_next((uint32_t) he)];
+		uint32_t c = 0, f = NIL;
+		for (uint32_t q = off[a]; q < off[a + 1]; q++) {
+			const uint32_t g = he_prev(lst[q]);
+			if (m.vt[g] == b) { if (c == 0) f = g; c++; }
+		}
+		ncand[he] = (uint8_t) std::min<uint32_t>(c, 2);
+		first[he] = f;
+	}
+	// Mutually unique pairs.
+	#pragma omp parallel for schedule(static)
+	for (int64_t he = 0; he < (int64_t) nhe; he++) {
+		if (ncand[he] == 1) {
+			const uint32_t g = first[he];
+			if (ncand[g] == 1) m.ot[he] = g;
+		}
+	}
+	// Several candidates: the reference's greedy scan, in ascending order.
+	for (uint32_t he = 0; he < nhe; he++) {
+		if (ncand[he] < 2 || m.ot[he] != NIL) continue;
+		const uint32_t a = m.vt[he], b = m.vt[he_next(he)];
+		for (uint32_t q = off[a]; q < off[a + 1]; q++) {
+			const uint32_t g = he_prev(lst[q]);
+			if (m.vt[g] == b && m.ot[he] == NIL && m.ot[g] == NIL) {
+				m.ot[he] = g;
+				m.ot[g] = he;
+			}
+		}
+	}
+	// ET holds he when it was still unpaired at its turn: unpaired, or paired
+	// with a larger half-edge.
+	uint32_t ne = 0;
+	#pragma omp parallel for reduction(+: ne)
+	for (int64_t he = 0; he < (int64_t) nhe; he++)
+		ne += (m.ot[he] == NIL || m.ot[he] > (uint32_t) he);
+	m.ne = ne;
+
+	// EVT: largest outgoing half-edge, then the border rule per vertex.
+	bool manifold = true;
+	#pragma omp parallel for schedule(static) reduction(&&: manifold)
+	for (int64_t v = 0; v < (int64_t) nv; v++) {
+		if (off[v] == off[v + 1]) continue;
+		uint32_t e = lst[off[v + 1] - 1];
+		for (uint32_t q = off[v]; q < off[v + 1]; q++) {
+			const uint32_t he = lst[q];
+			if (m.ot[he] != NIL || e == NIL) continue;
+			if (m.ot[e] == NIL && e != he) { manifold = false; e = NIL; }
+			else e = he;
+		}
+		m.evt[v] = e;
+	}
+	m.manifold = manifold;
+
+	// Fans.
+	m.fan_off.assign(nv + 1, 0);
+	m.fan_closed.assign(nv, 0);
+	#pragma omp parallel for schedule(static)
+	for (int64_t v = 0; v < (int64_t) nv; v++) {
+		const uint32_t stop = m.evt[v];
+		if (stop == NIL) continue;
+		uint32_t c = 0, he = stop, guard = 0;
+		bool closed = false;
+		while (he != NIL && guard++ <= nhe) {
+			c++;
+			const uint32_t h2 = m.ot[he_prev(he)];
+			if (h2 == stop) { closed = true; break; }
+			he = h2;
+		}
+		m.fan_off[v + 1] = c + 1;
+		m.fan_closed[v] = closed;
+	}
+	for (uint32_t v = 0; v < nv; v++) m.fan_off[v + 1] += m.fan_off[v];
+	m.fan.assign(m.fan_off[nv], NIL);
+	double esum = 0;
+	uint64_t ecnt = 0;
+	#pragma omp parallel for schedule(static) reduction(+: esum, ecnt)
+	for (int64_t v = 0; v < (int64_t) nv; v++) {
+		const uint32_t stop = m.evt[v];
+		if (stop == NIL) continue;
+		uint32_t *L = m.fan.data() + m.fan_off[v];
+		const uint32_t c = m.fan_off[v + 1] - m.fan_off[v] - 1;
+		uint32_t he = stop;
+		for (uint32_t k = 0; k < c; k++) {
+			L[k] = m.vt[he_next(he)];
+			if (k + 1 == c) L[c] = m.vt[he_prev(he)];
+			he = m.ot[he_prev(he)];
+		}
+		const double *p = &m.gt[3 * (size_t) v];
+		for (uint32_t k = 0; k <= c; k++) {
+			const double *q = &m.gt[3 * (size_t) L[k]];
+			esum += std::sqrt((q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) +
+			                  (q[2] - p[2]) * (q[2] - p[2]));
+			ecnt++;
+		}
+	}
+	m.mean_edge = (ecnt && esum > 0) ? esum / (double) ecnt : 1.0;
+	return GP_OK;
+}
+
+void build_geo32(const Che &m, std::vector<uint32_t> &fan, std::vector<Geo32> &geo, uint32_t &flagged)
+{
+	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.5;
+	const uint32_t nv = m.nv, nfan = m.fan_off[nv];
+	fan = m.fan;
+	geo.assign(std::max<uint32_t>(nfan, 1), Geo32{0.f, 0.f, 0.f, 0.f});
+	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
+	uint32_t nflag = 0;
+	#pragma omp parallel for schedule(static) reduction(+: nflag)
+	for (int64_t v = 0; v < (int64_t) nv; v++) {
+		const uint32_t s = m.fan_off[v], e = m.fan_off[v + 1];
+		const double *pv = &m.gt[3 * (size_t) v];
+		for (uint32_t k = s; k < e; k++) {
+			const double *pa = &m.gt[3 * (size_t) m.fan[k]];
+			const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+			const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+			Geo32 gk{ (float) (q0 * ih2), 0.f, 0.f, 0.f };
+			if (k + 1 < e) {
+				const double *pb = &m.gt[3 * (size_t) m.fan[k + 1]];
+				const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
+				const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
+				const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
+				const double cx = X0[1] * X1[2] - X0[2] * X1[1];
+				const double cy = X0[2] * X1[0] - X0[0] * X1[2];
+				const double cz = X0[0] * X1[1] - X0[1] * X1[0];
+				const double det = cx * cx + cy * cy + cz * cz;
+				const double E[3] = { pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2] };
+				gk.y = (float) (g * ih2);
+				gk.z = (float) (det * ih2 * ih2);
+				gk.w = (float) ((E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2);
+				const double qq = q0 * q1;
+				if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { fan[k] |= F64_FLAG; nflag++; }
+			}
+			geo[k] = gk;
+		}
+	}
+	flagged = nflag;
+}
+
+int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels,
+                   uint32_t *sorted, std::vector<uint32_t> &limits)
+{
+	limits.clear();
+	if (ns == 0) return GP_OK;                                   // src/che.cpp:548
+	for (uint32_t q = 0; q < ns; q++)
+		if (src[q] >= m.nv) return fail(GP_ERR_INVALID, "toplesets: source %u out of range", src[q]);
+	std::fill(levels, levels + m.nv, NIL);
+	uint32_t p = 0;
+	for (uint32_t q = 0; q < ns; q++) {
+		if (levels[src[q]] != NIL)
+			return fail(GP_ERR_DUPLICATE_SOURCE, "toplesets: source %u listed twice", src[q]);
+		sorted[p++] = src[q];
+		levels[src[q]] = 0;
+	}
+	limits.push_back(0);
+	uint32_t level = 0;
+	for (uint32_t i = 0; i < p; i++) {
+		const uint32_t v = sorted[i];
+		if (levels[v] > level) { level++; limits.push_back(i); }
+		const uint32_t *L = m.fan.data() + m.fan_off[v];
+		const uint32_t n = m.link_size(v);
+		const uint32_t lv = levels[v] + 1;
+		for (uint32_t k = 0; k < n; k++) {
+			const uint32_t u = L[k];
+			if (levels[u] == NIL) { levels[u] = lv; sorted[p++] = u; }
+		}
+	}
+	limits.push_back(p);
+	for (uint32_t i = p; i < m.nv; i++) sorted[i] = NIL;
+	return GP_OK;
+}
+
+int coalesce(const Che &m, const uint32_t *sorted, uint32_t nr, Che &out, std::vector<uint32_t> &inv)
+{
+	inv.assign(m.nv, NIL);
+	for (uint32_t q = 0; q < nr; q++) inv[sorted[q]] = q;
+	// Faces with all three vertices reached, in their original order, with
+	// renumbered ids (src/geodesics_ptp.cpp:33-35).
+	std::vector<uint32_t> F;
+	F.reserve(m.nhe);
+	for (uint32_t f = 0; f < m.nf; f++) {
+		const uint32_t a = inv[m.vt[3 * f]], b = inv[m.vt[3 * f + 1]], c = inv[m.vt[3 * f + 2]];
+		if (a != NIL && b != NIL && c != NIL) { F.push_back(a); F.push_back(b); F.push_back(c); }
+	}
+	std::vector<double> V(3 * (size_t) nr);
+	for (uint32_t q = 0; q < nr; q++)
+		for (int k = 0; k < 3; k++) V[3 * (size_t) q + k] = m.gt[3 * (size_t) sorted[q] + k];
+	if (nr == 0) return fail(GP_ERR_INVALID, "coalesce: nothing reached");
+	return build_che(V.data(), nr, F.data(), (uint32_t) (F.size() / 3), out);
+}
+
+}  // namespace gp

@@ -1,0 +1,98 @@
+// Internal declarations of libgproshan_ptp (host side).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gproshan_ptp.h"
+
+namespace gp {
+
+constexpr uint32_t NIL = GP_NIL;
+
+// Sets the thread-local error text and returns the status.
+int fail(int status, const char *fmt, ...);
+
+inline uint32_t he_next(uint32_t he) { return 3u * (he / 3u) + (he + 1u) % 3u; }
+inline uint32_t he_prev(uint32_t he) { return 3u * (he / 3u) + (he + 2u) % 3u; }
+
+struct DeviceMesh;  // ptp_gpu.hip
+
+// Compact half-edge mesh plus the per-vertex triangle fans the solvers use.
+//
+// Fan of v: the star walk from EVT[v] (he <- OT[prev(he)], include/che.h:10)
+// visits half-edges he_0..he_{m-1} leaving v; the fan list is
+// L = (VT[next(he_0)], ..., VT[next(he_{m-1})], VT[prev(he_{m-1})]), m+1 ids,
+// and triangle k of the star is (v, L[k], L[k+1]).  For a closed fan
+// L[m] == L[0].  The link of v (src/che.cpp:102-112) is L[0..m) for a closed
+// fan and L[0..m] for an open one.  A vertex with EVT == NIL has no entries.
+struct Che {
+	uint32_t nv = 0, nf = 0, nhe = 0, ne = 0;
+	bool manifold = true;
+	std::vector<double> gt;     // 3*nv
+	std::vector<uint32_t> vt, ot, evt;
+
+	std::vector<uint32_t> fan_off;    // nv+1
+	std::vector<uint32_t> fan;        // fan lists
+	std::vector<uint8_t> fan_closed;  // 1 when the walk returned to EVT[v]
+	double mean_edge = 1.0;           // mean length of the fan edges
+
+	DeviceMesh *dev = nullptr;        // lazily uploaded device tables
+
+	uint32_t n_tri(uint32_t v) const {
+		const uint32_t c = fan_off[v + 1] - fan_off[v];
+		return c ? c - 1 : 0;
+	}
+	uint32_t link_size(uint32_t v) const {
+		const uint32_t c = fan_off[v + 1] - fan_off[v];
+		return c ? (fan_closed[v] ? c - 1 : c) : 0;
+	}
+	~Che();
+};
+
+constexpr uint32_t ID_MASK = 0x7FFFFFFFu;
+constexpr uint32_t F64_FLAG = 0x80000000u;
+
+// FP32 geometry of fan entry k of a vertex, in units of the mean edge length h:
+// x = |X_k|^2, y = X_k.X_{k+1}, z = |X_k x X_{k+1}|^2, w = |X_{k+1} - X_k|^2
+// (y, z, w describe triangle (k, k+1) and are 0 on the last entry).
+struct Geo32 { float x, y, z, w; };
+
+// Fan ids with bit 31 set on entry k when triangle (k, k+1) takes the FP64
+// update: its angle at the updated vertex is below ~10 deg or above ~170 deg
+// (sin^2 < 0.03: near-degenerate Gram matrix) or above 120 deg (obtuse).
+void build_geo32(const Che &m, std::vector<uint32_t> &fan, std::vector<Geo32> &geo, uint32_t &flagged);
+
+// Builds VT/OT/EVT per the reference's rules and the fan tables.
+int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf, Che &m);
+
+// Host BFS rings (che::compute_toplesets).  limits is resized.
+int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels,
+                   uint32_t *sorted, std::vector<uint32_t> &limits);
+
+// Vertices renumbered by rings (ptp_coalescence, src/geodesics_ptp.cpp:15-38):
+// the mesh of the reached vertices in ring order, plus inv (old -> new).
+int coalesce(const Che &m, const uint32_t *sorted, uint32_t n_reached, Che &out,
+             std::vector<uint32_t> &inv);
+
+// FP64 host schedule (PTP_CPU semantics, last-read or last-written buffer).
+int ptp_host(const Che &m, const uint32_t *src, uint32_t ns, const uint32_t *limits,
+             uint32_t nl, const uint32_t *order, bool last_written, double tol, double *dist_out,
+             uint32_t *sweeps_out);
+
+// Device entry points (ptp_gpu.hip).
+int device_check(char *name, size_t cap);
+void device_mesh_free(DeviceMesh *d);
+int toplesets_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels, uint32_t *sorted,
+                  std::vector<uint32_t> &limits);
+int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_t *limits, uint32_t nl,
+                      const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out);
+int plan_run(gp_plan *p, gp_ptp_stats *stats);
+int plan_result(gp_plan *p, double *dist);
+int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n);
+
+const char *last_error();
+void clear_error();
+
+}  // namespace gp

@@ -1,0 +1,103 @@
+// Triangle update of the parallel toplesets propagation, shared by the host
+// solver (ptp_cpu.cpp, compiled by g++ with -ffp-contract=off) and the gfx950
+// kernels (compiled by hipcc; the FP64 form is evaluated with contraction off
+// so that it rounds like the reference's default x86-64 build).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#ifdef __HIPCC__
+#define GP_HD __host__ __device__ __forceinline__
+#else
+#define GP_HD inline
+#endif
+
+namespace gp {
+
+// The reference's planar-wavefront update of vertex v from the triangle
+// (v, a, b) with arrival times t0 at a and t1 at b (update_step,
+// src/geodesics_ptp.cpp:201-262; cu_update_step, src/cuda/geodesics_ptp.cu:301-369).
+// Same operations in the same order; NaN (det == 0) is returned as NaN and
+// loses every '<' in the caller, as in the reference.
+GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb, double t0, double t1)
+{
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+	const double X0[3] = { ga[0] - gv[0], ga[1] - gv[1], ga[2] - gv[2] };
+	const double X1[3] = { gb[0] - gv[0], gb[1] - gv[1], gb[2] - gv[2] };
+
+	const double q00 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+	const double q01 = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
+	const double q10 = X1[0] * X0[0] + X1[1] * X0[1] + X1[2] * X0[2];
+	const double q11 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
+
+	const double det = q00 * q11 - q01 * q10;
+	const double Q00 = q11 / det;
+	const double Q01 = -q01 / det;
+	const double Q10 = -q10 / det;
+	const double Q11 = q00 / det;
+
+	const double s = Q00 + Q01 + Q10 + Q11;
+	const double delta = t0 * (Q00 + Q10) + t1 * (Q01 + Q11);
+	const double dis = delta * delta - s * (t0 * t0 * Q00 + t0 * t1 * (Q10 + Q01) + t1 * t1 * Q11 - 1);
+
+	double p = (delta + sqrt(dis)) / s;
+
+	const double tp0 = t0 - p, tp1 = t1 - p;
+	const double n0 = tp0 * (X0[0] * Q00 + X1[0] * Q10) + tp1 * (X0[0] * Q01 + X1[0] * Q11);
+	const double n1 = tp0 * (X0[1] * Q00 + X1[1] * Q10) + tp1 * (X0[1] * Q01 + X1[1] * Q11);
+	const double n2 = tp0 * (X0[2] * Q00 + X1[2] * Q10) + tp1 * (X0[2] * Q01 + X1[2] * Q11);
+	const double cond0 = X0[0] * n0 + X0[1] * n1 + X0[2] * n2;
+	const double cond1 = X1[0] * n0 + X1[1] * n1 + X1[2] * n2;
+	const double c0 = cond0 * Q00 + cond1 * Q01;
+	const double c1 = cond0 * Q10 + cond1 * Q11;
+
+	if (t0 == INFINITY || t1 == INFINITY || dis < 0 || c0 >= 0 || c1 >= 0) {
+		const double dp0 = t0 + sqrt(X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]);
+		const double dp1 = t1 + sqrt(X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2]);
+		p = (dp1 < dp0) ? dp1 : dp0;
+	}
+	return p;
+}
+
+// Per-triangle FP32 geometry, precomputed on the host in FP64 in units of
+// the mesh's mean edge length h:  q0 = |X0|^2, q1 = |X1|^2 (from the next
+// fan entry), g = X0.X1, det = |X0 x X1|^2, S = |X1 - X0|^2.
+//
+// The same update as update_ref_f64, rewritten so that FP32 keeps its
+// precision: the quadratic (t - p1)^T Q (t - p1) = 1 is invariant under
+// t -> t - m1, p -> p - m, so it is solved for u = (t - min t)/h, which is
+// O(1), instead of t ~ distance/h^2 in Q-units where the discriminant
+// cancels catastrophically.  Multiplying by det > 0 gives the adjugate form:
+//   D = u0 (q1 - g) + u1 (q0 - g),  E = u0^2 q1 - 2 u0 u1 g + u1^2 q0 - det,
+//   p' = (D + sqrt(D^2 - S E)) / S,
+//   sign c0 = sign(q1 (u0 - p') - g (u1 - p')),  sign c1 = sign(q0 (u1 - p') - g (u0 - p')),
+// which are the reference's delta, dis, p and c scaled by det (and det^2).
+GP_HD float update_stable_f32(float q0, float q1, float g, float det, float S, float t0, float t1,
+                              float h, float inv_h)
+{
+	if (!(t0 != INFINITY && t1 != INFINITY)) {
+		const float dp0 = t0 + sqrtf(q0) * h;
+		const float dp1 = t1 + sqrtf(q1) * h;
+		return (dp1 < dp0) ? dp1 : dp0;
+	}
+	const float m = fminf(t0, t1);
+	const float u0 = (t0 - m) * inv_h;
+	const float u1 = (t1 - m) * inv_h;
+	const float D = u0 * (q1 - g) + u1 * (q0 - g);
+	const float E = u0 * u0 * q1 - 2.0f * u0 * u1 * g + u1 * u1 * q0 - det;
+	const float disc = D * D - S * E;
+	float pp = (D + sqrtf(disc)) / S;
+	const float c0 = q1 * (u0 - pp) - g * (u1 - pp);
+	const float c1 = q0 * (u1 - pp) - g * (u0 - pp);
+	if (disc < 0.0f || c0 >= 0.0f || c1 >= 0.0f) {
+		const float dp0 = t0 + sqrtf(q0) * h;
+		const float dp1 = t1 + sqrtf(q1) * h;
+		return (dp1 < dp0) ? dp1 : dp0;
+	}
+	return m + pp * h;
+}
+
+}  // namespace gp

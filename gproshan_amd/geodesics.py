@@ -1,0 +1,275 @@
+"""Python mirror of gproshan's PTP front end over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+  * ``che(vertices, faces)``                 -- include/che.h:53, src/che.cpp:84-87
+  * ``che.compute_toplesets(sources)``       -- include/che.h:107, src/che.cpp:546-593
+  * ``parallel_toplesets_propagation_*``     -- include/geodesics_ptp.h:34-40
+  * ``normalize_ptp``                        -- include/geodesics_ptp.h:46
+  * ``geodesics(mesh, sources, opt, ...)``   -- include/geodesics.h:18-72,
+    src/geodesics.cpp:16-110,192-240 (PTP_CPU and PTP_GPU only)
+
+Differences, by design: errors are raised as ``GPError`` instead of asserts or
+out-of-bounds accesses (duplicate sources, an empty source list); the GPU
+arithmetic precision and band schedule are keyword options.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import GPError, NIL, PTP_TOL, check, lib, ptr
+
+__all__ = ["che", "geodesics", "parallel_toplesets_propagation_gpu",
+           "parallel_toplesets_propagation_coalescence_gpu", "parallel_toplesets_propagation_cpu",
+           "parallel_toplesets_propagation_coalescence_cpu", "normalize_ptp", "GPError", "NIL", "PTP_TOL"]
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32).reshape(-1))
+
+
+class che:
+    """Compact half-edge mesh (VT/OT/EVT tables built by the library)."""
+
+    def __init__(self, vertices, faces):
+        self._xyz = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).reshape(-1, 3))
+        self._faces = _u32(faces)
+        h = C.c_void_p()
+        check(lib().gp_che_create(ptr(self._xyz), len(self._xyz), ptr(self._faces), len(self._faces) // 3,
+                                  C.byref(h)))
+        self._h = h
+        nv, nf, nhe, ne, man = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int()
+        check(lib().gp_che_sizes(self._h, C.byref(nv), C.byref(nf), C.byref(nhe), C.byref(ne), C.byref(man)))
+        self._nv, self._nf, self._nhe, self._ne, self._manifold = nv.value, nf.value, nhe.value, ne.value, bool(man.value)
+        self._tables = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().gp_che_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def n_vertices(self):
+        return self._nv
+
+    def n_faces(self):
+        return self._nf
+
+    def n_half_edges(self):
+        return self._nhe
+
+    def n_edges(self):
+        return self._ne
+
+    def is_manifold(self):
+        return self._manifold
+
+    def gt(self, v=None):
+        return self._xyz if v is None else self._xyz[v]
+
+    def tables(self):
+        if self._tables is None:
+            vt = np.zeros(max(self._nhe, 1), np.uint32)
+            ot = np.zeros(max(self._nhe, 1), np.uint32)
+            evt = np.zeros(self._nv, np.uint32)
+            check(lib().gp_che_tables(self._h, ptr(vt), ptr(ot), ptr(evt)))
+            self._tables = (vt[:self._nhe], ot[:self._nhe], evt)
+        return self._tables
+
+    def vt(self, he):
+        return int(self.tables()[0][he])
+
+    def ot(self, he):
+        return int(self.tables()[1][he])
+
+    def evt(self, v):
+        return int(self.tables()[2][v])
+
+    def link(self, v):
+        out = np.zeros(256, np.uint32)
+        n = C.c_uint32()
+        check(lib().gp_che_link(self._h, v, ptr(out), 256, C.byref(n)))
+        return out[:n.value].copy()
+
+    def compute_toplesets(self, sources, gpu: bool = False):
+        """Returns (toplesets u32[V], sorted u32[V], limits u32[R+1])."""
+        src = _u32(sources)
+        top = np.zeros(self._nv, np.uint32)
+        srt = np.zeros(self._nv, np.uint32)
+        cap = self._nv + 2
+        lim = np.zeros(cap, np.uint32)
+        nl = C.c_uint32()
+        fn = lib().gp_compute_toplesets_gpu if gpu else lib().gp_compute_toplesets
+        check(fn(self._h, ptr(src), len(src), ptr(top), ptr(srt), ptr(lim), cap, C.byref(nl)))
+        return top, srt, lim[:nl.value].copy()
+
+
+def _driver(fn, mesh, sources, toplesets, gpu):
+    src = _u32(sources)
+    _, srt, lim = toplesets
+    srt, lim = _u32(srt), _u32(lim)
+    dist = np.full(mesh.n_vertices(), np.inf)
+    if gpu:
+        secs = C.c_double()
+        check(fn(mesh.handle, ptr(src), len(src), ptr(lim), len(lim), ptr(srt), ptr(dist), None, C.byref(secs)))
+        return dist, secs.value
+    check(fn(mesh.handle, ptr(src), len(src), ptr(lim), len(lim), ptr(srt), ptr(dist)))
+    return dist
+
+
+def parallel_toplesets_propagation_gpu(mesh, sources, toplesets):
+    """Returns (dist, seconds); sorted-indirected window sweeps on the GPU."""
+    return _driver(lib().gp_parallel_toplesets_propagation_gpu, mesh, sources, toplesets, True)
+
+
+def parallel_toplesets_propagation_coalescence_gpu(mesh, sources, toplesets):
+    """Returns (dist, seconds); ring-renumbered window sweeps on the GPU."""
+    return _driver(lib().gp_parallel_toplesets_propagation_coalescence_gpu, mesh, sources, toplesets, True)
+
+
+def parallel_toplesets_propagation_cpu(mesh, sources, toplesets):
+    return _driver(lib().gp_parallel_toplesets_propagation_cpu, mesh, sources, toplesets, False)
+
+
+def parallel_toplesets_propagation_coalescence_cpu(mesh, sources, toplesets):
+    return _driver(lib().gp_parallel_toplesets_propagation_coalescence_cpu, mesh, sources, toplesets, False)
+
+
+def normalize_ptp(dist):
+    d = np.ascontiguousarray(dist, dtype=np.float64)
+    check(lib().gp_normalize_ptp(ptr(d), len(d)))
+    return d
+
+
+def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRITTEN, schedule=L.SCHED_RINGS,
+        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0):
+    """One PTP solve with explicit options; returns (dist, sorted, stats dict)."""
+    src = _u32(sources)
+    o = L.default_opts(precision=precision, result_buffer=result_buffer, schedule=schedule,
+                       coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid)
+    dist = np.zeros(mesh.n_vertices())
+    srt = np.zeros(mesh.n_vertices(), np.uint32)
+    st = L.PtpStats()
+    check(lib().gp_ptp(mesh.handle, ptr(src), len(src), C.byref(o), ptr(dist), ptr(srt), C.byref(st)))
+    return dist, srt, st.as_dict()
+
+
+class Plan:
+    """Device-resident solve (gp_plan_*): buffers stay in HBM between runs."""
+
+    def __init__(self, mesh, sources, **kw):
+        self.mesh = mesh
+        self._src = _u32(sources)
+        o = L.default_opts(**{k: (int(v) if k == "coalesce" else v) for k, v in kw.items()})
+        h = C.c_void_p()
+        check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
+        self._h = h
+
+    def run(self):
+        st = L.PtpStats()
+        check(lib().gp_plan_run(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def result(self):
+        d = np.zeros(self.mesh.n_vertices())
+        check(lib().gp_plan_result(self._h, ptr(d)))
+        return d
+
+    def trace(self):
+        n = C.c_uint32()
+        check(lib().gp_plan_trace(self._h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 8), np.uint32)
+        check(lib().gp_plan_trace(self._h, ptr(out), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gp_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class geodesics:
+    """gproshan::geodesics restricted to the PTP options.
+
+    The option numbering is the reference's CUDA build (GPROSHAN_CUDA defined,
+    include/geodesics.h:21-28).  FM, HEAT_FLOW and HEAT_FLOW_GPU are other
+    algorithms, outside this library: requesting them raises.
+    """
+    FM, PTP_GPU, HEAT_FLOW_GPU, PTP_CPU, HEAT_FLOW = range(5)
+
+    def __init__(self, mesh, sources, opt=FM, e_dist=None, cluster=False, n_iter=0, radio=math.inf, *,
+                 precision=L.PREC_F32, schedule=L.SCHED_RINGS):
+        n = mesh.n_vertices()
+        if n <= 0:
+            raise GPError(L.ERR_INVALID, "mesh has no vertices")
+        src = _u32(sources)
+        if len(src) == 0:
+            raise GPError(L.ERR_INVALID, "no sources")            # src/geodesics.cpp:31
+        if cluster:
+            raise GPError(L.ERR_INVALID, "cluster output is not supported")
+        if e_dist is not None and (not isinstance(e_dist, np.ndarray) or e_dist.dtype != np.float64
+                                   or e_dist.size < n or not e_dist.flags.c_contiguous):
+            raise GPError(L.ERR_INVALID, "e_dist must be a contiguous float64 array of n_vertices")
+        self.clusters = None
+        self._dist = e_dist if e_dist is not None else np.empty(n)
+        self._dist[:n] = np.inf                                     # src/geodesics.cpp:28-29
+        self._sorted = np.full(n, NIL, np.uint32)                   # :27
+        self._n_sorted = 0
+        self._n = n
+        if opt == self.PTP_CPU:
+            ts = mesh.compute_toplesets(src)                        # src/geodesics.cpp:192-207
+            self._sorted[:] = ts[1]
+            self._dist[:n] = parallel_toplesets_propagation_coalescence_cpu(mesh, src, ts)
+        elif opt == self.PTP_GPU:
+            # One source: ring-renumbered (coalesced) sweeps; several: sorted[]
+            # indirection (src/geodesics.cpp:225-240).
+            d, srt, _ = ptp(mesh, src, precision=precision, schedule=schedule, coalesce=len(src) == 1)
+            self._sorted[:] = srt
+            self._dist[:n] = d
+        else:
+            raise GPError(L.ERR_INVALID, f"option {opt} is not a PTP option")
+
+    def __getitem__(self, i):
+        if not 0 <= i < self._n:
+            raise IndexError(i)
+        return self._dist[i]
+
+    def __call__(self, i):
+        if not 0 <= i < self._n:
+            raise IndexError(i)
+        return int(self._sorted[i])
+
+    def n_sorted_index(self):
+        return self._n_sorted
+
+    def copy_sorted_index(self, n):
+        if n > self._n_sorted:
+            raise GPError(L.ERR_INVALID, "n exceeds n_sorted_index()")
+        return self._sorted[:n].copy()
+
+    def farthest(self):
+        if self._n_sorted == 0:                                     # src/geodesics.cpp:60-64 asserts
+            raise GPError(L.ERR_INVALID, "farthest() needs a sorted run (FM); PTP leaves n_sorted = 0")
+        return int(self._sorted[self._n_sorted - 1])
+
+    def radio(self):
+        return self._dist[self.farthest()]
+
+    def normalize(self):
+        # n_sorted is 0 after PTP, so the reference falls back to normalize_ptp
+        # (src/geodesics.cpp:77-90).
+        self._dist[:self._n] = normalize_ptp(self._dist[:self._n])
+
+    @property
+    def dist(self):
+        return self._dist[:self._n]

@@ -1,0 +1,194 @@
+/*
+ * gproshan_ptp.h -- C ABI of the MI355X (gfx950) parallel toplesets
+ * propagation (PTP) geodesics library, libgproshan_ptp.so.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (GP_OK == 0, negative on error; gp_last_error() gives the text of
+ * the last error on the calling thread) and never throws.  Each declaration
+ * names the reference interface it replaces (path:line in larc/gproshan).
+ *
+ * Index type: uint32_t (index_t, include/include.h:19), GP_NIL = 0xFFFFFFFF
+ * (NIL, include/include.h:12).  Distances cross the boundary as double
+ * (distance_t = real_t = double in the reference's default build,
+ * include/include.h:21-27); the arithmetic precision on the GPU is chosen by
+ * gp_ptp_opts.precision.
+ */
+#ifndef GPROSHAN_PTP_H
+#define GPROSHAN_PTP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_NIL 0xFFFFFFFFu
+#define GP_PTP_TOL 1e-3  /* PTP_TOL, include/geodesics_ptp.h:11 */
+
+enum gp_status {
+	GP_OK = 0,
+	GP_ERR_INVALID = -1,          /* bad argument (null pointer, index out of range, ...) */
+	GP_ERR_NOMEM = -2,            /* host or device allocation failed                      */
+	GP_ERR_DUPLICATE_SOURCE = -3, /* a source vertex is listed twice                       */
+	GP_ERR_HIP = -4,              /* a HIP runtime call failed                             */
+	GP_ERR_NO_DEVICE = -5,        /* no HIP device, or the device is not gfx950            */
+	GP_ERR_CAPACITY = -6,         /* output buffer too small / size over the supported max */
+	GP_ERR_KERNEL = -7            /* a kernel reported an internal error (bounded wait hit) */
+};
+
+/* Arithmetic precision of the GPU relaxation. */
+enum gp_precision {
+	GP_PREC_F32 = 0,  /* FP32 bulk update; FP64 update on obtuse/near-degenerate triangles;
+	                     FP64 convergence test.  Default.                                   */
+	GP_PREC_F64 = 1   /* FP64 throughout, the reference's default build.                    */
+};
+
+/* Which double buffer the call returns.  The reference's CPU drivers return
+ * the buffer the last sweep read (src/geodesics_ptp.cpp:113,193-198); its
+ * GPU drivers return the buffer the last sweep wrote
+ * (src/cuda/geodesics_ptp.cu:228-231, src/cuda/geodesics_ptp_coalescence.cu:156-159). */
+enum gp_result_buffer {
+	GP_RESULT_LAST_WRITTEN = 0,
+	GP_RESULT_LAST_READ = 1
+};
+
+/* Band schedule. */
+enum gp_schedule {
+	GP_SCHED_RINGS = 0,        /* toplesets (BFS rings), the reference schedule                */
+	GP_SCHED_LEARNED_BAND = 1, /* bands from the learned distance predictor (no reference counterpart) */
+	GP_SCHED_EUCLID_BAND = 2   /* bands from Euclidean distance to the nearest source (ablation) */
+};
+
+typedef struct gp_ptp_opts {
+	int precision;      /* gp_precision                                          */
+	int result_buffer;  /* gp_result_buffer                                      */
+	int schedule;       /* gp_schedule                                           */
+	int coalesce;       /* 1: renumber vertices by rings before the sweeps       */
+	double tol;         /* relative-change tolerance of the ring test (PTP_TOL)  */
+	uint32_t max_sweeps;/* 0: the reference's 2*|limits| cap                     */
+	uint32_t block;     /* threads per workgroup (0: autotuned/default)          */
+	uint32_t grid;      /* workgroups (0: autotuned/default)                     */
+} gp_ptp_opts;
+
+typedef struct gp_ptp_stats {
+	uint32_t sweeps;          /* sweeps run                                           */
+	uint32_t n_limits;        /* |limits| of the band structure used                  */
+	uint64_t relaxations;     /* sum over sweeps of the window size (vertex relaxations) */
+	uint64_t cond_checks;     /* sum over sweeps of the ring-i size (convergence checks) */
+	uint64_t f64_updates;     /* FP64-fallback triangle updates (GP_PREC_F32)         */
+	double kernel_ms;         /* device time of the sweep loop (hipEvent)             */
+	double total_ms;          /* host wall time of the whole call                     */
+} gp_ptp_stats;
+
+/* Fill opts with the defaults (FP32, last written, rings, coalesce, PTP_TOL). */
+void gp_ptp_opts_default(gp_ptp_opts *opts);
+
+/* Text of the last error on this thread ("" if none). */
+const char *gp_last_error(void);
+
+/* Library/device probe: returns GP_OK if a gfx950 device is usable. */
+int gp_device_check(char *name_out, size_t cap);
+
+/* ---- compact half-edge mesh ------------------------------------------- */
+
+typedef struct gp_che gp_che;
+
+/* che::che(const vertex*, const index_t& n_v, const index_t* faces, const index_t& n_f)
+ * -> che::init -> update_evt_ot_et (include/che.h:53; src/che.cpp:84-87,1254-1264,1295-1362).
+ * xyz: n_vertices*3 doubles; faces: n_faces*3 vertex ids, consistently oriented. */
+int gp_che_create(const double *xyz, uint32_t n_vertices, const uint32_t *faces,
+                  uint32_t n_faces, gp_che **out);
+
+/* che::~che (include/che.h:55; src/che.cpp:89-92,1404-1413). */
+void gp_che_destroy(gp_che *mesh);
+
+/* che::n_vertices/n_faces/n_half_edges/n_edges/is_manifold (include/che.h:80,93-96). */
+int gp_che_sizes(const gp_che *mesh, uint32_t *n_vertices, uint32_t *n_faces,
+                 uint32_t *n_half_edges, uint32_t *n_edges, int *manifold);
+
+/* che::vt / che::ot / che::evt accessors over the whole tables
+ * (include/che.h:81,89,91).  Any output may be NULL. */
+int gp_che_tables(const gp_che *mesh, uint32_t *vt, uint32_t *ot, uint32_t *evt);
+
+/* che::link (src/che.cpp:102-112): half-edge ids of the link of v. */
+int gp_che_link(const gp_che *mesh, uint32_t v, uint32_t *he_out, uint32_t cap, uint32_t *n);
+
+/* ---- toplesets ----------------------------------------------------------- */
+
+/* che::compute_toplesets(toplesets, sorted, limits, sources) (include/che.h:107;
+ * src/che.cpp:546-593), on the host.  toplesets[n_vertices], sorted[n_vertices]
+ * (unreached slots = GP_NIL), limits[cap_limits] with *n_limits = |limits|.
+ * Duplicate sources: GP_ERR_DUPLICATE_SOURCE. */
+int gp_compute_toplesets(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                         uint32_t *toplesets, uint32_t *sorted, uint32_t *limits,
+                         uint32_t cap_limits, uint32_t *n_limits);
+
+/* The same result computed on the GPU (frontier expansion + discovery-key
+ * ordering); bit-identical to gp_compute_toplesets. */
+int gp_compute_toplesets_gpu(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                             uint32_t *toplesets, uint32_t *sorted, uint32_t *limits,
+                             uint32_t cap_limits, uint32_t *n_limits);
+
+/* ---- PTP drivers (reference semantics) --------------------------------- */
+
+/* parallel_toplesets_propagation_gpu(ptp_out, mesh, sources, toplesets)
+ * (include/geodesics_ptp.h:36; src/cuda/geodesics_ptp.cu:20-85,174-299):
+ * the window schedule over sorted[] on the original numbering; writes
+ * dist[n_vertices] (INF where unreached) and returns the device seconds in
+ * *seconds (may be NULL).  clusters is reserved and must be NULL. */
+int gp_parallel_toplesets_propagation_gpu(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                                          const uint32_t *limits, uint32_t n_limits,
+                                          const uint32_t *sorted, double *dist,
+                                          uint32_t *clusters, double *seconds);
+
+/* parallel_toplesets_propagation_coalescence_gpu (include/geodesics_ptp.h:34;
+ * src/cuda/geodesics_ptp_coalescence.cu:21-160): vertices renumbered by rings
+ * (ptp_coalescence, src/geodesics_ptp.cpp:15-38) before the sweeps. */
+int gp_parallel_toplesets_propagation_coalescence_gpu(gp_che *mesh, const uint32_t *sources,
+                                                      uint32_t n_sources, const uint32_t *limits,
+                                                      uint32_t n_limits, const uint32_t *sorted,
+                                                      double *dist, uint32_t *clusters,
+                                                      double *seconds);
+
+/* parallel_toplesets_propagation_cpu (include/geodesics_ptp.h:40;
+ * src/geodesics_ptp.cpp:122-199): FP64 on the host (OpenMP), last-read buffer. */
+int gp_parallel_toplesets_propagation_cpu(const gp_che *mesh, const uint32_t *sources,
+                                          uint32_t n_sources, const uint32_t *limits,
+                                          uint32_t n_limits, const uint32_t *sorted, double *dist);
+
+/* parallel_toplesets_propagation_coalescence_cpu (include/geodesics_ptp.h:38;
+ * src/geodesics_ptp.cpp:40-120): the PTP_CPU path of the front end. */
+int gp_parallel_toplesets_propagation_coalescence_cpu(const gp_che *mesh, const uint32_t *sources,
+                                                      uint32_t n_sources, const uint32_t *limits,
+                                                      uint32_t n_limits, const uint32_t *sorted,
+                                                      double *dist);
+
+/* normalize_ptp (include/geodesics_ptp.h:46; src/geodesics_ptp.cpp:264-276). */
+int gp_normalize_ptp(double *dist, uint32_t n);
+
+/* ---- extended entry points (no single reference counterpart) ----------- */
+
+/* geodesics(mesh, sources, PTP_GPU, dist) front-end dispatch
+ * (src/geodesics.cpp:16-33,225-240) with explicit options: toplesets, then
+ * the sweeps.  sorted_out[n_vertices] (may be NULL) receives the BFS order
+ * the front end stores in sorted_index. */
+int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_ptp_opts *opts,
+           double *dist, uint32_t *sorted_out, gp_ptp_stats *stats);
+
+/* Device-resident solver plan: mesh tables, bands and buffers live in HBM;
+ * gp_plan_run() reruns the sweeps on them without host traffic (the timed
+ * region of the benchmarks). */
+typedef struct gp_plan gp_plan;
+int gp_plan_create(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
+                   const gp_ptp_opts *opts, gp_plan **out);
+int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats);
+int gp_plan_result(gp_plan *plan, double *dist);
+int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap_sweeps, uint32_t *n_sweeps);
+void gp_plan_destroy(gp_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPROSHAN_PTP_H */

@@ -1,0 +1,136 @@
+/*
+ * TEST INFRASTRUCTURE -- NOT PART OF THE PRODUCT.
+ * Oracle: update_step and the PTP window schedule in FP64 (reference default
+ * build) and FP32 (reference SINGLE_P build), the converged fixed point,
+ * compute_error and normalize_ptp.
+ *
+ * Restates src/geodesics_ptp.cpp:15-276, src/cuda/geodesics_ptp.cu:174-232,
+ * src/cuda/geodesics_ptp_coalescence.cu:102-160,
+ * src/cuda/test_geodesics_ptp_coalescence.cu:214-266 and
+ * src/test_geodesics_ptp.cpp:362-371.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define REAL double
+#define SFX(name) name##_f64
+#define SQRT sqrt
+#define FABS fabs
+#include "ptp_impl.h"
+#undef REAL
+#undef SFX
+#undef SQRT
+#undef FABS
+
+#define REAL float
+#define SFX(name) name##_f32
+#define SQRT sqrtf
+#define FABS fabsf
+#include "ptp_impl.h"
+#undef REAL
+#undef SFX
+#undef SQRT
+#undef FABS
+
+double or_update_step(const or_che *m, const double *dist, uint32_t he)
+{
+	mview_f64 mv = { m->nv, m->gt, m->vt, m->ot, m->evt };
+	return update_step_f64(&mv, dist, he);
+}
+
+float or_update_step_f32(const float *gt, const uint32_t *vt, const float *dist, uint32_t he)
+{
+	mview_f32 mv = { 0, gt, vt, NULL, NULL };
+	return update_step_f32(&mv, dist, he);
+}
+
+int or_ptp(const or_che *m, const uint32_t *src, uint32_t ns,
+           const uint32_t *limits, uint32_t nlimits, const uint32_t *sorted,
+           int variant, double tol, int n_threads,
+           double *dist, or_sweep *trace, uint32_t trace_cap, uint32_t *n_sweeps)
+{
+	return ptp_f64(m, src, ns, limits, nlimits, sorted, variant, tol, n_threads,
+	               dist, trace, trace_cap, n_sweeps);
+}
+
+int or_ptp_f32(const or_che *m, const uint32_t *src, uint32_t ns,
+               const uint32_t *limits, uint32_t nlimits, const uint32_t *sorted,
+               int variant, double tol, int n_threads,
+               float *dist, or_sweep *trace, uint32_t trace_cap, uint32_t *n_sweeps)
+{
+	return ptp_f32(m, src, ns, limits, nlimits, sorted, variant, tol, n_threads,
+	               dist, trace, trace_cap, n_sweeps);
+}
+
+/* FIFO label-correcting relaxation to the fixed point (see oracle.h).
+ * Pins: on flat meshes the fixed point is within the planar-update
+ * discretisation error of the Euclidean distance and below the reference
+ * schedule's result everywhere (tests/test_oracle_ptp.py). */
+int or_ptp_fixed_point(const or_che *m, const uint32_t *src, uint32_t ns,
+                       double rel_eps, double *dist, uint64_t *n_relax)
+{
+	const uint32_t nv = m->nv;
+	mview_f64 mv = { nv, m->gt, m->vt, m->ot, m->evt };
+	uint8_t *flag = (uint8_t *) calloc(nv, 1);      /* bit0 queued, bit1 source */
+	uint32_t *q = (uint32_t *) malloc(sizeof(uint32_t) * ((size_t) nv + 1));
+	if (!flag || !q) { free(flag); free(q); return OR_ENOMEM; }
+	for (uint32_t v = 0; v < nv; v++) dist[v] = INFINITY;
+	for (uint32_t s = 0; s < ns; s++) {
+		if (src[s] >= nv) { free(flag); free(q); return OR_EINVAL; }
+		dist[src[s]] = 0;
+		flag[src[s]] |= 2;
+	}
+	const size_t qn = (size_t) nv + 1;
+	size_t head = 0, tail = 0;
+	uint32_t lk[64];
+	for (uint32_t s = 0; s < ns; s++) {
+		const uint32_t n = or_link(m, src[s], lk, 64);
+		for (uint32_t k = 0; k < n && k < 64; k++) {
+			const uint32_t u = m->vt[lk[k]];
+			if (!flag[u]) { flag[u] |= 1; q[tail] = u; tail = (tail + 1) % qn; }
+		}
+	}
+	uint64_t relax = 0;
+	while (head != tail) {
+		const uint32_t v = q[head]; head = (head + 1) % qn;
+		flag[v] &= (uint8_t) ~1u;
+		const double nd = relax_f64(&mv, dist, v);
+		relax++;
+		const double od = dist[v];
+		if (nd < od && (od == INFINITY || od - nd > rel_eps * od)) {
+			dist[v] = nd;
+			const uint32_t n = or_link(m, v, lk, 64);
+			for (uint32_t k = 0; k < n && k < 64; k++) {
+				const uint32_t u = m->vt[lk[k]];
+				if (!flag[u]) { flag[u] |= 1; q[tail] = u; tail = (tail + 1) % qn; }
+			}
+		}
+	}
+	if (n_relax) *n_relax = relax;
+	free(flag); free(q);
+	return OR_OK;
+}
+
+double or_compute_error(const double *dist, const double *exact, uint32_t n, uint32_t s)
+{
+	double e = 0;
+	#pragma omp parallel for reduction(+: e)
+	for (int64_t v = 0; v < (int64_t) n; v++)
+		if (exact[v] > 0) e += fabs(dist[v] - exact[v]) / exact[v];
+	return e * 100 / (double) (n - s);
+}
+
+void or_normalize_ptp(double *dist, uint32_t n)
+{
+	double mx = 0;
+	#pragma omp parallel for reduction(max: mx)
+	for (int64_t v = 0; v < (int64_t) n; v++)
+		if (dist[v] < INFINITY && dist[v] > mx) mx = dist[v];
+	#pragma omp parallel for
+	for (int64_t v = 0; v < (int64_t) n; v++)
+		dist[v] /= mx;
+}

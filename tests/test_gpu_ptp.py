@@ -1,0 +1,80 @@
+"""GPU parity: the HIP path through the C ABI against the C oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from gproshan_amd import geodesics as P
+from gproshan_amd import meshgen as G
+from gproshan_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("ico5", lambda: G.icosphere(5, 0.01, 0), [0]),
+    ("hf128_4src", lambda: G.height_field(128, 1), [777, 5000, 9000, 16000]),
+    ("hf256", lambda: G.height_field(256, 2), [32896]),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request, oracle):
+    name, gen, src = request.param
+    xyz, f = gen()
+    print(f"\n[case] {name} V={len(xyz)} sources={src}", flush=True)
+    return name, xyz, f, np.array(src, np.uint32), oracle.Mesh(xyz, f), P.che(xyz, f)
+
+
+def test_device_is_gfx950():
+    import ctypes
+    buf = ctypes.create_string_buffer(64)
+    L.check(L.lib().gp_device_check(buf, 64))
+    assert buf.value.decode().startswith("gfx950")
+
+
+def test_toplesets_gpu_bit_exact(case):
+    name, xyz, f, src, M, C = case
+    ref = M.toplesets(src)
+    got = C.compute_toplesets(src, gpu=True)
+    for a, b, what in zip(got, ref, ("toplesets", "sorted", "limits")):
+        assert np.array_equal(a, b), what
+
+
+@pytest.mark.parametrize("coal", [True, False], ids=["coalesce", "plain"])
+@pytest.mark.parametrize("buf", [L.RESULT_LAST_WRITTEN, L.RESULT_LAST_READ], ids=["last_written", "last_read"])
+def test_f64_matches_oracle(case, oracle, coal, buf):
+    name, xyz, f, src, M, C = case
+    variant = (oracle.GPU_COAL if coal else oracle.GPU_PLAIN) if buf == L.RESULT_LAST_WRITTEN else \
+              (oracle.CPU_COAL if coal else oracle.CPU_PLAIN)
+    ref, nsw = M.ptp(src, variant)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F64, coalesce=coal, result_buffer=buf)
+    mx, mean = rel_err(d, ref)
+    print(f"[f64] {name} coal={coal} buf={buf} sweeps {st['sweeps']}/{nsw} max {mx:.3e} mean {mean:.3e} "
+          f"kernel {st['kernel_ms']:.2f} ms", flush=True)
+    assert st["sweeps"] == nsw
+    assert mx <= 1e-12
+
+
+def test_f32_matches_oracle(case, oracle):
+    name, xyz, f, src, M, C = case
+    ref, nsw, tr = M.ptp(src, oracle.GPU_COAL, trace=True)
+    plan = P.Plan(C, src, precision=L.PREC_F32)
+    st = plan.run()
+    d = plan.result()
+    gtr = plan.trace()
+    same_sched = len(gtr) == len(tr) and np.array_equal(gtr[:, 1:7], np.stack([tr[k] for k in ("i", "j", "start", "end", "ncond", "count")], 1))
+    mx, mean = rel_err(d, ref)
+    print(f"[f32] {name} sweeps {st['sweeps']}/{nsw} same schedule {same_sched} max {mx:.3e} mean {mean:.3e} "
+          f"f64 updates {st['f64_updates']} kernel {st['kernel_ms']:.2f} ms", flush=True)
+    if same_sched:
+        assert mx <= 1e-5
+    assert mx <= 1e-4 or not same_sched
+
+
+def test_frontend_ptp_gpu_dispatch(case, oracle):
+    name, xyz, f, src, M, C = case
+    g = P.geodesics(C, src, P.geodesics.PTP_GPU, precision=L.PREC_F64)
+    ref, _ = M.ptp(src, oracle.GPU_COAL if len(src) == 1 else oracle.GPU_PLAIN)
+    mx, _ = rel_err(g.dist, ref)
+    assert mx <= 1e-12
+    _, srt, _ = M.toplesets(src)
+    assert all(g(i) == srt[i] for i in range(0, len(srt), 101))
