@@ -21,6 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DUPLICATE_SOURCE, ERR_HIP, ERR_NO_DEVICE, ERR_CA
 PREC_F32, PREC_F64 = 0, 1
 RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
 SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND = 0, 1, 2
+LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
 
 # Every symbol include/gproshan_ptp.h declares.
 EXPORTS = [
@@ -43,7 +44,7 @@ class GPError(RuntimeError):
 class PtpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
-                ("block", C.c_uint32), ("grid", C.c_uint32)]
+                ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int)]
 
 
 class PtpStats(C.Structure):

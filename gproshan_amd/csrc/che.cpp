@@ -172,43 +172,81 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	return GP_OK;
 }
 
-void build_geo32(const Che &m, std::vector<uint32_t> &fan, std::vector<Geo32> &geo, uint32_t &flagged)
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, Layout &L)
 {
 	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.5;
-	const uint32_t nv = m.nv, nfan = m.fan_off[nv];
-	fan = m.fan;
-	geo.assign(std::max<uint32_t>(nfan, 1), Geo32{0.f, 0.f, 0.f, 0.f});
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
-	uint32_t nflag = 0;
-	#pragma omp parallel for schedule(static) reduction(+: nflag)
-	for (int64_t v = 0; v < (int64_t) nv; v++) {
-		const uint32_t s = m.fan_off[v], e = m.fan_off[v + 1];
-		const double *pv = &m.gt[3 * (size_t) v];
-		for (uint32_t k = s; k < e; k++) {
-			const double *pa = &m.gt[3 * (size_t) m.fan[k]];
-			const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
-			const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
-			Geo32 gk{ (float) (q0 * ih2), 0.f, 0.f, 0.f };
-			if (k + 1 < e) {
-				const double *pb = &m.gt[3 * (size_t) m.fan[k + 1]];
-				const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
-				const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
-				const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
-				const double cx = X0[1] * X1[2] - X0[2] * X1[1];
-				const double cy = X0[2] * X1[0] - X0[0] * X1[2];
-				const double cz = X0[0] * X1[1] - X0[1] * X1[0];
-				const double det = cx * cx + cy * cy + cz * cz;
-				const double E[3] = { pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2] };
-				gk.y = (float) (g * ih2);
-				gk.z = (float) (det * ih2 * ih2);
-				gk.w = (float) ((E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2);
-				const double qq = q0 * q1;
-				if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { fan[k] |= F64_FLAG; nflag++; }
-			}
-			geo[k] = gk;
+	L.cid.clear(); L.cgeo.clear(); L.cgeo64.clear(); L.meta.clear();
+	L.pchunk.assign(nr, 0);
+	L.fcum.assign((size_t) nr + 1, 0);
+	L.max_seg = 1; L.n_flagged = 0;
+	uint64_t head = 0;
+	uint32_t used = 0, vbase = 0;
+	auto close_chunk = [&]() {
+		for (uint32_t l = used; l < 64; l++) {
+			L.cid.push_back(0);
+			L.cgeo.push_back(Geo32{0.f, 0.f, 0.f, 0.f});
+			L.cgeo64.push_back(Geo64{0, 0, 0, 0});
 		}
+		L.meta.push_back(ChunkMeta{ (uint32_t) head, (uint32_t) (head >> 32), vbase, used });
+		head = 0; used = 0;
+	};
+	for (uint32_t pos = 0; pos < nr; pos++) {
+		const uint32_t v = order ? order[pos] : pos;
+		if (v >= m.nv) return fail(GP_ERR_INVALID, "layout: vertex %u out of range", v);
+		const uint32_t s = m.fan_off[v], e = m.fan_off[v + 1];
+		const uint32_t lanes = e > s ? e - s : 1;
+		if (lanes > 64) return fail(GP_ERR_CAPACITY, "layout: vertex %u has %u fan entries (max 64)", v, lanes);
+		if (used + lanes > 64) close_chunk();
+		if (used == 0) vbase = pos;
+		head |= 1ull << used;
+		L.pchunk[pos] = (uint32_t) L.meta.size();
+		L.max_seg = std::max(L.max_seg, lanes);
+		if (e == s) {
+			L.cid.push_back(v);
+			L.cgeo.push_back(Geo32{0.f, 0.f, 0.f, 0.f});
+			L.cgeo64.push_back(Geo64{0, 0, 0, 0});
+		} else {
+			const double *pv = &m.gt[3 * (size_t) v];
+			for (uint32_t k = s; k < e; k++) {
+				const double *pa = &m.gt[3 * (size_t) m.fan[k]];
+				const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+				const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+				Geo32 gk{ (float) (q0 * ih2), 0.f, 0.f, 0.f };
+				Geo64 gd{ q0 * ih2, 0, 0, 0 };
+				uint32_t id = m.fan[k];
+				if (k + 1 < e) {
+					const double *pb = &m.gt[3 * (size_t) m.fan[k + 1]];
+					const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
+					const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
+					const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
+					const double cx = X0[1] * X1[2] - X0[2] * X1[1];
+					const double cy = X0[2] * X1[0] - X0[0] * X1[2];
+					const double cz = X0[0] * X1[1] - X0[1] * X1[0];
+					const double det = cx * cx + cy * cy + cz * cz;
+					const double E[3] = { pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2] };
+					gd.y = g * ih2;
+					gd.z = det * ih2 * ih2;
+					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
+					gk.y = (float) gd.y;
+					gk.z = (float) gd.z;
+					gk.w = (float) gd.w;
+					const double qq = q0 * q1;
+					if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; L.n_flagged++; }
+				}
+				L.cid.push_back(id);
+				L.cgeo.push_back(gk);
+				L.cgeo64.push_back(gd);
+			}
+		}
+		used += lanes;
+		L.fcum[pos + 1] = L.n_flagged;
 	}
-	flagged = nflag;
+	if (used) close_chunk();
+	L.nchunks = (uint32_t) L.meta.size();
+	L.red_steps = 0;
+	while ((1u << L.red_steps) < L.max_seg) L.red_steps++;
+	return GP_OK;
 }
 
 int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels,

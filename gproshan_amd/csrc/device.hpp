@@ -17,16 +17,15 @@ namespace gp {
 static_assert(sizeof(Geo32) == sizeof(float4), "Geo32 must match float4");
 
 struct DeviceMesh {
-	uint32_t nv = 0, nfan = 0, n_flagged = 0;
+	uint32_t nv = 0, nfan = 0;
 	uint32_t *fan_off = nullptr;
-	uint32_t *fan = nullptr;   // ids; bit 31 marks triangle (k, k+1) for the FP64 update
-	float4 *geo = nullptr;     // per entry: q0, g, det, S in units of h
+	uint32_t *fan = nullptr;   // fan lists (vertex ids)
 	double *pos = nullptr;     // 3*nv
 	uint8_t *closed = nullptr; // 1 when the fan of v is closed (link = first m entries)
 	float h = 1.f, inv_h = 1.f;
 };
 
-// Uploads the fan tables and FP32 geometry of m once (m.dev caches them).
+// Uploads the fan tables and FP64 positions of m once (m.dev caches them).
 int upload_mesh(Che &m);
 
 

@@ -58,11 +58,30 @@ constexpr uint32_t F64_FLAG = 0x80000000u;
 // x = |X_k|^2, y = X_k.X_{k+1}, z = |X_k x X_{k+1}|^2, w = |X_{k+1} - X_k|^2
 // (y, z, w describe triangle (k, k+1) and are 0 on the last entry).
 struct Geo32 { float x, y, z, w; };
+struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
 
-// Fan ids with bit 31 set on entry k when triangle (k, k+1) takes the FP64
-// update: its angle at the updated vertex is below ~10 deg or above ~170 deg
+// FP64 update for triangle (k, k+1) of a fan when bit 31 of entry k is set:
+// its angle at the updated vertex is below ~10 deg or above ~170 deg
 // (sin^2 < 0.03: near-degenerate Gram matrix) or above 120 deg (obtuse).
-void build_geo32(const Che &m, std::vector<uint32_t> &fan, std::vector<Geo32> &geo, uint32_t &flagged);
+
+// Lane layout of a solve.  Vertices, in ring order (order[pos], or pos when
+// the mesh is numbered by rings), are packed whole into 64-lane chunks, one
+// lane per fan entry (one lane for a vertex without a fan).  Lane l of chunk
+// c holds cid[64c+l] = fan entry id (bit 31: FP64 update for the triangle
+// (l, l+1)) and cgeo[64c+l].  meta[c]: bit mask of lanes that start a vertex,
+// position of the chunk's first vertex, lanes used.  pchunk[pos] = chunk of
+// position pos.  red_steps = ceil(log2(longest fan)).
+struct ChunkMeta { uint32_t head_lo, head_hi, vbase, used; };
+struct Layout {
+	uint32_t nchunks = 0, max_seg = 1, red_steps = 0, n_flagged = 0;
+	std::vector<uint32_t> cid;
+	std::vector<Geo32> cgeo;
+	std::vector<Geo64> cgeo64;
+	std::vector<ChunkMeta> meta;
+	std::vector<uint32_t> pchunk;
+	std::vector<unsigned long long> fcum;  // FP64-flagged triangles of positions [0, pos)
+};
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, Layout &L);
 
 // Builds VT/OT/EVT per the reference's rules and the fan tables.
 int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf, Che &m);

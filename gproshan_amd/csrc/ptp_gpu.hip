@@ -1,17 +1,31 @@
-// gfx950 parallel toplesets propagation: device mesh tables, the window
-// sweep kernels and the device-resident solver plan.
+// gfx950 parallel toplesets propagation: device mesh tables, the window sweep
+// kernels and the device-resident solver plan.
 //
 // Schedule (src/geodesics_ptp.cpp:61-109; src/cuda/geodesics_ptp.cu:198-231):
 // rings [i, j) form the window, i is raised to j/2 when it lags (K/2 band
 // limit), ring i retires when every vertex in it changed by less than tol
 // (relative), j grows by one ring per sweep, the double buffer flips each
-// sweep, and at most 2*|limits| sweeps run.  Unlike the reference, which
-// launches a relax kernel, a relative-error kernel and a Thrust count per
-// sweep with three host synchronisations, the schedule state lives in device
-// memory: each sweep kernel reads it, relaxes its window, counts
-// non-converged ring-i vertices with workgroup reductions and one atomic per
-// workgroup, and the last workgroup to finish advances the schedule.  The
-// host only polls a "done" flag every few dozen sweeps.
+// sweep, and at most 2*|limits| sweeps run.
+//
+// The reference launches a relax kernel, a relative-error kernel and a Thrust
+// count per sweep with three host synchronisations.  Here the schedule lives
+// in device memory: a sweep kernel relaxes the window it reads from there and
+// counts non-converged ring-i vertices (wave reductions, one atomic per wave
+// with a non-zero count), and a one-thread kernel advances the schedule.  The
+// pairs are captured once into a hipGraph and replayed until the device sets
+// its done flag; the host polls that flag once per replay.  (A persistent
+// kernel with a grid barrier per sweep measured slower on MI355X: the
+// agent-scope fence each workgroup needs to publish its writes across the
+// eight XCD L2s costs ~36 ns per workgroup, ~19 us per barrier at 512
+// workgroups, against ~1.6 us per graph-launched kernel.)
+//
+// Work layout (see Layout in gp_internal.hpp): one lane per fan entry,
+// vertices packed whole into wave64 chunks in ring order.  A lane gathers the
+// old distance of its fan vertex, takes the next lane's distance with a
+// shuffle (the triangle (k, k+1) of the fan), runs the triangle update, and a
+// segmented shuffle-min leaves each vertex's minimum in its first lane, which
+// writes the new distance and tests ring-i convergence.  Each XCD takes one
+// contiguous eighth of the window's chunks so neighbouring gathers share its L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +45,6 @@ void device_mesh_free(DeviceMesh *d)
 	if (!d) return;
 	(void) hipFree(d->fan_off);
 	(void) hipFree(d->fan);
-	(void) hipFree(d->geo);
 	(void) hipFree(d->pos);
 	(void) hipFree(d->closed);
 	delete d;
@@ -53,31 +66,24 @@ int device_check(char *name, size_t cap)
 	return GP_OK;
 }
 
-// Host precompute of the FP32 triangle geometry and the FP64 flags, then upload.
+// Uploads the fan tables (ring builder) and FP64 positions once per mesh.
 int upload_mesh(Che &m)
 {
 	if (m.dev) return GP_OK;
 	int rc = device_check(nullptr, 0);
 	if (rc) return rc;
 	const uint32_t nv = m.nv, nfan = m.fan_off[nv];
-	std::vector<uint32_t> fan;
-	std::vector<Geo32> geo;
-	uint32_t flagged = 0;
-	build_geo32(m, fan, geo, flagged);
-	const double h = m.mean_edge, ih = 1.0 / h;
 	auto dm = std::make_unique<DeviceMesh>();
-	dm->nv = nv; dm->nfan = nfan; dm->n_flagged = flagged;
-	dm->h = (float) h; dm->inv_h = (float) ih;
+	dm->nv = nv; dm->nfan = nfan;
+	dm->h = (float) m.mean_edge; dm->inv_h = (float) (1.0 / m.mean_edge);
 	auto cleanup = [&](int r) { device_mesh_free(dm.release()); return r; };
 	if (hipMalloc(&dm->fan_off, sizeof(uint32_t) * (nv + 1)) != hipSuccess ||
 	    hipMalloc(&dm->fan, sizeof(uint32_t) * std::max<uint32_t>(nfan, 1)) != hipSuccess ||
-	    hipMalloc(&dm->geo, sizeof(float4) * std::max<uint32_t>(nfan, 1)) != hipSuccess ||
 	    hipMalloc(&dm->pos, sizeof(double) * 3 * (size_t) nv) != hipSuccess ||
 	    hipMalloc(&dm->closed, nv) != hipSuccess)
 		return cleanup(fail(GP_ERR_NOMEM, "device mesh allocation failed"));
 	if (hipMemcpy(dm->fan_off, m.fan_off.data(), sizeof(uint32_t) * (nv + 1), hipMemcpyHostToDevice) != hipSuccess ||
-	    (nfan && hipMemcpy(dm->fan, fan.data(), sizeof(uint32_t) * nfan, hipMemcpyHostToDevice) != hipSuccess) ||
-	    (nfan && hipMemcpy(dm->geo, geo.data(), sizeof(Geo32) * nfan, hipMemcpyHostToDevice) != hipSuccess) ||
+	    (nfan && hipMemcpy(dm->fan, m.fan.data(), sizeof(uint32_t) * nfan, hipMemcpyHostToDevice) != hipSuccess) ||
 	    hipMemcpy(dm->pos, m.gt.data(), sizeof(double) * 3 * (size_t) nv, hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(dm->closed, m.fan_closed.data(), nv, hipMemcpyHostToDevice) != hipSuccess)
 		return cleanup(fail(GP_ERR_HIP, "device mesh upload failed"));
@@ -88,137 +94,188 @@ int upload_mesh(Che &m)
 // ---------------------------------------------------------------------------
 // Kernels
 
-struct State {
-	uint32_t i, j, d, iter, done, bad, arrived, sweeps;
+// Schedule state in device memory.  Sweep kernel k of a run (k counted from
+// 0) reads the state of sweep k-1 from slot[(k-1) & 3] together with its
+// non-converged count bad[(k-1) & 3], advances it (i, j, d as the reference
+// loop does), publishes it in slot[k & 3] (workgroup 0), zeroes bad[(k+1) & 3]
+// for the next sweep, and relaxes its own window counting into bad[k & 3].
+// No workgroup reads a slot or counter while another workgroup of the same
+// launch writes it, so kernel boundaries are the only synchronisation.  The
+// captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
+struct Slot {
+	uint32_t i, j, d, iter, done, fresh, pad[2];
+};
+struct PState {
+	Slot slot[4];
+	uint32_t bad[4];
+	uint32_t done, sweeps, final_d, pad;
 	unsigned long long relax, cond, f64;
 };
 
-struct SweepArgs {
-	State *st;
-	const uint32_t *limits;
-	const uint32_t *order;     // ring order; null when the mesh is numbered by rings
-	const uint32_t *fan_off;
-	const uint32_t *fan;
-	const float4 *geo;
+struct KArgs {
+	const uint4 *rinfo;        // per ring r: limits[r], first chunk of ring r, end chunk of rings < r
+	const uint32_t *cid;
+	const float4 *cgeo;
+	const double4 *cgeo64;
+	const uint4 *meta;
+	const unsigned long long *fcum;  // FP64-update triangles of positions [0, pos)
+	const uint32_t *order;     // position -> vertex; null when the mesh is numbered by rings
 	const double *pos;
 	void *dist0, *dist1;
+	PState *st;
 	uint32_t *trace;           // 8 words per sweep, or null
 	double tol;
 	float h, inv_h;
-	uint32_t nl, max_iter;
+	uint32_t nl, max_iter, red_steps;
 };
 
-__device__ __forceinline__ float relax_f32(const SweepArgs &a, const float *od, uint32_t v, float best,
-                                           uint32_t &f64n)
-{
-	const uint32_t s = a.fan_off[v], e = a.fan_off[v + 1];
-	if (e - s < 2) return best;
-	uint32_t ida = a.fan[s];
-	uint32_t va = ida & ID_MASK;
-	float ta = od[va];
-	float4 ga = a.geo[s];
-	for (uint32_t k = s + 1; k < e; k++) {
-		const uint32_t idb = a.fan[k];
-		const uint32_t vb = idb & ID_MASK;
-		const float tb = od[vb];
-		const float4 gb = a.geo[k];
-		float p;
-		if (ida & F64_FLAG) {
-			p = (float) update_ref_f64(a.pos + 3 * (size_t) v, a.pos + 3 * (size_t) va, a.pos + 3 * (size_t) vb,
-			                           (double) ta, (double) tb);
-			f64n++;
-		} else {
-			p = update_stable_f32(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
-		}
-		if (p < best) best = p;
-		ida = idb; va = vb; ta = tb; ga = gb;
-	}
-	return best;
-}
-
-__device__ __forceinline__ double relax_f64(const SweepArgs &a, const double *od, uint32_t v, double best)
-{
-	const uint32_t s = a.fan_off[v], e = a.fan_off[v + 1];
-	if (e - s < 2) return best;
-	const double *gv = a.pos + 3 * (size_t) v;
-	uint32_t va = a.fan[s] & ID_MASK;
-	double ta = od[va];
-	for (uint32_t k = s + 1; k < e; k++) {
-		const uint32_t vb = a.fan[k] & ID_MASK;
-		const double tb = od[vb];
-		const double p = update_ref_f64(gv, a.pos + 3 * (size_t) va, a.pos + 3 * (size_t) vb, ta, tb);
-		if (p < best) best = p;
-		va = vb; ta = tb;
-	}
-	return best;
-}
-
+// Relaxes the window [start, end) (positions); returns this thread's count of
+// non-converged vertices among positions [start, cend).
 template <bool F64>
-__global__ __launch_bounds__(256) void k_sweep(SweepArgs a)
+__device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
+                                               uint32_t c0, uint32_t c1,
+                                               const typename std::conditional<F64, double, float>::type *od,
+                                               typename std::conditional<F64, double, float>::type *nd)
 {
 	using T = typename std::conditional<F64, double, float>::type;
-	State *st = a.st;
-	if (st->done) return;
-	uint32_t i = st->i;
-	const uint32_t j = st->j, d = st->d;
-	if (i < (j >> 1)) i = j >> 1;                   // K/2 limit band size
-	const uint32_t start = a.limits[i], end = a.limits[j], cend = a.limits[i + 1];
-	const T *od = (const T *) (d ? a.dist1 : a.dist0);
-	T *nd = (T *) (d ? a.dist0 : a.dist1);
-
-	uint32_t bad = 0, f64n = 0;
-	const uint32_t stride = gridDim.x * blockDim.x;
-	for (uint32_t k = start + blockIdx.x * blockDim.x + threadIdx.x; k < end; k += stride) {
-		const uint32_t v = a.order ? a.order[k] : k;
-		const T old = od[v];
-		T best;
-		if constexpr (F64) best = relax_f64(a, od, v, old);
-		else best = relax_f32(a, od, v, old, f64n);
-		nd[v] = best;
-		if (k < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+	if (start >= end || c0 >= c1) return 0;
+	const uint32_t lane = threadIdx.x & 63u;
+	// XCD-aware split: workgroups b with b % 8 == x run on XCD x and take the
+	// x-th contiguous eighth of the window's chunks.
+	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
+	const uint32_t x = blockIdx.x & 7u;
+	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
+	const uint32_t wpb = blockDim.x >> 6;
+	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
+	const uint32_t nw_x = nbx * wpb;
+	const uint32_t len = c1 - c0;
+	const uint32_t part = (len + ngrp - 1) / ngrp;
+	const uint32_t lo = c0 + x * part;
+	const uint32_t hi = min(c1, lo + part);
+	const unsigned long long upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
+	uint32_t bad = 0;
+	for (uint32_t c = lo + lw; c < hi; c += nw_x) {
+		const uint4 mt = a.meta[c];
+		const unsigned long long head = ((unsigned long long) mt.y << 32) | mt.x;
+		const uint32_t used = mt.w;
+		const uint32_t id = a.cid[64u * c + lane];
+		const uint32_t nb = id & ID_MASK;
+		const T t = od[nb];
+		const T t1 = __shfl_down(t, 1);
+		const uint32_t nb1 = __shfl_down(nb, 1);
+		const unsigned long long above = head & ~upto;
+		const uint32_t seg_end = above ? (uint32_t) __builtin_ctzll(above) : used;
+		const uint32_t vpos = mt.z + (uint32_t) __builtin_popcountll(head & upto) - 1u;
+		const bool is_head = ((head >> lane) & 1ull) && lane < used;
+		const bool has_tri = lane + 1u < seg_end;
+		T p = (T) INFINITY;
+		if constexpr (F64) {
+			if (has_tri) {
+				const uint32_t v = a.order ? a.order[vpos] : vpos;
+				p = update_ref_f64(a.pos + 3 * (size_t) v, a.pos + 3 * (size_t) nb, a.pos + 3 * (size_t) nb1, t, t1);
+			}
+		} else {
+			const float4 g = a.cgeo[64u * c + lane];
+			const float q1 = __shfl_down(g.x, 1);
+			if (has_tri) {
+				if (id & F64_FLAG) {
+					// has_tri implies lane + 1 < 64: the next entry is in this chunk.
+					const double4 gd = a.cgeo64[64u * c + lane];
+					const double q1d = a.cgeo64[64u * c + lane + 1u].x;
+					p = (float) update_stable<double>(gd.x, q1d, gd.y, gd.z, gd.w, (double) t, (double) t1,
+					                                  (double) a.h, (double) a.inv_h);
+				} else {
+					p = update_stable<float>(g.x, q1, g.y, g.z, g.w, t, t1, a.h, a.inv_h);
+				}
+			}
+		}
+		// Segmented minimum over the vertex's lanes; NaN never wins.
+		for (uint32_t s = 0; s < a.red_steps; s++) {
+			const uint32_t off = 1u << s;
+			const T q = __shfl_down(p, off);
+			if (lane + off < seg_end && q < p) p = q;
+		}
+		if (is_head && vpos >= start && vpos < end) {
+			const uint32_t v = a.order ? a.order[vpos] : vpos;
+			const T old = od[v];
+			const T best = (p < old) ? p : old;
+			nd[v] = best;
+			if (vpos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+		}
 	}
+	return bad;
+}
 
-	__shared__ uint32_t s_bad, s_f64;
-	if (threadIdx.x == 0) { s_bad = 0; s_f64 = 0; }
-	__syncthreads();
-	if (bad) atomicAdd(&s_bad, bad);
-	if (f64n) atomicAdd(&s_f64, f64n);
-	__syncthreads();
-	if (threadIdx.x != 0) return;
-	if (s_bad) atomicAdd(&st->bad, s_bad);
-	if (s_f64) atomicAdd(&st->f64, (unsigned long long) s_f64);
-	__threadfence();
-	const uint32_t prev = atomicAdd(&st->arrived, 1u);
-	if (prev != gridDim.x - 1) return;
-	// Last workgroup of this sweep: advance the schedule.
-	__threadfence();
-	const uint32_t nbad = atomicAdd(&st->bad, 0u);
-	const uint32_t ncond = cend - start;
-	if (a.trace) {
-		uint32_t *r = a.trace + 8 * (size_t) st->sweeps;
-		r[0] = st->sweeps + 1; r[1] = i; r[2] = j; r[3] = start;
-		r[4] = end; r[5] = ncond; r[6] = ncond - nbad; r[7] = 0;
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+	for (uint32_t o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+	return x;
+}
+
+// Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
+template <bool F64>
+__global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
+{
+	using T = typename std::conditional<F64, double, float>::type;
+	PState *st = a.st;
+	const uint32_t kp = (k4 + 3u) & 3u;
+	const Slot prev = st->slot[kp];
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if (prev.done) {
+		// Later launches of the replay read slot k & 3 in turn: keep it done.
+		if (lead) st->slot[k4] = prev;
+		return;
 	}
-	st->sweeps += 1;
-	st->relax += end - start;
-	st->cond += ncond;
-	const uint32_t ni = i + (nbad == 0 ? 1u : 0u);
-	const uint32_t nj = j + (j < a.nl - 1 ? 1u : 0u);
-	const uint32_t it = st->iter + 1;
-	st->i = ni; st->j = nj; st->d = d ^ 1u; st->iter = it;
-	st->done = !(ni < nj) || !(it < a.max_iter);
-	st->bad = 0;
-	st->arrived = 0;
+	Slot cur = prev;
+	cur.fresh = 0;
+	if (!prev.fresh) {
+		// The previous sweep's outcome: the reference loop tail.
+		const uint32_t nbad = st->bad[kp];
+		const uint32_t pi = prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
+		cur.i = pi + (nbad == 0 ? 1u : 0u);
+		cur.j = prev.j + (prev.j < a.nl - 1 ? 1u : 0u);
+		cur.d = prev.d ^ 1u;
+		cur.iter = prev.iter + 1;
+		cur.done = !(cur.i < cur.j) || !(cur.iter < a.max_iter);
+		if (lead) {
+			const uint4 r0 = a.rinfo[pi], r1 = a.rinfo[pi + 1], rj = a.rinfo[prev.j];
+			const uint32_t ncond = r1.x - r0.x;
+			if (a.trace) {
+				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
+				r[0] = prev.iter + 1; r[1] = pi; r[2] = prev.j; r[3] = r0.x; r[4] = rj.x; r[5] = ncond;
+				r[6] = ncond - nbad; r[7] = 0;
+			}
+			st->relax += rj.x - r0.x;
+			st->cond += ncond;
+			if (a.fcum) st->f64 += a.fcum[rj.x] - a.fcum[r0.x];
+			st->sweeps = cur.iter;
+		}
+	}
+	if (lead) {
+		st->slot[k4] = cur;
+		st->bad[(k4 + 1u) & 3u] = 0;
+		if (cur.done) { st->final_d = cur.d; st->done = 1; }
+	}
+	if (cur.done) return;
+	const uint32_t i = cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
+	const uint4 ri = a.rinfo[i], ri1 = a.rinfo[i + 1], rj = a.rinfo[cur.j];
+	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
+	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
+	const uint32_t bad = wave_sum(sweep_body<F64>(a, ri.x, rj.x, ri1.x, ri.y, rj.z, od, nd));
+	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 }
 
 template <typename T>
-__global__ void k_init(T *d0, T *d1, uint32_t n, const uint32_t *src, uint32_t ns, State *st, uint32_t done)
+__global__ void k_init(T *d0, T *d1, uint32_t n, PState *st, uint32_t done)
 {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	for (uint32_t v = t; v < n; v += gridDim.x * blockDim.x) { d0[v] = (T) INFINITY; d1[v] = (T) INFINITY; }
 	if (t == 0) {
-		st->i = 1; st->j = 2; st->d = 0; st->iter = 0; st->done = done;
-		st->bad = 0; st->arrived = 0; st->sweeps = 0; st->relax = 0; st->cond = 0; st->f64 = 0;
+		uint32_t *sw = (uint32_t *) st;
+		for (uint32_t q = 0; q < sizeof(PState) / 4; q++) sw[q] = 0;
+		// The state "before" sweep 0 lives in slot 3 and is taken as is.
+		st->slot[3].i = 1; st->slot[3].j = 2; st->slot[3].fresh = 1; st->slot[3].done = done;
+		st->done = done;
 	}
 }
 
@@ -247,19 +304,30 @@ struct gp_plan {
 	std::vector<uint32_t> inv;               // original -> ring numbering
 	gp::DeviceMesh *dm = nullptr;
 	gp_ptp_opts opts{};
-	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, max_iter = 0, grid = 0, block = 256;
-	uint32_t *d_limits = nullptr, *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
+	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
+	uint32_t nchunks = 0, red_steps = 0, n_flagged = 0;
+	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
+	uint4 *d_rinfo = nullptr;
+	uint32_t *d_cid = nullptr;
+	unsigned long long *d_fcum = nullptr;
+	float4 *d_cgeo = nullptr;
+	double4 *d_cgeo64 = nullptr;
+	uint4 *d_meta = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
 	double *d_out = nullptr;
-	gp::State *d_state = nullptr, *h_state = nullptr;
+	gp::PState *d_state = nullptr, *h_state = nullptr;
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	bool has_run = false;
 
 	~gp_plan()
 	{
-		(void) hipFree(d_limits); (void) hipFree(d_order); (void) hipFree(d_src); (void) hipFree(d_trace);
-		(void) hipFree(d_dist0); (void) hipFree(d_dist1); (void) hipFree(d_out); (void) hipFree(d_state);
+		for (void *p : { (void *) d_rinfo, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_meta, d_dist0, d_dist1,
+		                 (void *) d_out, (void *) d_state, (void *) d_fcum })
+			(void) hipFree(p);
+		if (g_big) (void) hipGraphExecDestroy(g_big);
+		if (g_small) (void) hipGraphExecDestroy(g_small);
 		if (h_state) (void) hipHostFree(h_state);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
@@ -269,6 +337,15 @@ struct gp_plan {
 
 namespace gp {
 
+template <typename V>
+static int upload(V *&dst, const std::vector<typename std::remove_const<V>::type> &src)
+{
+	const size_t bytes = sizeof(V) * std::max<size_t>(src.size(), 1);
+	if (hipMalloc(&dst, bytes) != hipSuccess) return fail(GP_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
+	if (!src.empty()) GP_HIP(hipMemcpy(dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+	return GP_OK;
+}
+
 // Builds a plan from given rings (limits/sorted in the original numbering).
 int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_t *limits, uint32_t nl,
                       const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out)
@@ -276,12 +353,6 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	*out = nullptr;
 	if (!mesh || !src || !limits || !sorted) return fail(GP_ERR_INVALID, "plan: null argument");
 	if (ns == 0 || nl < 2) return fail(GP_ERR_INVALID, "plan: no sources / empty rings");
-	auto p = std::make_unique<gp_plan>();
-	p->mesh = mesh;
-	p->opts = opts;
-	p->nv_orig = mesh->nv;
-	p->nl = nl;
-	p->ns = ns;
 	const uint32_t nr = limits[nl - 1];
 	if (nr > mesh->nv) return fail(GP_ERR_INVALID, "plan: limits exceed the vertex count");
 	// The kernels index with these values: check them all on the host.
@@ -292,6 +363,13 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		if (sorted[q] >= mesh->nv) return fail(GP_ERR_INVALID, "plan: sorted[%u] out of range", q);
 	for (uint32_t q = 0; q < ns; q++)
 		if (src[q] >= mesh->nv) return fail(GP_ERR_INVALID, "plan: source %u out of range", src[q]);
+	auto p = std::make_unique<gp_plan>();
+	p->mesh = mesh;
+	p->opts = opts;
+	p->nv_orig = mesh->nv;
+	p->nl = nl;
+	p->ns = ns;
+	p->nr = nr;
 	std::vector<uint32_t> srcm(src, src + ns);
 	Che *use = mesh;
 	if (opts.coalesce) {
@@ -299,7 +377,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		int rc = coalesce(*mesh, sorted, nr, *p->coal, p->inv);
 		if (rc) return rc;
 		for (auto &s : srcm) {
-			if (s >= mesh->nv || p->inv[s] == NIL) return fail(GP_ERR_INVALID, "plan: source not in ring 0");
+			if (p->inv[s] == NIL) return fail(GP_ERR_INVALID, "plan: source not in ring 0");
 			s = p->inv[s];
 		}
 		use = p->coal.get();
@@ -308,30 +386,88 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (rc) return rc;
 	p->dm = use->dev;
 	p->n = use->nv;
+	Layout L;
+	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, L);
+	if (rc) return rc;
+	p->nchunks = L.nchunks;
+	p->red_steps = L.red_steps;
+	p->n_flagged = L.n_flagged;
 	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (nl << 1);
 	p->block = 256;
-	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));
 	const bool f64 = opts.precision == GP_PREC_F64;
+	// One wave per chunk up to full occupancy (8 workgroups of 4 waves per CU).
+	int dev = 0, ncu = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	const uint32_t gmax = (uint32_t) ncu * 8u;
+	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
+
 	const size_t es = f64 ? sizeof(double) : sizeof(float);
 	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 	GP_HIP(hipEventCreate(&p->e0));
 	GP_HIP(hipEventCreate(&p->e1));
-	if (hipMalloc(&p->d_limits, sizeof(uint32_t) * nl) != hipSuccess ||
-	    hipMalloc(&p->d_src, sizeof(uint32_t) * ns) != hipSuccess ||
-	    hipMalloc(&p->d_dist0, es * p->n) != hipSuccess ||
-	    hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
-	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
-	    hipMalloc(&p->d_state, sizeof(State)) != hipSuccess ||
-	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) p->max_iter) != hipSuccess ||
-	    hipHostMalloc(&p->h_state, sizeof(State)) != hipSuccess)
-		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
-	GP_HIP(hipMemcpy(p->d_limits, limits, sizeof(uint32_t) * nl, hipMemcpyHostToDevice));
-	GP_HIP(hipMemcpy(p->d_src, srcm.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
-	if (!opts.coalesce) {
-		GP_HIP(hipMalloc(&p->d_order, sizeof(uint32_t) * std::max<uint32_t>(nr, 1)));
-		GP_HIP(hipMemcpy(p->d_order, sorted, sizeof(uint32_t) * nr, hipMemcpyHostToDevice));
+	std::vector<uint4> rinfo(nl);
+	for (uint32_t r = 0; r < nl; r++) {
+		const uint32_t lr = limits[r];
+		rinfo[r] = make_uint4(lr, lr < nr ? L.pchunk[lr] : L.nchunks, lr > 0 ? L.pchunk[lr - 1] + 1 : 0, 0);
 	}
+	if ((rc = upload(p->d_rinfo, rinfo)) || (rc = upload(p->d_src, srcm)) || (rc = upload(p->d_cid, L.cid)))
+		return rc;
+	std::vector<float4> cg(L.cgeo.size());
+	std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+	std::vector<uint4> mt(L.meta.size());
+	std::memcpy(mt.data(), L.meta.data(), sizeof(uint4) * mt.size());
+	if ((rc = upload(p->d_cgeo, cg)) || (rc = upload(p->d_meta, mt))) return rc;
+	if (!f64) {
+		std::vector<double4> cg64(L.cgeo64.size());
+		std::memcpy(cg64.data(), L.cgeo64.data(), sizeof(double4) * cg64.size());
+		if ((rc = upload(p->d_cgeo64, cg64))) return rc;
+		if ((rc = upload(p->d_fcum, L.fcum))) return rc;
+	}
+	if (!opts.coalesce) {
+		std::vector<uint32_t> ord(sorted, sorted + nr);
+		if ((rc = upload(p->d_order, ord))) return rc;
+	}
+	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
+	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
+	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
+	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) p->max_iter) != hipSuccess ||
+	    hipHostMalloc(&p->h_state, sizeof(PState)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	*out = p.release();
+	return GP_OK;
+}
+
+static KArgs make_args(const gp_plan *p)
+{
+	KArgs a{};
+	a.rinfo = p->d_rinfo; a.cid = p->d_cid; a.cgeo = p->d_cgeo; a.cgeo64 = p->d_cgeo64; a.meta = p->d_meta;
+	a.fcum = p->d_fcum; a.order = p->d_order; a.pos = p->dm->pos; a.dist0 = p->d_dist0; a.dist1 = p->d_dist1;
+	a.st = p->d_state; a.trace = p->d_trace;
+	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
+	a.nl = p->nl; a.max_iter = p->max_iter; a.red_steps = p->red_steps;
+	return a;
+}
+
+// Enqueues sweeps k0 .. k0+count-1 (their k & 3 slots).
+static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32_t count)
+{
+	for (uint32_t q = 0; q < count; q++) {
+		const uint32_t k4 = (k0 + q) & 3u;
+		if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		else k_sweep<false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+	}
+}
+
+static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *out)
+{
+	hipGraph_t g = nullptr;
+	GP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+	enqueue_sweeps(p, a, 0, count);
+	GP_HIP(hipStreamEndCapture(p->stream, &g));
+	const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(g);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
 	return GP_OK;
 }
 
@@ -341,37 +477,44 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	const auto t0 = std::chrono::steady_clock::now();
 	const bool f64 = p->opts.precision == GP_PREC_F64;
 	const uint32_t done0 = p->nl < 3 ? 1u : 0u;
-	const uint32_t ig = std::min<uint32_t>(1024, (p->n + 255) / 256);
+	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));
+	const KArgs a = make_args(p);
+	constexpr uint32_t BIG = 64, SMALL = 8;
+	const bool graph = p->opts.launch_mode == GP_LAUNCH_GRAPH;
+	if (graph && !p->g_big) {
+		int rc = capture(p, a, BIG, &p->g_big);
+		if (!rc) rc = capture(p, a, SMALL, &p->g_small);
+		if (rc) return rc;
+	}
 	if (f64) {
-		k_init<double><<<ig, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->n, p->d_src, p->ns, p->d_state, done0);
+		k_init<double><<<ig, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->n, p->d_state, done0);
 		k_sources<double><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->d_src, p->ns);
 	} else {
-		k_init<float><<<ig, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->n, p->d_src, p->ns, p->d_state, done0);
+		k_init<float><<<ig, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->n, p->d_state, done0);
 		k_sources<float><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->d_src, p->ns);
 	}
 	GP_HIP(hipGetLastError());
-	SweepArgs a{};
-	a.st = p->d_state; a.limits = p->d_limits; a.order = p->d_order;
-	a.fan_off = p->dm->fan_off; a.fan = p->dm->fan; a.geo = p->dm->geo; a.pos = p->dm->pos;
-	a.dist0 = p->d_dist0; a.dist1 = p->d_dist1; a.trace = p->d_trace;
-	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h; a.nl = p->nl; a.max_iter = p->max_iter;
-
 	GP_HIP(hipEventRecord(p->e0, p->stream));
-	const uint32_t chunk = 64;
-	for (uint32_t launched = 0; launched < p->max_iter && !done0;) {
-		const uint32_t c = std::min(chunk, p->max_iter - launched);
-		for (uint32_t q = 0; q < c; q++) {
-			if (f64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a);
-			else k_sweep<false><<<p->grid, p->block, 0, p->stream>>>(a);
-		}
-		launched += c;
+	// Replays until the device reports done.  One sweep more than the sweep
+	// count runs (it records the last sweep's outcome).  j grows by one ring
+	// per sweep until |limits|-1, so at least nl-1-j sweeps remain: big graphs
+	// while that bound allows, small ones near the end.
+	uint32_t launched = 0, j_seen = 2;
+	while (!done0 && launched <= p->max_iter) {
+		const bool big = p->nl - 1 > j_seen + BIG;
+		const uint32_t cnt = big ? BIG : SMALL;
+		if (graph) GP_HIP(hipGraphLaunch(big ? p->g_big : p->g_small, p->stream));
+		else enqueue_sweeps(p, a, launched, cnt);
+		launched += cnt;
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(State), hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipStreamSynchronize(p->stream));
 		if (p->h_state->done) break;
+		j_seen = p->h_state->slot[(launched - 1) & 3u].j;
 	}
+	if (!done0 && !p->h_state->done) return fail(GP_ERR_KERNEL, "ptp: schedule did not finish in %u sweeps", launched);
 	GP_HIP(hipEventRecord(p->e1, p->stream));
-	GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(State), hipMemcpyDeviceToHost, p->stream));
+	GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
 	GP_HIP(hipStreamSynchronize(p->stream));
 	float ms = 0;
 	GP_HIP(hipEventElapsedTime(&ms, p->e0, p->e1));
@@ -392,7 +535,7 @@ int plan_result(gp_plan *p, double *dist)
 {
 	if (!p || !dist) return fail(GP_ERR_INVALID, "plan_result: null argument");
 	if (!p->has_run) return fail(GP_ERR_INVALID, "plan_result: plan has not run");
-	const uint32_t d = p->h_state->d;
+	const uint32_t d = p->h_state->final_d;
 	const bool last_read = p->opts.result_buffer == GP_RESULT_LAST_READ;
 	const void *buf = (last_read ? !d : d) ? p->d_dist1 : p->d_dist0;
 	const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));

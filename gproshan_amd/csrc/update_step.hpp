@@ -62,9 +62,10 @@ GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb
 	return p;
 }
 
-// Per-triangle FP32 geometry, precomputed on the host in FP64 in units of
-// the mesh's mean edge length h:  q0 = |X0|^2, q1 = |X1|^2 (from the next
-// fan entry), g = X0.X1, det = |X0 x X1|^2, S = |X1 - X0|^2.
+// Per-triangle geometry, precomputed on the host in FP64 in units of the
+// mesh's mean edge length h:  q0 = |X0|^2, q1 = |X1|^2 (from the next fan
+// entry), g = X0.X1, det = |X0 x X1|^2 (cross product, not q0 q1 - g^2),
+// S = |X1 - X0|^2.
 //
 // The same update as update_ref_f64, rewritten so that FP32 keeps its
 // precision: the quadratic (t - p1)^T Q (t - p1) = 1 is invariant under
@@ -75,26 +76,29 @@ GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb
 //   p' = (D + sqrt(D^2 - S E)) / S,
 //   sign c0 = sign(q1 (u0 - p') - g (u1 - p')),  sign c1 = sign(q0 (u1 - p') - g (u0 - p')),
 // which are the reference's delta, dis, p and c scaled by det (and det^2).
-GP_HD float update_stable_f32(float q0, float q1, float g, float det, float S, float t0, float t1,
-                              float h, float inv_h)
+// T = float is the bulk path; T = double is the fallback for triangles whose
+// angle at the updated vertex is near-degenerate or obtuse.
+template <typename T>
+GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 {
-	if (!(t0 != INFINITY && t1 != INFINITY)) {
-		const float dp0 = t0 + sqrtf(q0) * h;
-		const float dp1 = t1 + sqrtf(q1) * h;
+	const T inf = (T) INFINITY;
+	if (!(t0 != inf && t1 != inf)) {
+		const T dp0 = t0 + sqrt(q0) * h;
+		const T dp1 = t1 + sqrt(q1) * h;
 		return (dp1 < dp0) ? dp1 : dp0;
 	}
-	const float m = fminf(t0, t1);
-	const float u0 = (t0 - m) * inv_h;
-	const float u1 = (t1 - m) * inv_h;
-	const float D = u0 * (q1 - g) + u1 * (q0 - g);
-	const float E = u0 * u0 * q1 - 2.0f * u0 * u1 * g + u1 * u1 * q0 - det;
-	const float disc = D * D - S * E;
-	float pp = (D + sqrtf(disc)) / S;
-	const float c0 = q1 * (u0 - pp) - g * (u1 - pp);
-	const float c1 = q0 * (u1 - pp) - g * (u0 - pp);
-	if (disc < 0.0f || c0 >= 0.0f || c1 >= 0.0f) {
-		const float dp0 = t0 + sqrtf(q0) * h;
-		const float dp1 = t1 + sqrtf(q1) * h;
+	const T m = t0 < t1 ? t0 : t1;
+	const T u0 = (t0 - m) * inv_h;
+	const T u1 = (t1 - m) * inv_h;
+	const T D = u0 * (q1 - g) + u1 * (q0 - g);
+	const T E = u0 * u0 * q1 - (T) 2 * u0 * u1 * g + u1 * u1 * q0 - det;
+	const T disc = D * D - S * E;
+	const T pp = (D + sqrt(disc)) / S;
+	const T c0 = q1 * (u0 - pp) - g * (u1 - pp);
+	const T c1 = q0 * (u1 - pp) - g * (u0 - pp);
+	if (disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0) {
+		const T dp0 = t0 + sqrt(q0) * h;
+		const T dp1 = t1 + sqrt(q1) * h;
 		return (dp1 < dp0) ? dp1 : dp0;
 	}
 	return m + pp * h;

@@ -149,11 +149,12 @@ def normalize_ptp(dist):
 
 
 def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRITTEN, schedule=L.SCHED_RINGS,
-        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0):
+        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH):
     """One PTP solve with explicit options; returns (dist, sorted, stats dict)."""
     src = _u32(sources)
     o = L.default_opts(precision=precision, result_buffer=result_buffer, schedule=schedule,
-                       coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid)
+                       coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid,
+                       launch_mode=launch_mode)
     dist = np.zeros(mesh.n_vertices())
     srt = np.zeros(mesh.n_vertices(), np.uint32)
     st = L.PtpStats()

@@ -60,6 +60,12 @@ enum gp_schedule {
 	GP_SCHED_EUCLID_BAND = 2   /* bands from Euclidean distance to the nearest source (ablation) */
 };
 
+/* How the sweeps are launched. */
+enum gp_launch_mode {
+	GP_LAUNCH_GRAPH = 0,   /* sweep + schedule kernels replayed from captured hipGraphs (default) */
+	GP_LAUNCH_STREAM = 1   /* the same kernels launched one by one on the stream                 */
+};
+
 typedef struct gp_ptp_opts {
 	int precision;      /* gp_precision                                          */
 	int result_buffer;  /* gp_result_buffer                                      */
@@ -69,6 +75,7 @@ typedef struct gp_ptp_opts {
 	uint32_t max_sweeps;/* 0: the reference's 2*|limits| cap                     */
 	uint32_t block;     /* threads per workgroup (0: autotuned/default)          */
 	uint32_t grid;      /* workgroups (0: autotuned/default)                     */
+	int launch_mode;    /* gp_launch_mode                                        */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {

@@ -39,31 +39,33 @@ def test_toplesets_gpu_bit_exact(case):
         assert np.array_equal(a, b), what
 
 
+@pytest.mark.parametrize("launch", [L.LAUNCH_GRAPH, L.LAUNCH_STREAM], ids=["graph", "stream"])
 @pytest.mark.parametrize("coal", [True, False], ids=["coalesce", "plain"])
 @pytest.mark.parametrize("buf", [L.RESULT_LAST_WRITTEN, L.RESULT_LAST_READ], ids=["last_written", "last_read"])
-def test_f64_matches_oracle(case, oracle, coal, buf):
+def test_f64_matches_oracle(case, oracle, coal, buf, launch):
     name, xyz, f, src, M, C = case
     variant = (oracle.GPU_COAL if coal else oracle.GPU_PLAIN) if buf == L.RESULT_LAST_WRITTEN else \
               (oracle.CPU_COAL if coal else oracle.CPU_PLAIN)
     ref, nsw = M.ptp(src, variant)
-    d, _, st = P.ptp(C, src, precision=L.PREC_F64, coalesce=coal, result_buffer=buf)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F64, coalesce=coal, result_buffer=buf, launch_mode=launch)
     mx, mean = rel_err(d, ref)
-    print(f"[f64] {name} coal={coal} buf={buf} sweeps {st['sweeps']}/{nsw} max {mx:.3e} mean {mean:.3e} "
+    print(f"[f64] {name} coal={coal} buf={buf} launch={launch} sweeps {st['sweeps']}/{nsw} max {mx:.3e} mean {mean:.3e} "
           f"kernel {st['kernel_ms']:.2f} ms", flush=True)
     assert st["sweeps"] == nsw
     assert mx <= 1e-12
 
 
-def test_f32_matches_oracle(case, oracle):
+@pytest.mark.parametrize("launch", [L.LAUNCH_GRAPH, L.LAUNCH_STREAM], ids=["graph", "stream"])
+def test_f32_matches_oracle(case, oracle, launch):
     name, xyz, f, src, M, C = case
     ref, nsw, tr = M.ptp(src, oracle.GPU_COAL, trace=True)
-    plan = P.Plan(C, src, precision=L.PREC_F32)
+    plan = P.Plan(C, src, precision=L.PREC_F32, launch_mode=launch)
     st = plan.run()
     d = plan.result()
     gtr = plan.trace()
     same_sched = len(gtr) == len(tr) and np.array_equal(gtr[:, 1:7], np.stack([tr[k] for k in ("i", "j", "start", "end", "ncond", "count")], 1))
     mx, mean = rel_err(d, ref)
-    print(f"[f32] {name} sweeps {st['sweeps']}/{nsw} same schedule {same_sched} max {mx:.3e} mean {mean:.3e} "
+    print(f"[f32] {name} launch={launch} sweeps {st['sweeps']}/{nsw} same schedule {same_sched} max {mx:.3e} mean {mean:.3e} "
           f"f64 updates {st['f64_updates']} kernel {st['kernel_ms']:.2f} ms", flush=True)
     if same_sched:
         assert mx <= 1e-5
