@@ -20,7 +20,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DUPLICATE_SOURCE, ERR_HIP, ERR_NO_DEVICE, ERR_CA
     0, -1, -2, -3, -4, -5, -6, -7
 PREC_F32, PREC_F64 = 0, 1
 RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
-SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND = 0, 1, 2
+SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND, SCHED_RING_BAND = 0, 1, 2, 3
 LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
 
 # Every symbol include/gproshan_ptp.h declares.
@@ -31,7 +31,7 @@ EXPORTS = [
     "gp_parallel_toplesets_propagation_gpu", "gp_parallel_toplesets_propagation_coalescence_gpu",
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
     "gp_normalize_ptp", "gp_ptp",
-    "gp_plan_create", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
+    "gp_plan_create", "gp_plan_create_bands", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
 ]
 
 
@@ -44,7 +44,8 @@ class GPError(RuntimeError):
 class PtpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
-                ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int)]
+                ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int),
+                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("judge_cap", C.c_uint32)]
 
 
 class PtpStats(C.Structure):
@@ -88,6 +89,7 @@ def lib() -> C.CDLL:
     L.gp_normalize_ptp.argtypes = [P, u32]
     L.gp_ptp.argtypes = [P, P, u32, C.POINTER(PtpOpts), P, P, C.POINTER(PtpStats)]
     L.gp_plan_create.argtypes = [P, P, u32, C.POINTER(PtpOpts), C.POINTER(P)]
+    L.gp_plan_create_bands.argtypes = [P, P, u32, P, C.POINTER(PtpOpts), C.POINTER(P)]
     L.gp_plan_run.argtypes = [P, C.POINTER(PtpStats)]
     L.gp_plan_result.argtypes = [P, P]
     L.gp_plan_trace.argtypes = [P, P, u32, C.POINTER(u32)]

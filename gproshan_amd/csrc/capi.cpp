@@ -1,5 +1,6 @@
 // extern "C" boundary of libgproshan_ptp.  Every function catches all C++
 // exceptions and maps them to a status code; nothing unwinds across the ABI.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -23,6 +24,9 @@ struct gp_che {
 		return gp::fail(GP_ERR_INVALID, "unknown exception");                          \
 	}
 
+static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                           const gp_ptp_opts &o, gp_plan **out);
+
 extern "C" {
 
 void gp_ptp_opts_default(gp_ptp_opts *o)
@@ -34,6 +38,10 @@ void gp_ptp_opts_default(gp_ptp_opts *o)
 	o->schedule = GP_SCHED_RINGS;
 	o->coalesce = 1;
 	o->tol = GP_PTP_TOL;
+	o->launch_mode = GP_LAUNCH_GRAPH;
+	o->band_width = 4.0;
+	o->local_iters = 8;
+	o->judge_cap = 64;
 }
 
 const char *gp_last_error(void) { return gp::last_error(); }
@@ -224,12 +232,13 @@ int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_p
 		gp_ptp_opts o;
 		if (opts) o = *opts; else gp_ptp_opts_default(&o);
 		const gp::Che &m = mesh->m;
-		std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
-		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
-		if (rc) return rc;
-		if (sorted_out) std::memcpy(sorted_out, sorted.data(), sizeof(uint32_t) * m.nv);
+		if (sorted_out) {
+			std::vector<uint32_t> lev(m.nv), lim;
+			int rc0 = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted_out, lim);
+			if (rc0) return rc0;
+		}
 		gp_plan *p = nullptr;
-		rc = gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, &p);
+		int rc = plan_create_any(mesh, sources, n_sources, nullptr, o, &p);
 		if (rc) return rc;
 		std::unique_ptr<gp_plan, void (*)(gp_plan *)> guard(p, gp_plan_destroy);
 		rc = gp::plan_run(p, stats);
@@ -238,17 +247,69 @@ int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_p
 	)
 }
 
+}  // extern "C"
+
+// Plan of any schedule: rings, or bands from dhat (caller-supplied for the
+// learned schedule; Euclidean distance or ring index otherwise).
+static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                           const gp_ptp_opts &o, gp_plan **out)
+{
+	const gp::Che &m = mesh->m;
+	std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
+	int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
+	if (rc) return rc;
+	if (o.schedule == GP_SCHED_RINGS)
+		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
+	std::vector<double> d;
+	if (!dhat) {
+		d.assign(m.nv, INFINITY);
+		if (o.schedule == GP_SCHED_EUCLID_BAND) {
+			#pragma omp parallel for schedule(static)
+			for (int64_t v = 0; v < (int64_t) m.nv; v++) {
+				double best = INFINITY;
+				for (uint32_t q = 0; q < n_sources; q++) {
+					const double *a = &m.gt[3 * (size_t) v], *b = &m.gt[3 * (size_t) sources[q]];
+					best = std::min(best, std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+					                                (a[2] - b[2]) * (a[2] - b[2])));
+				}
+				d[v] = best;
+			}
+		} else if (o.schedule == GP_SCHED_RING_BAND) {
+			for (uint32_t v = 0; v < m.nv; v++) d[v] = lev[v] == gp::NIL ? INFINITY : (double) lev[v] * m.mean_edge;
+		} else {
+			return gp::fail(GP_ERR_INVALID, "learned bands need predicted distances (gp_plan_create_bands)");
+		}
+		dhat = d.data();
+	}
+	std::vector<uint32_t> order, blim;
+	rc = gp::band_order(m, sources, n_sources, lev.data(), dhat, o.band_width * m.mean_edge, order, blim);
+	if (rc) return rc;
+	gp_ptp_opts ob = o;
+	ob.coalesce = 1;
+	return gp::plan_create_rings(&mesh->m, sources, n_sources, blim.data(), (uint32_t) blim.size(), order.data(), ob, out);
+}
+
+extern "C" {
+
 int gp_plan_create(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_ptp_opts *opts, gp_plan **out)
 {
 	GP_GUARD(
 		if (!mesh || !sources || !out) return gp::fail(GP_ERR_INVALID, "null argument");
 		gp_ptp_opts o;
 		if (opts) o = *opts; else gp_ptp_opts_default(&o);
-		const gp::Che &m = mesh->m;
-		std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
-		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
-		if (rc) return rc;
-		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
+		return plan_create_any(mesh, sources, n_sources, nullptr, o, out);
+	)
+}
+
+int gp_plan_create_bands(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                         const gp_ptp_opts *opts, gp_plan **out)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !out || !dhat) return gp::fail(GP_ERR_INVALID, "null argument");
+		gp_ptp_opts o;
+		if (opts) o = *opts; else gp_ptp_opts_default(&o);
+		if (o.schedule == GP_SCHED_RINGS) return gp::fail(GP_ERR_INVALID, "gp_plan_create_bands needs a band schedule");
+		return plan_create_any(mesh, sources, n_sources, dhat, o, out);
 	)
 }
 

@@ -172,51 +172,62 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	return GP_OK;
 }
 
-int build_layout(const Che &m, const uint32_t *order, uint32_t nr, Layout &L)
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile)
 {
-	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.5;
+	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.9;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
-	L.cid.clear(); L.cgeo.clear(); L.cgeo64.clear(); L.meta.clear();
-	L.pchunk.assign(nr, 0);
+	L.nchunks = (nr + 63) / 64;
+	L.coff.assign((size_t) L.nchunks + 1, 0);
 	L.fcum.assign((size_t) nr + 1, 0);
-	L.max_seg = 1; L.n_flagged = 0;
-	uint64_t head = 0;
-	uint32_t used = 0, vbase = 0;
-	auto close_chunk = [&]() {
-		for (uint32_t l = used; l < 64; l++) {
-			L.cid.push_back(0);
-			L.cgeo.push_back(Geo32{0.f, 0.f, 0.f, 0.f});
-			L.cgeo64.push_back(Geo64{0, 0, 0, 0});
+	L.max_slots = 0;
+	L.n_flagged = 0;
+	for (uint32_t c = 0; c < L.nchunks; c++) {
+		uint32_t S = 0;
+		for (uint32_t pos = 64 * c; pos < std::min(nr, 64 * c + 64); pos++) {
+			const uint32_t v = order ? order[pos] : pos;
+			if (v >= m.nv) return fail(GP_ERR_INVALID, "layout: vertex %u out of range", v);
+			if (v >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "layout: vertex ids must stay below 2^30");
+			S = std::max(S, m.fan_off[v + 1] - m.fan_off[v]);
 		}
-		L.meta.push_back(ChunkMeta{ (uint32_t) head, (uint32_t) (head >> 32), vbase, used });
-		head = 0; used = 0;
-	};
-	for (uint32_t pos = 0; pos < nr; pos++) {
+		L.coff[c + 1] = L.coff[c] + S;
+		L.max_slots = std::max(L.max_slots, S);
+	}
+	const size_t lanes = (size_t) L.coff[L.nchunks] * 64;
+	L.cid.assign(std::max<size_t>(lanes, 1), 0);
+	if (f64) {
+		L.cx64.assign(std::max<size_t>(lanes, 1), Geo64{0, 0, 0, 0});
+		L.cgeo.clear(); L.cgeo64.clear();
+	} else {
+		L.cgeo.assign(std::max<size_t>(lanes, 1), Geo32{0.f, 0.f, 0.f, 0.f});
+		L.cgeo64.assign(std::max<size_t>(lanes, 1), Geo64{0, 0, 0, 0});
+		L.cx64.clear();
+	}
+	std::vector<uint32_t> flagged_at(nr, 0);
+	#pragma omp parallel for schedule(dynamic, 64)
+	for (int64_t pp = 0; pp < (int64_t) nr; pp++) {
+		const uint32_t pos = (uint32_t) pp;
+		const uint32_t c = pos / 64, lane = pos % 64;
+		const uint32_t S = L.coff[c + 1] - L.coff[c];
+		const size_t base = (size_t) L.coff[c] * 64 + lane;
 		const uint32_t v = order ? order[pos] : pos;
-		if (v >= m.nv) return fail(GP_ERR_INVALID, "layout: vertex %u out of range", v);
-		const uint32_t s = m.fan_off[v], e = m.fan_off[v + 1];
-		const uint32_t lanes = e > s ? e - s : 1;
-		if (lanes > 64) return fail(GP_ERR_CAPACITY, "layout: vertex %u has %u fan entries (max 64)", v, lanes);
-		if (used + lanes > 64) close_chunk();
-		if (used == 0) vbase = pos;
-		head |= 1ull << used;
-		L.pchunk[pos] = (uint32_t) L.meta.size();
-		L.max_seg = std::max(L.max_seg, lanes);
-		if (e == s) {
-			L.cid.push_back(v);
-			L.cgeo.push_back(Geo32{0.f, 0.f, 0.f, 0.f});
-			L.cgeo64.push_back(Geo64{0, 0, 0, 0});
-		} else {
-			const double *pv = &m.gt[3 * (size_t) v];
-			for (uint32_t k = s; k < e; k++) {
-				const double *pa = &m.gt[3 * (size_t) m.fan[k]];
-				const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+		const uint32_t s0 = m.fan_off[v], n = m.fan_off[v + 1] - s0;
+		const double *pv = &m.gt[3 * (size_t) v];
+		uint32_t nflag = 0;
+		for (uint32_t k = 0; k < S; k++) {
+			const size_t at = base + 64 * (size_t) k;
+			if (k >= n) { L.cid[at] = v | SLOT_EMPTY; continue; }
+			const uint32_t a = m.fan[s0 + k];
+			const double *pa = &m.gt[3 * (size_t) a];
+			// Same subtraction as the reference's vertex::operator-.
+			const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+			uint32_t id = a;
+			if (f64) {
+				L.cx64[at] = Geo64{ X0[0], X0[1], X0[2], 0 };
+			} else {
 				const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
-				Geo32 gk{ (float) (q0 * ih2), 0.f, 0.f, 0.f };
 				Geo64 gd{ q0 * ih2, 0, 0, 0 };
-				uint32_t id = m.fan[k];
-				if (k + 1 < e) {
-					const double *pb = &m.gt[3 * (size_t) m.fan[k + 1]];
+				if (k + 1 < n) {
+					const double *pb = &m.gt[3 * (size_t) m.fan[s0 + k + 1]];
 					const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
 					const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
 					const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
@@ -228,24 +239,85 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, Layout &L)
 					gd.y = g * ih2;
 					gd.z = det * ih2 * ih2;
 					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
-					gk.y = (float) gd.y;
-					gk.z = (float) gd.z;
-					gk.w = (float) gd.w;
 					const double qq = q0 * q1;
-					if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; L.n_flagged++; }
+					if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; nflag++; }
 				}
-				L.cid.push_back(id);
-				L.cgeo.push_back(gk);
-				L.cgeo64.push_back(gd);
+				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w };
+				L.cgeo64[at] = gd;
+			}
+			L.cid[at] = id;
+		}
+		flagged_at[pos] = nflag;
+	}
+	for (uint32_t pos = 0; pos < nr; pos++) L.fcum[pos + 1] = L.fcum[pos] + flagged_at[pos];
+	L.n_flagged = L.fcum[nr];
+	L.lidx.clear();
+	if (tile) {
+		if (order) return fail(GP_ERR_INVALID, "layout: tiles need a mesh numbered by positions");
+		L.lidx.assign(L.cid.size(), LOCAL_OUT);
+		#pragma omp parallel for schedule(static)
+		for (int64_t pp = 0; pp < (int64_t) nr; pp++) {
+			const uint32_t pos = (uint32_t) pp, c = pos / 64, lane = pos % 64;
+			const uint32_t S = L.coff[c + 1] - L.coff[c];
+			for (uint32_t k = 0; k < S; k++) {
+				const size_t at = (size_t) (L.coff[c] + k) * 64 + lane;
+				const uint32_t id = L.cid[at];
+				if (id & SLOT_EMPTY) continue;
+				const uint32_t u = id & ID30_MASK;
+				if (u / tile == pos / tile) L.lidx[at] = (uint16_t) (u % tile);
 			}
 		}
-		used += lanes;
-		L.fcum[pos + 1] = L.n_flagged;
 	}
-	if (used) close_chunk();
-	L.nchunks = (uint32_t) L.meta.size();
-	L.red_steps = 0;
-	while ((1u << L.red_steps) < L.max_seg) L.red_steps++;
+	return GP_OK;
+}
+
+static inline uint32_t spread10(uint32_t x)
+{
+	x &= 0x3FF;
+	x = (x | (x << 16)) & 0x030000FF;
+	x = (x | (x << 8)) & 0x0300F00F;
+	x = (x | (x << 4)) & 0x030C30C3;
+	x = (x | (x << 2)) & 0x09249249;
+	return x;
+}
+
+int band_order(const Che &m, const uint32_t *src, uint32_t ns, const uint32_t *levels, const double *dhat,
+               double width, std::vector<uint32_t> &order, std::vector<uint32_t> &limits)
+{
+	if (!(width > 0)) return fail(GP_ERR_INVALID, "band width must be positive");
+	double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+	for (uint32_t v = 0; v < m.nv; v++)
+		for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], m.gt[3 * (size_t) v + k]); hi[k] = std::max(hi[k], m.gt[3 * (size_t) v + k]); }
+	double sc[3];
+	for (int k = 0; k < 3; k++) sc[k] = hi[k] > lo[k] ? 1023.0 / (hi[k] - lo[k]) : 0.0;
+	std::vector<uint8_t> is_src(m.nv, 0);
+	for (uint32_t q = 0; q < ns; q++) is_src[src[q]] = 1;
+	// (band, morton, vertex) keys of the reached non-source vertices.
+	std::vector<std::pair<uint64_t, uint32_t>> keys;
+	keys.reserve(m.nv);
+	for (uint32_t v = 0; v < m.nv; v++) {
+		if (levels[v] == NIL || is_src[v]) continue;
+		const double d = dhat[v];
+		const double bq = std::isfinite(d) && d > 0 ? std::floor(d / width) : 0.0;
+		const uint64_t band = 1 + (uint64_t) std::min(bq, 4.0e9);
+		const double *p = &m.gt[3 * (size_t) v];
+		const uint32_t mort = spread10((uint32_t) ((p[0] - lo[0]) * sc[0])) |
+		                      (spread10((uint32_t) ((p[1] - lo[1]) * sc[1])) << 1) |
+		                      (spread10((uint32_t) ((p[2] - lo[2]) * sc[2])) << 2);
+		keys.emplace_back((band << 30) | mort, v);
+	}
+	std::sort(keys.begin(), keys.end());
+	order.clear();
+	order.reserve(ns + keys.size());
+	for (uint32_t q = 0; q < ns; q++) order.push_back(src[q]);
+	limits.assign(1, 0);
+	uint64_t cur = ~0ull;
+	for (size_t q = 0; q < keys.size(); q++) {
+		const uint64_t band = keys[q].first >> 30;
+		if (band != cur) { limits.push_back((uint32_t) order.size()); cur = band; }
+		order.push_back(keys[q].second);
+	}
+	limits.push_back((uint32_t) order.size());
 	return GP_OK;
 }
 

@@ -19,13 +19,13 @@
 // eight XCD L2s costs ~36 ns per workgroup, ~19 us per barrier at 512
 // workgroups, against ~1.6 us per graph-launched kernel.)
 //
-// Work layout (see Layout in gp_internal.hpp): one lane per fan entry,
-// vertices packed whole into wave64 chunks in ring order.  A lane gathers the
-// old distance of its fan vertex, takes the next lane's distance with a
-// shuffle (the triangle (k, k+1) of the fan), runs the triangle update, and a
-// segmented shuffle-min leaves each vertex's minimum in its first lane, which
-// writes the new distance and tests ring-i convergence.  Each XCD takes one
-// contiguous eighth of the window's chunks so neighbouring gathers share its L2.
+// Work layout (see Layout in gp_internal.hpp): ELL, one lane per vertex in
+// ring order, 64 vertices per wave.  A wave issues all slot loads of its
+// chunk (fan ids and triangle geometry, one contiguous access per slot), then
+// all neighbour-distance gathers, then runs the triangle updates from
+// registers: two dependent memory round trips per 64 vertices.  Each XCD
+// takes one contiguous eighth of the window's chunks so neighbouring gathers
+// share its L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,7 +103,7 @@ int upload_mesh(Che &m)
 // launch writes it, so kernel boundaries are the only synchronisation.  The
 // captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
 struct Slot {
-	uint32_t i, j, d, iter, done, fresh, pad[2];
+	uint32_t i, j, d, iter, done, fresh, phase, judge;
 };
 struct PState {
 	Slot slot[4];
@@ -113,32 +113,129 @@ struct PState {
 };
 
 struct KArgs {
-	const uint4 *rinfo;        // per ring r: limits[r], first chunk of ring r, end chunk of rings < r
+	const uint32_t *limits;
+	const uint32_t *coff;      // per chunk: first slot row
 	const uint32_t *cid;
 	const float4 *cgeo;
 	const double4 *cgeo64;
-	const uint4 *meta;
+	const double4 *cx64;
 	const unsigned long long *fcum;  // FP64-update triangles of positions [0, pos)
 	const uint32_t *order;     // position -> vertex; null when the mesh is numbered by rings
-	const double *pos;
 	void *dist0, *dist1;
 	PState *st;
 	uint32_t *trace;           // 8 words per sweep, or null
+	const uint16_t *lidx;      // band solves: slot neighbour's index in its tile, or LOCAL_OUT
 	double tol;
 	float h, inv_h;
-	uint32_t nl, max_iter, red_steps;
+	uint32_t nl, max_iter;
+	uint32_t nr, local_iters, judge_cap;
 };
+
+constexpr uint32_t MAXS = 10;  // register-resident slots; longer fans take the streaming loop
+
+// FP64 update of triangle (k, k+1) of the lane at slot base (flagged triangles).
+__device__ __forceinline__ float tri_flagged(const KArgs &a, size_t at, float ta, float tb)
+{
+	const double4 gd = a.cgeo64[at];
+	const double q1 = a.cgeo64[at + 64].x;
+	return (float) update_stable<double>(gd.x, q1, gd.y, gd.z, gd.w, (double) ta, (double) tb, (double) a.h,
+	                                     (double) a.inv_h);
+}
+
+__device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, double ta, double tb)
+{
+	const double X0[3] = { xa.x, xa.y, xa.z };
+	const double X1[3] = { xb.x, xb.y, xb.z };
+	return update_ref_f64_x(X0, X1, ta, tb);
+}
+
+// New value of the vertex at lane slot base (chunk with S slots).  FP32:
+// flagged triangles are skipped in the slot loop and done afterwards in a
+// loop that runs as many times as the most-flagged lane of the wave needs,
+// so the FP64 code runs ~1-2 times per wave instead of once per slot.
+template <bool F64>
+__device__ __forceinline__ typename std::conditional<F64, double, float>::type
+relax_lane(const KArgs &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
+           typename std::conditional<F64, double, float>::type best)
+{
+	using T = typename std::conditional<F64, double, float>::type;
+	using G = typename std::conditional<F64, double4, float4>::type;
+	const G *geo;
+	if constexpr (F64) geo = a.cx64; else geo = a.cgeo;
+	uint32_t fmask = 0;
+	if (S <= MAXS) {
+		uint32_t id[MAXS];
+		G g[MAXS];
+		T t[MAXS];
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++) {
+			id[k] = SLOT_EMPTY;
+			if (k < S) { id[k] = a.cid[base + 64 * k]; g[k] = geo[base + 64 * k]; }
+		}
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++)
+			t[k] = (k < S) ? od[id[k] & ID30_MASK] : (T) INFINITY;
+#pragma unroll
+		for (uint32_t k = 0; k + 1 < MAXS; k++) {
+			if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
+				if constexpr (F64) {
+					const T p = tri_f64(g[k], g[k + 1], t[k], t[k + 1]);
+					if (p < best) best = p;
+				} else {
+					if (id[k] & F64_FLAG) {
+						fmask |= 1u << k;
+					} else {
+						const T p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1],
+						                                 a.h, a.inv_h);
+						if (p < best) best = p;
+					}
+				}
+			}
+		}
+	} else {
+		uint32_t ida = a.cid[base];
+		G ga = geo[base];
+		T ta = od[ida & ID30_MASK];
+		for (uint32_t k = 1; k < S; k++) {
+			const uint32_t idb = a.cid[base + 64 * k];
+			if (idb & SLOT_EMPTY) break;
+			const G gb = geo[base + 64 * k];
+			const T tb = od[idb & ID30_MASK];
+			if constexpr (F64) {
+				const T p = tri_f64(ga, gb, ta, tb);
+				if (p < best) best = p;
+			} else {
+				const T p = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
+				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
+				if (p < best) best = p;
+			}
+			ida = idb; ga = gb; ta = tb;
+		}
+	}
+	if constexpr (!F64) {
+		while (fmask) {
+			const uint32_t k = __builtin_ctz(fmask);
+			fmask &= fmask - 1u;
+			const size_t at = base + 64 * (size_t) k;
+			const float ta = od[a.cid[at] & ID30_MASK];
+			const float tb = od[a.cid[at + 64] & ID30_MASK];
+			const float p = tri_flagged(a, at, ta, tb);
+			if (p < best) best = p;
+		}
+	}
+	return best;
+}
 
 // Relaxes the window [start, end) (positions); returns this thread's count of
 // non-converged vertices among positions [start, cend).
 template <bool F64>
 __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
-                                               uint32_t c0, uint32_t c1,
                                                const typename std::conditional<F64, double, float>::type *od,
                                                typename std::conditional<F64, double, float>::type *nd)
 {
 	using T = typename std::conditional<F64, double, float>::type;
-	if (start >= end || c0 >= c1) return 0;
+	if (start >= end) return 0;
+	const uint32_t c0 = start >> 6, c1 = (end + 63u) >> 6;
 	const uint32_t lane = threadIdx.x & 63u;
 	// XCD-aware split: workgroups b with b % 8 == x run on XCD x and take the
 	// x-th contiguous eighth of the window's chunks.
@@ -152,55 +249,16 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 	const uint32_t part = (len + ngrp - 1) / ngrp;
 	const uint32_t lo = c0 + x * part;
 	const uint32_t hi = min(c1, lo + part);
-	const unsigned long long upto = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
 	uint32_t bad = 0;
 	for (uint32_t c = lo + lw; c < hi; c += nw_x) {
-		const uint4 mt = a.meta[c];
-		const unsigned long long head = ((unsigned long long) mt.y << 32) | mt.x;
-		const uint32_t used = mt.w;
-		const uint32_t id = a.cid[64u * c + lane];
-		const uint32_t nb = id & ID_MASK;
-		const T t = od[nb];
-		const T t1 = __shfl_down(t, 1);
-		const uint32_t nb1 = __shfl_down(nb, 1);
-		const unsigned long long above = head & ~upto;
-		const uint32_t seg_end = above ? (uint32_t) __builtin_ctzll(above) : used;
-		const uint32_t vpos = mt.z + (uint32_t) __builtin_popcountll(head & upto) - 1u;
-		const bool is_head = ((head >> lane) & 1ull) && lane < used;
-		const bool has_tri = lane + 1u < seg_end;
-		T p = (T) INFINITY;
-		if constexpr (F64) {
-			if (has_tri) {
-				const uint32_t v = a.order ? a.order[vpos] : vpos;
-				p = update_ref_f64(a.pos + 3 * (size_t) v, a.pos + 3 * (size_t) nb, a.pos + 3 * (size_t) nb1, t, t1);
-			}
-		} else {
-			const float4 g = a.cgeo[64u * c + lane];
-			const float q1 = __shfl_down(g.x, 1);
-			if (has_tri) {
-				if (id & F64_FLAG) {
-					// has_tri implies lane + 1 < 64: the next entry is in this chunk.
-					const double4 gd = a.cgeo64[64u * c + lane];
-					const double q1d = a.cgeo64[64u * c + lane + 1u].x;
-					p = (float) update_stable<double>(gd.x, q1d, gd.y, gd.z, gd.w, (double) t, (double) t1,
-					                                  (double) a.h, (double) a.inv_h);
-				} else {
-					p = update_stable<float>(g.x, q1, g.y, g.z, g.w, t, t1, a.h, a.inv_h);
-				}
-			}
-		}
-		// Segmented minimum over the vertex's lanes; NaN never wins.
-		for (uint32_t s = 0; s < a.red_steps; s++) {
-			const uint32_t off = 1u << s;
-			const T q = __shfl_down(p, off);
-			if (lane + off < seg_end && q < p) p = q;
-		}
-		if (is_head && vpos >= start && vpos < end) {
-			const uint32_t v = a.order ? a.order[vpos] : vpos;
+		const uint32_t pos = (c << 6) + lane;
+		const uint32_t cb = a.coff[c], S = a.coff[c + 1] - cb;
+		if (pos >= start && pos < end) {
+			const uint32_t v = a.order ? a.order[pos] : pos;
 			const T old = od[v];
-			const T best = (p < old) ? p : old;
+			const T best = relax_lane<F64>(a, (size_t) cb * 64 + lane, S, od, old);
 			nd[v] = best;
-			if (vpos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+			if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
 		}
 	}
 	return bad;
@@ -238,16 +296,16 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 		cur.iter = prev.iter + 1;
 		cur.done = !(cur.i < cur.j) || !(cur.iter < a.max_iter);
 		if (lead) {
-			const uint4 r0 = a.rinfo[pi], r1 = a.rinfo[pi + 1], rj = a.rinfo[prev.j];
-			const uint32_t ncond = r1.x - r0.x;
+			const uint32_t l0 = a.limits[pi], l1 = a.limits[pi + 1], lj = a.limits[prev.j];
+			const uint32_t ncond = l1 - l0;
 			if (a.trace) {
 				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
-				r[0] = prev.iter + 1; r[1] = pi; r[2] = prev.j; r[3] = r0.x; r[4] = rj.x; r[5] = ncond;
+				r[0] = prev.iter + 1; r[1] = pi; r[2] = prev.j; r[3] = l0; r[4] = lj; r[5] = ncond;
 				r[6] = ncond - nbad; r[7] = 0;
 			}
-			st->relax += rj.x - r0.x;
+			st->relax += lj - l0;
 			st->cond += ncond;
-			if (a.fcum) st->f64 += a.fcum[rj.x] - a.fcum[r0.x];
+			if (a.fcum) st->f64 += a.fcum[lj] - a.fcum[l0];
 			st->sweeps = cur.iter;
 		}
 	}
@@ -258,10 +316,194 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	}
 	if (cur.done) return;
 	const uint32_t i = cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
-	const uint4 ri = a.rinfo[i], ri1 = a.rinfo[i + 1], rj = a.rinfo[cur.j];
+	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = wave_sum(sweep_body<F64>(a, ri.x, rj.x, ri1.x, ri.y, rj.z, od, nd));
+	const uint32_t bad = wave_sum(sweep_body<F64>(a, start, end, cend, od, nd));
+	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
+}
+
+// ---------------------------------------------------------------------------
+// Band schedule (learned / Euclidean / ring-group bands; no reference
+// counterpart).  Vertices are numbered by band, then by Morton code inside a
+// band, so a tile of TILE consecutive positions is a compact patch of one or
+// two bands.  A workgroup loads its tile's distances into LDS and runs up to
+// local_iters Jacobi iterations on it (out-of-tile neighbours held at their
+// values from the previous sweep), so the front crosses up to local_iters
+// hops per sweep instead of one.  The window [i, j) of bands grows by one band
+// per sweep; band i retires when it changed by less than tol and band i+1 was
+// relaxed with it.  When the window closes, judge sweeps relax the whole mesh
+// until no vertex changes by tol: a bad prediction costs sweeps, never
+// accuracy.
+
+constexpr uint32_t TILE = 256;
+
+__device__ __forceinline__ float lb_gather(uint32_t li, const float *sd, float tout)
+{
+	return li != LOCAL_OUT ? sd[li] : tout;
+}
+
+// Relaxes positions [ws, we) tile by tile; returns this thread's count of
+// non-converged vertices among [cs, ce).
+__device__ uint32_t lb_tiles(const KArgs &a, uint32_t ws, uint32_t we, uint32_t cs, uint32_t ce, const float *od,
+                             float *nd)
+{
+	__shared__ float sd[2][TILE];
+	if (ws >= we) return 0;
+	const uint32_t t0 = ws / TILE, t1 = (we + TILE - 1) / TILE;
+	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
+	const uint32_t x = blockIdx.x & 7u;
+	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
+	const uint32_t len = t1 - t0;
+	const uint32_t part = (len + ngrp - 1) / ngrp;
+	const uint32_t lo = t0 + x * part;
+	const uint32_t hi = min(t1, lo + part);
+	const uint32_t tid = threadIdx.x, lane = tid & 63u;
+	uint32_t bad = 0;
+	for (uint32_t tile = lo + (blockIdx.x >> 3); tile < hi; tile += nbx) {
+		const uint32_t pos = tile * TILE + tid;
+		const bool live = pos < a.nr;
+		const bool act = live && pos >= ws && pos < we;
+		uint32_t S = 0;
+		size_t base = 0;
+		if (live) {
+			const uint32_t c = pos >> 6, cb = a.coff[c];
+			S = a.coff[c + 1] - cb;
+			base = (size_t) cb * 64 + lane;
+		}
+		const float old = live ? od[pos] : (float) INFINITY;
+		const bool fast = S <= MAXS;
+		uint32_t id[MAXS], li[MAXS];
+		float4 g[MAXS];
+		float tout[MAXS];
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++) {
+			id[k] = SLOT_EMPTY; li[k] = LOCAL_OUT; tout[k] = (float) INFINITY;
+			if (act && fast && k < S) {
+				id[k] = a.cid[base + 64 * k];
+				li[k] = a.lidx[base + 64 * k];
+				g[k] = a.cgeo[base + 64 * k];
+			}
+		}
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++)
+			if (act && fast && k < S && li[k] == LOCAL_OUT) tout[k] = od[id[k] & ID30_MASK];
+		sd[0][tid] = old;
+		__syncthreads();
+		uint32_t b = 0;
+		for (uint32_t it = 0; it < a.local_iters; it++) {
+			const float cur = sd[b][tid];
+			float best = cur;
+			if (act && fast) {
+				float t[MAXS];
+#pragma unroll
+				for (uint32_t k = 0; k < MAXS; k++) t[k] = (k < S) ? lb_gather(li[k], sd[b], tout[k]) : (float) INFINITY;
+#pragma unroll
+				for (uint32_t k = 0; k + 1 < MAXS; k++) {
+					if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
+						const float p = (id[k] & F64_FLAG)
+						                    ? tri_flagged(a, base + 64 * k, t[k], t[k + 1])
+						                    : update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k],
+						                                           t[k + 1], a.h, a.inv_h);
+						if (p < best) best = p;
+					}
+				}
+			} else if (act) {
+				// Fans longer than MAXS: stream the slots.
+				uint32_t ida = a.cid[base];
+				uint32_t lia = a.lidx[base];
+				float4 ga = a.cgeo[base];
+				float ta = lia != LOCAL_OUT ? sd[b][lia] : od[ida & ID30_MASK];
+				for (uint32_t k = 1; k < S; k++) {
+					const uint32_t idb = a.cid[base + 64 * k];
+					if (idb & SLOT_EMPTY) break;
+					const uint32_t lib = a.lidx[base + 64 * k];
+					const float4 gb = a.cgeo[base + 64 * k];
+					const float tb = lib != LOCAL_OUT ? sd[b][lib] : od[idb & ID30_MASK];
+					const float p = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
+					                                 : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h,
+					                                                        a.inv_h);
+					if (p < best) best = p;
+					ida = idb; ga = gb; ta = tb;
+				}
+			}
+			sd[b ^ 1u][tid] = best;
+			const int changed = __syncthreads_or(best < cur);
+			b ^= 1u;
+			if (!changed) break;
+		}
+		if (act) {
+			const float v = sd[b][tid];
+			nd[pos] = v;
+			if (pos >= cs && pos < ce) bad += !(fabs((double) v - (double) old) / (double) old < a.tol);
+		}
+		__syncthreads();
+	}
+	return bad;
+}
+
+__device__ __forceinline__ void lb_window(const KArgs &a, const Slot &s, uint32_t &ws, uint32_t &we, uint32_t &cs,
+                                          uint32_t &ce)
+{
+	if (s.phase == 0) {
+		ws = a.limits[s.i]; we = a.limits[s.j]; cs = ws; ce = a.limits[s.i + 1];
+	} else {
+		ws = 0; we = a.nr; cs = 0; ce = a.nr;
+	}
+}
+
+// Band-schedule sweep k (k4 = k & 3).
+__global__ __launch_bounds__(256) void k_lb_sweep(KArgs a, uint32_t k4)
+{
+	PState *st = a.st;
+	const uint32_t kp = (k4 + 3u) & 3u;
+	const Slot prev = st->slot[kp];
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if (prev.done) {
+		if (lead) st->slot[k4] = prev;
+		return;
+	}
+	Slot cur = prev;
+	cur.fresh = 0;
+	if (!prev.fresh) {
+		const uint32_t nbad = st->bad[kp];
+		uint32_t ws, we, cs, ce;
+		lb_window(a, prev, ws, we, cs, ce);
+		if (prev.phase == 0) {
+			const bool last = prev.j == a.nl - 1;
+			const bool retire = nbad == 0 && (prev.i + 1 < prev.j || last);
+			cur.i = prev.i + (retire ? 1u : 0u);
+			cur.j = prev.j + (last ? 0u : 1u);
+			if (!(cur.i < cur.j) || !(prev.iter + 1 < a.max_iter)) { cur.phase = 1; cur.judge = 0; }
+		} else {
+			cur.judge = prev.judge + 1;
+			cur.done = nbad == 0 || cur.judge >= a.judge_cap;
+		}
+		cur.iter = prev.iter + 1;
+		cur.d = prev.d ^ 1u;
+		if (lead) {
+			if (a.trace) {
+				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
+				r[0] = prev.iter + 1; r[1] = prev.phase ? 0xFFFFFFFFu : prev.i; r[2] = prev.j; r[3] = ws; r[4] = we;
+				r[5] = ce - cs; r[6] = ce - cs - nbad; r[7] = prev.phase;
+			}
+			st->relax += we - ws;
+			st->cond += ce - cs;
+			if (a.fcum) st->f64 += a.fcum[we] - a.fcum[ws];
+			st->sweeps = cur.iter;
+		}
+	}
+	if (lead) {
+		st->slot[k4] = cur;
+		st->bad[(k4 + 1u) & 3u] = 0;
+		if (cur.done) { st->final_d = cur.d; st->done = 1; }
+	}
+	if (cur.done) return;
+	uint32_t ws, we, cs, ce;
+	lb_window(a, cur, ws, we, cs, ce);
+	const float *od = (const float *) (cur.d ? a.dist1 : a.dist0);
+	float *nd = (float *) (cur.d ? a.dist0 : a.dist1);
+	const uint32_t bad = wave_sum(lb_tiles(a, ws, we, cs, ce, od, nd));
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 }
 
@@ -305,14 +547,15 @@ struct gp_plan {
 	gp::DeviceMesh *dm = nullptr;
 	gp_ptp_opts opts{};
 	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
-	uint32_t nchunks = 0, red_steps = 0, n_flagged = 0;
+	uint32_t nchunks = 0, max_slots = 0, judge_cap = 0;
+	unsigned long long n_flagged = 0;
+	bool lb = false;
+	uint16_t *d_lidx = nullptr;
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
-	uint4 *d_rinfo = nullptr;
-	uint32_t *d_cid = nullptr;
+	uint32_t *d_limits = nullptr, *d_coff = nullptr, *d_cid = nullptr;
 	unsigned long long *d_fcum = nullptr;
 	float4 *d_cgeo = nullptr;
-	double4 *d_cgeo64 = nullptr;
-	uint4 *d_meta = nullptr;
+	double4 *d_cgeo64 = nullptr, *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
 	double *d_out = nullptr;
 	gp::PState *d_state = nullptr, *h_state = nullptr;
@@ -323,8 +566,9 @@ struct gp_plan {
 
 	~gp_plan()
 	{
-		for (void *p : { (void *) d_rinfo, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_meta, d_dist0, d_dist1,
-		                 (void *) d_out, (void *) d_state, (void *) d_fcum })
+		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
+		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
+		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_lidx })
 			(void) hipFree(p);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
 		if (g_small) (void) hipGraphExecDestroy(g_small);
@@ -386,43 +630,54 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (rc) return rc;
 	p->dm = use->dev;
 	p->n = use->nv;
+	const bool f64 = opts.precision == GP_PREC_F64;
+	p->lb = opts.schedule != GP_SCHED_RINGS;
+	if (p->lb && (f64 || !opts.coalesce)) return fail(GP_ERR_INVALID, "band schedules run in FP32 on a coalesced mesh");
+	if (p->lb && (opts.local_iters == 0 || opts.local_iters > 1024)) return fail(GP_ERR_INVALID, "local_iters out of range");
 	Layout L;
-	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, L);
+	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, p->lb ? TILE : 0);
 	if (rc) return rc;
 	p->nchunks = L.nchunks;
-	p->red_steps = L.red_steps;
+	p->max_slots = L.max_slots;
 	p->n_flagged = L.n_flagged;
-	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (nl << 1);
+	p->judge_cap = opts.judge_cap ? opts.judge_cap : 64;
+	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (p->lb ? 4 * nl + 64 : (nl << 1));
 	p->block = 256;
-	const bool f64 = opts.precision == GP_PREC_F64;
-	// One wave per chunk up to full occupancy (8 workgroups of 4 waves per CU).
-	int dev = 0, ncu = 0;
+	// One wave per chunk, at most as many workgroups as fit on the device at
+	// once (idle workgroups of small windows then cost no second dispatch round).
+	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	const uint32_t gmax = (uint32_t) ncu * 8u;
+	if (p->lb) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lb_sweep, 256, 0));
+	else if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true>, 256, 0));
+	else GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<false>, 256, 0));
+	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
+	if (p->lb) {
+		if ((rc = upload(p->d_lidx, L.lidx))) return rc;
+	}
 
 	const size_t es = f64 ? sizeof(double) : sizeof(float);
 	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 	GP_HIP(hipEventCreate(&p->e0));
 	GP_HIP(hipEventCreate(&p->e1));
-	std::vector<uint4> rinfo(nl);
-	for (uint32_t r = 0; r < nl; r++) {
-		const uint32_t lr = limits[r];
-		rinfo[r] = make_uint4(lr, lr < nr ? L.pchunk[lr] : L.nchunks, lr > 0 ? L.pchunk[lr - 1] + 1 : 0, 0);
-	}
-	if ((rc = upload(p->d_rinfo, rinfo)) || (rc = upload(p->d_src, srcm)) || (rc = upload(p->d_cid, L.cid)))
+	std::vector<uint32_t> lim(limits, limits + nl);
+	if ((rc = upload(p->d_limits, lim)) || (rc = upload(p->d_src, srcm)) || (rc = upload(p->d_coff, L.coff)) ||
+	    (rc = upload(p->d_cid, L.cid)))
 		return rc;
-	std::vector<float4> cg(L.cgeo.size());
-	std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
-	std::vector<uint4> mt(L.meta.size());
-	std::memcpy(mt.data(), L.meta.data(), sizeof(uint4) * mt.size());
-	if ((rc = upload(p->d_cgeo, cg)) || (rc = upload(p->d_meta, mt))) return rc;
-	if (!f64) {
-		std::vector<double4> cg64(L.cgeo64.size());
-		std::memcpy(cg64.data(), L.cgeo64.data(), sizeof(double4) * cg64.size());
-		if ((rc = upload(p->d_cgeo64, cg64))) return rc;
-		if ((rc = upload(p->d_fcum, L.fcum))) return rc;
+	auto as_d4 = [](const std::vector<Geo64> &v) {
+		std::vector<double4> o(v.size());
+		std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
+		return o;
+	};
+	if (f64) {
+		if ((rc = upload(p->d_cx64, as_d4(L.cx64)))) return rc;
+	} else {
+		std::vector<float4> cg(L.cgeo.size());
+		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+		if ((rc = upload(p->d_cgeo, cg)) || (rc = upload(p->d_cgeo64, as_d4(L.cgeo64))) ||
+		    (rc = upload(p->d_fcum, L.fcum)))
+			return rc;
 	}
 	if (!opts.coalesce) {
 		std::vector<uint32_t> ord(sorted, sorted + nr);
@@ -431,7 +686,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
-	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) p->max_iter) != hipSuccess ||
+	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + p->judge_cap + 1)) != hipSuccess ||
 	    hipHostMalloc(&p->h_state, sizeof(PState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	*out = p.release();
@@ -441,11 +696,12 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 static KArgs make_args(const gp_plan *p)
 {
 	KArgs a{};
-	a.rinfo = p->d_rinfo; a.cid = p->d_cid; a.cgeo = p->d_cgeo; a.cgeo64 = p->d_cgeo64; a.meta = p->d_meta;
-	a.fcum = p->d_fcum; a.order = p->d_order; a.pos = p->dm->pos; a.dist0 = p->d_dist0; a.dist1 = p->d_dist1;
+	a.limits = p->d_limits; a.coff = p->d_coff; a.cid = p->d_cid; a.cgeo = p->d_cgeo; a.cgeo64 = p->d_cgeo64;
+	a.cx64 = p->d_cx64; a.fcum = p->d_fcum; a.order = p->d_order; a.dist0 = p->d_dist0; a.dist1 = p->d_dist1;
 	a.st = p->d_state; a.trace = p->d_trace;
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
-	a.nl = p->nl; a.max_iter = p->max_iter; a.red_steps = p->red_steps;
+	a.nl = p->nl; a.max_iter = p->max_iter;
+	a.lidx = p->d_lidx; a.nr = p->nr; a.local_iters = p->opts.local_iters; a.judge_cap = p->judge_cap;
 	return a;
 }
 
@@ -454,7 +710,8 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 {
 	for (uint32_t q = 0; q < count; q++) {
 		const uint32_t k4 = (k0 + q) & 3u;
-		if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		if (p->lb) k_lb_sweep<<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		else if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		else k_sweep<false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 	}
 }
@@ -500,7 +757,8 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	// per sweep until |limits|-1, so at least nl-1-j sweeps remain: big graphs
 	// while that bound allows, small ones near the end.
 	uint32_t launched = 0, j_seen = 2;
-	while (!done0 && launched <= p->max_iter) {
+	const uint32_t cap = p->max_iter + (p->lb ? p->judge_cap : 0);
+	while (!done0 && launched <= cap) {
 		const bool big = p->nl - 1 > j_seen + BIG;
 		const uint32_t cnt = big ? BIG : SMALL;
 		if (graph) GP_HIP(hipGraphLaunch(big ? p->g_big : p->g_small, p->stream));

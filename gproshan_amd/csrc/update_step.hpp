@@ -16,18 +16,16 @@
 namespace gp {
 
 // The reference's planar-wavefront update of vertex v from the triangle
-// (v, a, b) with arrival times t0 at a and t1 at b (update_step,
-// src/geodesics_ptp.cpp:201-262; cu_update_step, src/cuda/geodesics_ptp.cu:301-369).
-// Same operations in the same order; NaN (det == 0) is returned as NaN and
-// loses every '<' in the caller, as in the reference.
-GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb, double t0, double t1)
+// (v, a, b) with arrival times t0 at a and t1 at b, given the edge vectors
+// X0 = a - v and X1 = b - v (update_step, src/geodesics_ptp.cpp:201-262;
+// cu_update_step, src/cuda/geodesics_ptp.cu:301-369).  Same operations in the
+// same order; NaN (det == 0) is returned as NaN and loses every '<' in the
+// caller, as in the reference.
+GP_HD double update_ref_f64_x(const double X0[3], const double X1[3], double t0, double t1)
 {
 #ifdef __clang__
 #pragma clang fp contract(off)
 #endif
-	const double X0[3] = { ga[0] - gv[0], ga[1] - gv[1], ga[2] - gv[2] };
-	const double X1[3] = { gb[0] - gv[0], gb[1] - gv[1], gb[2] - gv[2] };
-
 	const double q00 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
 	const double q01 = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
 	const double q10 = X1[0] * X0[0] + X1[1] * X0[1] + X1[2] * X0[2];
@@ -62,6 +60,18 @@ GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb
 	return p;
 }
 
+// The same from the three positions: X0 = ga - gv, X1 = gb - gv, as the
+// reference subtracts them (vertex::operator-, src/vertex.cpp).
+GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb, double t0, double t1)
+{
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
+	const double X0[3] = { ga[0] - gv[0], ga[1] - gv[1], ga[2] - gv[2] };
+	const double X1[3] = { gb[0] - gv[0], gb[1] - gv[1], gb[2] - gv[2] };
+	return update_ref_f64_x(X0, X1, t0, t1);
+}
+
 // Per-triangle geometry, precomputed on the host in FP64 in units of the
 // mesh's mean edge length h:  q0 = |X0|^2, q1 = |X1|^2 (from the next fan
 // entry), g = X0.X1, det = |X0 x X1|^2 (cross product, not q0 q1 - g^2),
@@ -78,13 +88,34 @@ GP_HD double update_ref_f64(const double *gv, const double *ga, const double *gb
 // which are the reference's delta, dis, p and c scaled by det (and det^2).
 // T = float is the bulk path; T = double is the fallback for triangles whose
 // angle at the updated vertex is near-degenerate or obtuse.
+// Square root and reciprocal of the stable update.  The FP32 bulk path uses
+// the hardware v_sqrt_f32 / v_rcp_f32 (1 ulp) on the GPU.
+GP_HD float stable_sqrt(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+	return __builtin_amdgcn_sqrtf(x);
+#else
+	return std::sqrt(x);
+#endif
+}
+GP_HD double stable_sqrt(double x) { return sqrt(x); }
+GP_HD float stable_rcp(float x)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+	return __builtin_amdgcn_rcpf(x);
+#else
+	return 1.0f / x;
+#endif
+}
+GP_HD double stable_rcp(double x) { return 1.0 / x; }
+
 template <typename T>
 GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 {
 	const T inf = (T) INFINITY;
 	if (!(t0 != inf && t1 != inf)) {
-		const T dp0 = t0 + sqrt(q0) * h;
-		const T dp1 = t1 + sqrt(q1) * h;
+		const T dp0 = t0 + stable_sqrt(q0) * h;
+		const T dp1 = t1 + stable_sqrt(q1) * h;
 		return (dp1 < dp0) ? dp1 : dp0;
 	}
 	const T m = t0 < t1 ? t0 : t1;
@@ -93,12 +124,12 @@ GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 	const T D = u0 * (q1 - g) + u1 * (q0 - g);
 	const T E = u0 * u0 * q1 - (T) 2 * u0 * u1 * g + u1 * u1 * q0 - det;
 	const T disc = D * D - S * E;
-	const T pp = (D + sqrt(disc)) / S;
+	const T pp = (D + stable_sqrt(disc)) * stable_rcp(S);
 	const T c0 = q1 * (u0 - pp) - g * (u1 - pp);
 	const T c1 = q0 * (u1 - pp) - g * (u0 - pp);
 	if (disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0) {
-		const T dp0 = t0 + sqrt(q0) * h;
-		const T dp1 = t1 + sqrt(q1) * h;
+		const T dp0 = t0 + stable_sqrt(q0) * h;
+		const T dp1 = t1 + stable_sqrt(q1) * h;
 		return (dp1 < dp0) ? dp1 : dp0;
 	}
 	return m + pp * h;

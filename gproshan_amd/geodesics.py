@@ -165,12 +165,19 @@ def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRIT
 class Plan:
     """Device-resident solve (gp_plan_*): buffers stay in HBM between runs."""
 
-    def __init__(self, mesh, sources, **kw):
+    def __init__(self, mesh, sources, dhat=None, **kw):
         self.mesh = mesh
         self._src = _u32(sources)
         o = L.default_opts(**{k: (int(v) if k == "coalesce" else v) for k, v in kw.items()})
         h = C.c_void_p()
-        check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
+        if dhat is None:
+            check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
+        else:
+            d = np.ascontiguousarray(dhat, dtype=np.float64)
+            if d.shape != (mesh.n_vertices(),):
+                raise GPError(L.ERR_INVALID, "dhat must hold one value per vertex")
+            check(lib().gp_plan_create_bands(mesh.handle, ptr(self._src), len(self._src), ptr(d), C.byref(o),
+                                             C.byref(h)))
         self._h = h
 
     def run(self):

@@ -53,11 +53,17 @@ enum gp_result_buffer {
 	GP_RESULT_LAST_READ = 1
 };
 
-/* Band schedule. */
+/* Band schedule.  GP_SCHED_RINGS is the reference's; the band schedules have
+ * no reference counterpart: vertices are grouped into bands of predicted
+ * distance, relaxed tile by tile with local iterations, and finished by judge
+ * sweeps over the whole mesh until nothing changes by tol (they converge to
+ * the fixed point of the relaxation, not to the reference's truncated
+ * schedule). */
 enum gp_schedule {
-	GP_SCHED_RINGS = 0,        /* toplesets (BFS rings), the reference schedule                */
-	GP_SCHED_LEARNED_BAND = 1, /* bands from the learned distance predictor (no reference counterpart) */
-	GP_SCHED_EUCLID_BAND = 2   /* bands from Euclidean distance to the nearest source (ablation) */
+	GP_SCHED_RINGS = 0,        /* toplesets (BFS rings), the reference schedule                     */
+	GP_SCHED_LEARNED_BAND = 1, /* bands of the learned distance predictor (gp_mlp_*)                */
+	GP_SCHED_EUCLID_BAND = 2,  /* bands of the Euclidean distance to the nearest source (ablation)  */
+	GP_SCHED_RING_BAND = 3     /* bands of band_width consecutive BFS rings (ablation: no predictor) */
 };
 
 /* How the sweeps are launched. */
@@ -76,6 +82,9 @@ typedef struct gp_ptp_opts {
 	uint32_t block;     /* threads per workgroup (0: autotuned/default)          */
 	uint32_t grid;      /* workgroups (0: autotuned/default)                     */
 	int launch_mode;    /* gp_launch_mode                                        */
+	double band_width;  /* band schedules: band width in mean edge lengths       */
+	uint32_t local_iters;/* band schedules: Jacobi iterations per tile per sweep */
+	uint32_t judge_cap; /* band schedules: maximum judge sweeps                  */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {
@@ -183,10 +192,16 @@ int gp_normalize_ptp(double *dist, uint32_t n);
 int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_ptp_opts *opts,
            double *dist, uint32_t *sorted_out, gp_ptp_stats *stats);
 
+/* Band-schedule plan from caller-supplied predicted distances dhat[n_vertices]
+ * (opts->schedule must be a band schedule; the bands are 1 + floor(dhat /
+ * (band_width * mean edge length)), sources form band 0). */
+typedef struct gp_plan gp_plan;
+int gp_plan_create_bands(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                         const gp_ptp_opts *opts, gp_plan **out);
+
 /* Device-resident solver plan: mesh tables, bands and buffers live in HBM;
  * gp_plan_run() reruns the sweeps on them without host traffic (the timed
  * region of the benchmarks). */
-typedef struct gp_plan gp_plan;
 int gp_plan_create(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
                    const gp_ptp_opts *opts, gp_plan **out);
 int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats);
