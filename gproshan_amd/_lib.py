@@ -32,7 +32,11 @@ EXPORTS = [
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
     "gp_normalize_ptp", "gp_ptp",
     "gp_plan_create", "gp_plan_create_bands", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
+    "gp_mlp_create", "gp_mlp_destroy", "gp_mlp_params", "gp_mlp_features", "gp_mlp_forward", "gp_mlp_train",
+    "gp_mlp_predict", "gp_plan_create_learned",
 ]
+MLP_NPARAM = 6337
+MLP_NFEAT = 10
 
 
 class GPError(RuntimeError):
@@ -95,6 +99,15 @@ def lib() -> C.CDLL:
     L.gp_plan_trace.argtypes = [P, P, u32, C.POINTER(u32)]
     L.gp_plan_destroy.argtypes = [P]
     L.gp_plan_destroy.restype = None
+    L.gp_mlp_create.argtypes = [u32, C.POINTER(P)]
+    L.gp_mlp_destroy.argtypes = [P]
+    L.gp_mlp_destroy.restype = None
+    L.gp_mlp_params.argtypes = [P, P, P, u32]
+    L.gp_mlp_features.argtypes = [P, P, u32, P, P]
+    L.gp_mlp_forward.argtypes = [P, P, u32, P]
+    L.gp_mlp_train.argtypes = [P, P, P, u32, u32, u32, C.c_float, u32, P, C.POINTER(C.c_float)]
+    L.gp_mlp_predict.argtypes = [P, P, P, u32, P, C.POINTER(C.c_float)]
+    L.gp_plan_create_learned.argtypes = [P, P, u32, P, C.POINTER(PtpOpts), C.POINTER(P)]
     _lib = L
     return L
 

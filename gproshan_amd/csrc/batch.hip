@@ -1,0 +1,301 @@
+// Batched PTP: many (mesh, source) fields of uneven size in one persistent
+// launch (captured in a hipGraph with its queue reset).
+//
+// Each 1024-thread workgroup repeatedly takes the next job from an atomic
+// queue (jobs sorted by decreasing cost V^1.5, so the grid drains evenly), and
+// runs it entirely by itself: the ring build (the same BFS as the toplesets
+// kernel), then the reference window schedule (K/2 band limit, PTP_TOL test,
+// 2*|limits| cap, last-written buffer) with __syncthreads as the only barrier,
+// relaxing vertices through the mesh's ELL slot layout (original numbering,
+// sorted[] indirection as in the reference's multi-source driver,
+// src/cuda/geodesics_ptp.cu:174-299).  Every workgroup leaves the loop when the
+// queue is exhausted, so the launch always drains.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "device.hpp"
+#include "rings_dev.hpp"
+#include "sweep_dev.hpp"
+
+namespace gp {
+
+struct BMesh {
+	const uint32_t *fan_off, *fan;
+	const uint8_t *closed;
+	const uint32_t *coff, *cid;
+	const float4 *cgeo;
+	const double4 *cgeo64;
+	const double4 *cx64;
+	float h, inv_h;
+	uint32_t nv, pad;
+};
+
+struct BJob {
+	uint32_t mesh, source, sweeps, pad;
+	unsigned long long out, relax;
+};
+
+struct BArgs {
+	const BMesh *meshes;
+	BJob *jobs;
+	uint32_t njobs, max_nv;
+	uint32_t *queue;
+	uint32_t *level, *sorted, *limits, *nlim;
+	unsigned long long *key;
+	float *dist;
+	float *out;
+	double tol;
+};
+
+constexpr uint32_t BATCH_THREADS = 1024;
+
+__global__ void k_batch_reset(uint32_t *queue) { *queue = 0; }
+
+__global__ __launch_bounds__(BATCH_THREADS) void k_batch(BArgs a)
+{
+	__shared__ uint32_t s_job, s_bad;
+	const uint32_t tid = threadIdx.x, nt = blockDim.x, wg = blockIdx.x;
+	const size_t mv = a.max_nv;
+	uint32_t *level = a.level + wg * mv;
+	uint32_t *sorted = a.sorted + wg * mv;
+	uint32_t *limits = a.limits + wg * (mv + 2);
+	unsigned long long *key = a.key + wg * mv;
+	float *d0 = a.dist + 2 * wg * mv, *d1 = d0 + mv;
+	for (;;) {
+		if (tid == 0) s_job = atomicAdd(a.queue, 1u);
+		__syncthreads();
+		const uint32_t job = s_job;
+		__syncthreads();
+		if (job >= a.njobs) break;
+		const uint32_t src = a.jobs[job].source;
+		const BMesh M = a.meshes[a.jobs[job].mesh];
+		const uint32_t nv = M.nv;
+		for (uint32_t v = tid; v < nv; v += nt) {
+			level[v] = GP_NIL; key[v] = ~0ull; sorted[v] = GP_NIL;
+			d0[v] = INFINITY; d1[v] = INFINITY;
+		}
+		__syncthreads();
+		if (tid == 0) { d0[src] = 0.f; d1[src] = 0.f; }
+		const BfsJob bj{ &a.jobs[job].source, 1u, level, key, sorted, limits, a.nlim + wg };
+		bfs_block(M.fan_off, M.fan, M.closed, bj);
+		const uint32_t nl = __hip_atomic_load(a.nlim + wg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t i = 1, j = 2, d = 0, iter = 0;
+		unsigned long long relax = 0;
+		const uint32_t max_iter = nl << 1;
+		bool done = nl < 3;
+		while (!done) {
+			const uint32_t ii = i < (j >> 1) ? (j >> 1) : i;
+			const uint32_t start = limits[ii], end = limits[j], cend = limits[ii + 1];
+			const float *od = d ? d1 : d0;
+			float *nd = d ? d0 : d1;
+			if (tid == 0) s_bad = 0;
+			__syncthreads();
+			uint32_t bad = 0;
+			for (uint32_t pos = start + tid; pos < end; pos += nt) {
+				const uint32_t v = sorted[pos];
+				const uint32_t c = v >> 6, cb = M.coff[c];
+				const float old = od[v];
+				const float best = relax_lane<false>(M, (size_t) cb * 64 + (v & 63u), M.coff[c + 1] - cb, od, old);
+				nd[v] = best;
+				if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+			}
+			for (uint32_t o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+			if ((tid & 63u) == 0 && bad) atomicAdd(&s_bad, bad);
+			__syncthreads();
+			const uint32_t nbad = s_bad;
+			relax += end - start;
+			i = ii + (nbad == 0 ? 1u : 0u);
+			if (j < nl - 1) j++;
+			d ^= 1u;
+			iter++;
+			done = !(i < j) || !(iter < max_iter);
+			__syncthreads();
+		}
+		const float *res = d ? d1 : d0;            // the buffer the last sweep wrote
+		float *out = a.out + a.jobs[job].out;
+		for (uint32_t v = tid; v < nv; v += nt) out[v] = res[v];
+		if (tid == 0) { a.jobs[job].sweeps = iter; a.jobs[job].relax = relax; }
+		__syncthreads();
+	}
+}
+
+}  // namespace gp
+
+// ---------------------------------------------------------------------------
+
+struct gp_batch {
+	struct MeshTables {
+		uint32_t *coff = nullptr, *cid = nullptr;
+		float4 *cgeo = nullptr;
+		double4 *cgeo64 = nullptr;
+	};
+	std::vector<MeshTables> tabs;
+	std::vector<uint32_t> order;          // internal job slot -> user job index
+	std::vector<uint32_t> slot_of;        // user job -> internal slot
+	std::vector<unsigned long long> out_off;
+	std::vector<uint32_t> job_nv;
+	gp::BMesh *d_meshes = nullptr;
+	gp::BJob *d_jobs = nullptr;
+	uint32_t *d_queue = nullptr, *d_level = nullptr, *d_sorted = nullptr, *d_limits = nullptr, *d_nlim = nullptr;
+	unsigned long long *d_key = nullptr;
+	float *d_dist = nullptr, *d_out = nullptr;
+	uint32_t njobs = 0, max_nv = 0, grid = 0;
+	unsigned long long total_out = 0;
+	double tol = GP_PTP_TOL;
+	hipStream_t stream = nullptr;
+	hipGraphExec_t graph = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+
+	~gp_batch()
+	{
+		for (auto &t : tabs)
+			for (void *p : { (void *) t.coff, (void *) t.cid, (void *) t.cgeo, (void *) t.cgeo64 }) (void) hipFree(p);
+		for (void *p : { (void *) d_meshes, (void *) d_jobs, (void *) d_queue, (void *) d_level, (void *) d_sorted,
+		                 (void *) d_limits, (void *) d_nlim, (void *) d_key, (void *) d_dist, (void *) d_out })
+			(void) hipFree(p);
+		if (graph) (void) hipGraphExecDestroy(graph);
+		if (e0) (void) hipEventDestroy(e0);
+		if (e1) (void) hipEventDestroy(e1);
+		if (stream) (void) hipStreamDestroy(stream);
+	}
+};
+
+namespace gp {
+
+template <typename V>
+static int up(V *&dst, const V *src, size_t n)
+{
+	if (hipMalloc(&dst, sizeof(V) * std::max<size_t>(n, 1)) != hipSuccess) return fail(GP_ERR_NOMEM, "batch: allocation failed");
+	if (n) GP_HIP(hipMemcpy(dst, src, sizeof(V) * n, hipMemcpyHostToDevice));
+	return GP_OK;
+}
+
+int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh, const uint32_t *job_source,
+                 uint32_t njobs, double tol, gp_batch **out)
+{
+	*out = nullptr;
+	if (!meshes || !nmeshes || !job_mesh || !job_source || !njobs) return fail(GP_ERR_INVALID, "batch: empty input");
+	auto b = std::make_unique<gp_batch>();
+	b->njobs = njobs;
+	b->tol = tol;
+	std::vector<BMesh> bm(nmeshes);
+	b->tabs.resize(nmeshes);
+	for (uint32_t k = 0; k < nmeshes; k++) {
+		Che &m = *meshes[k];
+		int rc = upload_mesh(m);
+		if (rc) return rc;
+		for (uint32_t v = 0; v < m.nv; v++)
+			if (m.fan_off[v + 1] - m.fan_off[v] > MAX_LINK) return fail(GP_ERR_CAPACITY, "batch: vertex link over %u", MAX_LINK);
+		Layout L;
+		rc = build_layout(m, nullptr, m.nv, false, L);
+		if (rc) return rc;
+		auto &t = b->tabs[k];
+		std::vector<float4> cg(L.cgeo.size());
+		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+		std::vector<double4> cg64(L.cgeo64.size());
+		std::memcpy(cg64.data(), L.cgeo64.data(), sizeof(double4) * cg64.size());
+		if ((rc = up(t.coff, L.coff.data(), L.coff.size())) || (rc = up(t.cid, L.cid.data(), L.cid.size())) ||
+		    (rc = up(t.cgeo, cg.data(), cg.size())) || (rc = up(t.cgeo64, cg64.data(), cg64.size())))
+			return rc;
+		bm[k] = BMesh{ m.dev->fan_off, m.dev->fan, m.dev->closed, t.coff, t.cid, t.cgeo, t.cgeo64, nullptr,
+		               m.dev->h, m.dev->inv_h, m.nv, 0 };
+		b->max_nv = std::max(b->max_nv, m.nv);
+	}
+	// Longest jobs first.
+	b->order.resize(njobs);
+	std::iota(b->order.begin(), b->order.end(), 0u);
+	for (uint32_t q = 0; q < njobs; q++) {
+		if (job_mesh[q] >= nmeshes) return fail(GP_ERR_INVALID, "batch: job %u mesh out of range", q);
+		if (job_source[q] >= meshes[job_mesh[q]]->nv) return fail(GP_ERR_INVALID, "batch: job %u source out of range", q);
+	}
+	std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
+		return meshes[job_mesh[x]]->nv > meshes[job_mesh[y]]->nv;
+	});
+	b->slot_of.resize(njobs);
+	b->out_off.resize(njobs);
+	b->job_nv.resize(njobs);
+	std::vector<BJob> jobs(njobs);
+	unsigned long long off = 0;
+	for (uint32_t s = 0; s < njobs; s++) {
+		const uint32_t q = b->order[s];
+		b->slot_of[q] = s;
+		jobs[s] = BJob{ job_mesh[q], job_source[q], 0, 0, off, 0 };
+		b->out_off[q] = off;
+		b->job_nv[q] = meshes[job_mesh[q]]->nv;
+		off += meshes[job_mesh[q]]->nv;
+	}
+	b->total_out = off;
+	int rc;
+	if ((rc = up(b->d_meshes, bm.data(), bm.size())) || (rc = up(b->d_jobs, jobs.data(), jobs.size()))) return rc;
+	int dev = 0, ncu = 0, occ = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch, BATCH_THREADS, 0));
+	b->grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t) ncu * (uint32_t) std::max(1, occ)));
+	const size_t mv = b->max_nv, g = b->grid;
+	if (hipMalloc(&b->d_queue, 4) != hipSuccess || hipMalloc(&b->d_level, 4 * mv * g) != hipSuccess ||
+	    hipMalloc(&b->d_sorted, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_limits, 4 * (mv + 2) * g) != hipSuccess ||
+	    hipMalloc(&b->d_nlim, 4 * g) != hipSuccess || hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess ||
+	    hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_out, 4 * std::max<size_t>(off, 1)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "batch: scratch allocation failed");
+	GP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+	GP_HIP(hipEventCreate(&b->e0));
+	GP_HIP(hipEventCreate(&b->e1));
+	// One graph: queue reset + the persistent launch.
+	BArgs a{ b->d_meshes, b->d_jobs, njobs, b->max_nv, b->d_queue, b->d_level, b->d_sorted, b->d_limits, b->d_nlim,
+	         b->d_key, b->d_dist, b->d_out, tol };
+	hipGraph_t gr = nullptr;
+	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+	k_batch_reset<<<1, 1, 0, b->stream>>>(b->d_queue);
+	k_batch<<<b->grid, BATCH_THREADS, 0, b->stream>>>(a);
+	GP_HIP(hipStreamEndCapture(b->stream, &gr));
+	const hipError_t e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(gr);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "batch: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	*out = b.release();
+	return GP_OK;
+}
+
+int batch_run(gp_batch *b, double *ms, unsigned long long *sweeps, unsigned long long *relax)
+{
+	if (!b) return fail(GP_ERR_INVALID, "batch is null");
+	GP_HIP(hipEventRecord(b->e0, b->stream));
+	GP_HIP(hipGraphLaunch(b->graph, b->stream));
+	GP_HIP(hipEventRecord(b->e1, b->stream));
+	GP_HIP(hipStreamSynchronize(b->stream));
+	float t = 0;
+	GP_HIP(hipEventElapsedTime(&t, b->e0, b->e1));
+	if (ms) *ms = t;
+	if (sweeps || relax) {
+		std::vector<BJob> jobs(b->njobs);
+		GP_HIP(hipMemcpy(jobs.data(), b->d_jobs, sizeof(BJob) * b->njobs, hipMemcpyDeviceToHost));
+		unsigned long long s = 0, r = 0;
+		for (auto &j : jobs) { s += j.sweeps; r += j.relax; }
+		if (sweeps) *sweeps = s;
+		if (relax) *relax = r;
+	}
+	return GP_OK;
+}
+
+int batch_result(gp_batch *b, uint32_t job, double *dist, uint32_t cap)
+{
+	if (!b || !dist) return fail(GP_ERR_INVALID, "null argument");
+	if (job >= b->njobs) return fail(GP_ERR_INVALID, "job out of range");
+	const uint32_t n = b->job_nv[job];
+	if (cap < n) return fail(GP_ERR_CAPACITY, "result needs %u entries", n);
+	std::vector<float> tmp(n);
+	GP_HIP(hipMemcpy(tmp.data(), b->d_out + b->out_off[job], 4ull * n, hipMemcpyDeviceToHost));
+	for (uint32_t v = 0; v < n; v++) dist[v] = tmp[v];
+	return GP_OK;
+}
+
+}  // namespace gp
+
+extern "C" void gp_batch_destroy(gp_batch *b) { delete b; }

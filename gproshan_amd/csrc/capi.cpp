@@ -321,4 +321,74 @@ int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap, uint32_t *n)
 }
 void gp_plan_destroy(gp_plan *plan);
 
+int gp_mlp_create(uint32_t seed, gp_mlp **out)
+{
+	GP_GUARD(
+		if (!out) return gp::fail(GP_ERR_INVALID, "null argument");
+		return gp::mlp_create(seed, out);
+	)
+}
+
+int gp_mlp_params(gp_mlp *mlp, float *get, const float *set, uint32_t n)
+{
+	GP_GUARD(
+		if (!mlp) return gp::fail(GP_ERR_INVALID, "null argument");
+		return gp::mlp_params(mlp, get, set, n);
+	)
+}
+
+int gp_mlp_features(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, float *features, float *euclid)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !n_sources) return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<uint32_t> lev(mesh->m.nv), srt(mesh->m.nv), lim;
+		int rc = gp::toplesets_host(mesh->m, sources, n_sources, lev.data(), srt.data(), lim);
+		if (rc) return rc;
+		return gp::mlp_features(mesh->m, sources, n_sources, lev.data(), features, euclid);
+	)
+}
+
+int gp_mlp_forward(gp_mlp *mlp, const float *features, uint32_t n, float *y)
+{
+	GP_GUARD(
+		if (!mlp || !features || !y || !n) return gp::fail(GP_ERR_INVALID, "null argument");
+		return gp::mlp_forward(mlp, features, n, y);
+	)
+}
+
+int gp_mlp_train(gp_mlp *mlp, const float *features, const float *target, uint32_t n, uint32_t steps,
+                 uint32_t batch, float lr, uint32_t seed, float *grad, float *loss)
+{
+	GP_GUARD(
+		if (!mlp || !features || !target) return gp::fail(GP_ERR_INVALID, "null argument");
+		return gp::mlp_train(mlp, features, target, n, steps, batch, lr, seed, grad, loss);
+	)
+}
+
+int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t n_sources, double *dhat, float *ms)
+{
+	GP_GUARD(
+		if (!mlp || !mesh || !sources || !n_sources || !dhat) return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<uint32_t> lev(mesh->m.nv), srt(mesh->m.nv), lim;
+		int rc = gp::toplesets_host(mesh->m, sources, n_sources, lev.data(), srt.data(), lim);
+		if (rc) return rc;
+		return gp::mlp_predict(mlp, mesh->m, sources, n_sources, lev.data(), dhat, ms);
+	)
+}
+
+int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, gp_mlp *mlp,
+                           const gp_ptp_opts *opts, gp_plan **out)
+{
+	GP_GUARD(
+		if (!mlp || !mesh || !sources || !n_sources || !out) return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<double> dhat(mesh->m.nv);
+		int rc = gp_mlp_predict(mlp, mesh, sources, n_sources, dhat.data(), nullptr);
+		if (rc) return rc;
+		gp_ptp_opts o;
+		if (opts) o = *opts; else gp_ptp_opts_default(&o);
+		o.schedule = GP_SCHED_LEARNED_BAND;
+		return plan_create_any(mesh, sources, n_sources, dhat.data(), o, out);
+	)
+}
+
 }  // extern "C"

@@ -132,4 +132,15 @@ int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n);
 const char *last_error();
 void clear_error();
 
+// Learned band predictor (mlp.hip).
+constexpr uint32_t MLP_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1;
+int mlp_create(uint32_t seed, gp_mlp **out);
+int mlp_params(gp_mlp *m, float *get, const float *set, uint32_t n);
+int mlp_features(Che &mesh, const uint32_t *src, uint32_t ns, const uint32_t *levels, float *F_out, float *e_out);
+int mlp_forward(gp_mlp *m, const float *F, uint32_t n, float *y);
+int mlp_train(gp_mlp *m, const float *F, const float *t, uint32_t n, uint32_t steps, uint32_t batch, float lr,
+              uint32_t seed, float *grad_out, float *loss_out);
+int mlp_predict(gp_mlp *m, Che &mesh, const uint32_t *src, uint32_t ns, const uint32_t *levels, double *dhat,
+                float *ms_out);
+
 }  // namespace gp

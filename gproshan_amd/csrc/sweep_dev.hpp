@@ -1,0 +1,109 @@
+// Per-vertex relaxation on the ELL slot layout, shared by the sweep kernels
+// (ptp_gpu.hip) and the batched solver (batch.hip).  A is any argument struct
+// with the fields cid, cgeo, cgeo64, cx64, h, inv_h.
+#pragma once
+
+#include <type_traits>
+
+#include "device.hpp"
+#include "update_step.hpp"
+
+namespace gp {
+
+constexpr uint32_t MAXS = 10;  // register-resident slots; longer fans take the streaming loop
+
+// FP64 update of triangle (k, k+1) of the lane at slot base (flagged triangles).
+template <typename A>
+__device__ __forceinline__ float tri_flagged(const A &a, size_t at, float ta, float tb)
+{
+	const double4 gd = a.cgeo64[at];
+	const double q1 = a.cgeo64[at + 64].x;
+	return (float) update_stable<double>(gd.x, q1, gd.y, gd.z, gd.w, (double) ta, (double) tb, (double) a.h,
+	                                     (double) a.inv_h);
+}
+
+__device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, double ta, double tb)
+{
+	const double X0[3] = { xa.x, xa.y, xa.z };
+	const double X1[3] = { xb.x, xb.y, xb.z };
+	return update_ref_f64_x(X0, X1, ta, tb);
+}
+
+// New value of the vertex at lane slot base (chunk with S slots).  FP32:
+// flagged triangles are skipped in the slot loop and done afterwards in a
+// loop that runs as many times as the most-flagged lane of the wave needs,
+// so the FP64 code runs ~1-2 times per wave instead of once per slot.
+template <bool F64, typename A>
+__device__ __forceinline__ typename std::conditional<F64, double, float>::type
+relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
+           typename std::conditional<F64, double, float>::type best)
+{
+	using T = typename std::conditional<F64, double, float>::type;
+	using G = typename std::conditional<F64, double4, float4>::type;
+	const G *geo;
+	if constexpr (F64) geo = a.cx64; else geo = a.cgeo;
+	uint32_t fmask = 0;
+	if (S <= MAXS) {
+		uint32_t id[MAXS];
+		G g[MAXS];
+		T t[MAXS];
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++) {
+			id[k] = SLOT_EMPTY;
+			if (k < S) { id[k] = a.cid[base + 64 * k]; g[k] = geo[base + 64 * k]; }
+		}
+#pragma unroll
+		for (uint32_t k = 0; k < MAXS; k++)
+			t[k] = (k < S) ? od[id[k] & ID30_MASK] : (T) INFINITY;
+#pragma unroll
+		for (uint32_t k = 0; k + 1 < MAXS; k++) {
+			if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
+				if constexpr (F64) {
+					const T p = tri_f64(g[k], g[k + 1], t[k], t[k + 1]);
+					if (p < best) best = p;
+				} else {
+					if (id[k] & F64_FLAG) {
+						fmask |= 1u << k;
+					} else {
+						const T p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1],
+						                                 a.h, a.inv_h);
+						if (p < best) best = p;
+					}
+				}
+			}
+		}
+	} else {
+		uint32_t ida = a.cid[base];
+		G ga = geo[base];
+		T ta = od[ida & ID30_MASK];
+		for (uint32_t k = 1; k < S; k++) {
+			const uint32_t idb = a.cid[base + 64 * k];
+			if (idb & SLOT_EMPTY) break;
+			const G gb = geo[base + 64 * k];
+			const T tb = od[idb & ID30_MASK];
+			if constexpr (F64) {
+				const T p = tri_f64(ga, gb, ta, tb);
+				if (p < best) best = p;
+			} else {
+				const T p = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
+				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
+				if (p < best) best = p;
+			}
+			ida = idb; ga = gb; ta = tb;
+		}
+	}
+	if constexpr (!F64) {
+		while (fmask) {
+			const uint32_t k = __builtin_ctz(fmask);
+			fmask &= fmask - 1u;
+			const size_t at = base + 64 * (size_t) k;
+			const float ta = od[a.cid[at] & ID30_MASK];
+			const float tb = od[a.cid[at + 64] & ID30_MASK];
+			const float p = tri_flagged(a, at, ta, tb);
+			if (p < best) best = p;
+		}
+	}
+	return best;
+}
+
+}  // namespace gp

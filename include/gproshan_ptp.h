@@ -209,6 +209,38 @@ int gp_plan_result(gp_plan *plan, double *dist);
 int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap_sweeps, uint32_t *n_sweeps);
 void gp_plan_destroy(gp_plan *plan);
 
+/* ---- learned band predictor (no reference counterpart) ------------------ */
+
+/* A 10 -> 64 -> 64 -> 1 ReLU MLP on per-vertex features, BF16 weights and
+ * activations with FP32 accumulation (gfx950 MFMA), FP32 master weights and
+ * Adam.  It predicts y = log(geodesic / Euclidean distance to the nearest
+ * source); the band schedule orders vertices by dhat = e * exp(y). */
+typedef struct gp_mlp gp_mlp;
+#define GP_MLP_NPARAM 6337u  /* W1[64][32] (inputs padded to 32), b1[64], W2[64][64], b2[64], w3[64], b3 */
+#define GP_MLP_NFEAT 10u
+
+int gp_mlp_create(uint32_t seed, gp_mlp **out);
+void gp_mlp_destroy(gp_mlp *mlp);
+/* Copy the FP32 parameters out (get != NULL) and/or in (set != NULL); n == GP_MLP_NPARAM. */
+int gp_mlp_params(gp_mlp *mlp, float *get, const float *set, uint32_t n);
+/* Per-vertex features [n_vertices][GP_MLP_NFEAT] and Euclidean distance to the
+ * nearest source, computed on the GPU. */
+int gp_mlp_features(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, float *features, float *euclid);
+/* y[n] for feature rows F[n][GP_MLP_NFEAT]. */
+int gp_mlp_forward(gp_mlp *mlp, const float *features, uint32_t n, float *y);
+/* steps == 0: the gradient of mean((y - target)^2) over all n rows into
+ * grad[GP_MLP_NPARAM] (no update).  steps > 0: that many Adam steps on
+ * minibatches of `batch` rows (cosine-decayed lr).  *loss: last batch loss. */
+int gp_mlp_train(gp_mlp *mlp, const float *features, const float *target, uint32_t n, uint32_t steps,
+                 uint32_t batch, float lr, uint32_t seed, float *grad, float *loss);
+/* dhat[n_vertices] for a mesh and sources (features + inference on the GPU);
+ * *ms: device time of features + inference (may be NULL). */
+int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t n_sources, double *dhat,
+                   float *ms);
+/* Learned-band plan: gp_mlp_predict, then gp_plan_create_bands. */
+int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, gp_mlp *mlp,
+                           const gp_ptp_opts *opts, gp_plan **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,59 @@
+// Exercises the C++ drop-in header (include/gproshan/geodesics.h) the way the
+// reference's callers use it (src/app_viewer.cpp:657-716).  Prints one line per
+// check; argv[1] == "gpu" also runs PTP_GPU.
+#include <gproshan/geodesics.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace gproshan;
+
+int main(int argc, char **argv)
+{
+	const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+	// 20 x 20 planar grid, CCW triangles.
+	const index_t n = 20;
+	std::vector<vertex> V;
+	std::vector<index_t> F;
+	for (index_t r = 0; r < n; r++)
+		for (index_t c = 0; c < n; c++) V.push_back({ c / (n - 1.0), r / (n - 1.0), 0.0 });
+	for (index_t r = 0; r + 1 < n; r++)
+		for (index_t c = 0; c + 1 < n; c++) {
+			const index_t a = r * n + c, b = a + 1, d = a + n, e = d + 1;
+			F.insert(F.end(), { a, b, e, a, e, d });
+		}
+	che mesh(V.data(), (index_t) V.size(), F.data(), (index_t) F.size() / 3);
+	std::printf("mesh %zu vertices %zu faces %zu edges manifold %d\n", mesh.n_vertices(), mesh.n_faces(),
+	            mesh.n_edges(), (int) mesh.is_manifold());
+	const std::vector<index_t> src = { (n / 2) * n + n / 2 };
+
+	geodesics cpu(&mesh, src, geodesics::PTP_CPU);
+	double maxerr = 0;
+	for (index_t v = 0; v < mesh.n_vertices(); v++) {
+		const double eu = std::hypot(V[v].x - V[src[0]].x, V[v].y - V[src[0]].y);
+		if (eu > 0.3) maxerr = std::fmax(maxerr, std::fabs(cpu[v] - eu) / eu);
+	}
+	std::printf("ptp_cpu far-field max rel err vs euclid %.4f sorted0 %u\n", maxerr, cpu(0));
+	int ok = maxerr < 0.1 && cpu(0) == src[0];
+
+	try {
+		geodesics bad(&mesh, { 3, 3 }, geodesics::PTP_CPU);
+		ok = 0;
+	} catch (const ptp_error &e) {
+		std::printf("duplicate sources -> status %d\n", e.status);
+		ok &= e.status == GP_ERR_DUPLICATE_SOURCE;
+	}
+
+	if (gpu) {
+		std::vector<distance_t> ext(mesh.n_vertices());
+		geodesics g(&mesh, src, geodesics::PTP_GPU, ext.data());
+		double d = 0;
+		for (index_t v = 0; v < mesh.n_vertices(); v++) d = std::fmax(d, std::fabs(ext[v] - cpu[v]) / std::fmax(cpu[v], 1e-30));
+		std::printf("ptp_gpu vs ptp_cpu max rel diff %.3e\n", d);
+		ok &= d < 2e-3;   // last-written (GPU) vs last-read (CPU) buffer, FP32 vs FP64
+	}
+	std::printf(ok ? "OK\n" : "FAILED\n");
+	return ok ? 0 : 1;
+}
