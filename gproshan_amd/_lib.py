@@ -20,7 +20,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DUPLICATE_SOURCE, ERR_HIP, ERR_NO_DEVICE, ERR_CA
     0, -1, -2, -3, -4, -5, -6, -7
 PREC_F32, PREC_F64 = 0, 1
 RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
-SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND, SCHED_RING_BAND = 0, 1, 2, 3
+SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND, SCHED_RING_BAND, SCHED_PATCHES = 0, 1, 2, 3, 4
 LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
 
 # Every symbol include/gproshan_ptp.h declares.
@@ -34,6 +34,7 @@ EXPORTS = [
     "gp_plan_create", "gp_plan_create_bands", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
     "gp_mlp_create", "gp_mlp_destroy", "gp_mlp_params", "gp_mlp_features", "gp_mlp_forward", "gp_mlp_train",
     "gp_mlp_predict", "gp_plan_create_learned",
+    "gp_batch_create", "gp_batch_run", "gp_batch_result", "gp_batch_destroy",
 ]
 MLP_NPARAM = 6337
 MLP_NFEAT = 10
@@ -49,13 +50,21 @@ class PtpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
                 ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int),
-                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("judge_cap", C.c_uint32)]
+                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int)]
 
 
 class PtpStats(C.Structure):
     _fields_ = [("sweeps", C.c_uint32), ("n_limits", C.c_uint32), ("relaxations", C.c_uint64),
-                ("cond_checks", C.c_uint64), ("f64_updates", C.c_uint64), ("kernel_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("cond_checks", C.c_uint64), ("f64_updates", C.c_uint64), ("patch_visits", C.c_uint64),
+                ("deferred", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("sweeps", C.c_uint64), ("relaxations", C.c_uint64), ("vertex_fields", C.c_uint64),
+                ("block", C.c_uint32), ("grid", C.c_uint32), ("kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +117,11 @@ def lib() -> C.CDLL:
     L.gp_mlp_train.argtypes = [P, P, P, u32, u32, u32, C.c_float, u32, P, C.POINTER(C.c_float)]
     L.gp_mlp_predict.argtypes = [P, P, P, u32, P, C.POINTER(C.c_float)]
     L.gp_plan_create_learned.argtypes = [P, P, u32, P, C.POINTER(PtpOpts), C.POINTER(P)]
+    L.gp_batch_create.argtypes = [P, u32, P, P, u32, C.c_double, u32, C.POINTER(P)]
+    L.gp_batch_run.argtypes = [P, C.POINTER(BatchStats)]
+    L.gp_batch_result.argtypes = [P, u32, P, u32, C.POINTER(u32)]
+    L.gp_batch_destroy.argtypes = [P]
+    L.gp_batch_destroy.restype = None
     _lib = L
     return L
 

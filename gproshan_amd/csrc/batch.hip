@@ -54,14 +54,14 @@ struct BArgs {
 	double tol;
 };
 
-constexpr uint32_t BATCH_THREADS = 1024;
-
 __global__ void k_batch_reset(uint32_t *queue) { *queue = 0; }
 
-__global__ __launch_bounds__(BATCH_THREADS) void k_batch(BArgs a)
+// NT threads per field (a multiple of 64, at most 1024).
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void k_batch(BArgs a)
 {
 	__shared__ uint32_t s_job, s_bad;
-	const uint32_t tid = threadIdx.x, nt = blockDim.x, wg = blockIdx.x;
+	const uint32_t tid = threadIdx.x, nt = NT, wg = blockIdx.x;
 	const size_t mv = a.max_nv;
 	uint32_t *level = a.level + wg * mv;
 	uint32_t *sorted = a.sorted + wg * mv;
@@ -137,30 +137,41 @@ struct gp_batch {
 		double4 *cgeo64 = nullptr;
 	};
 	std::vector<MeshTables> tabs;
-	std::vector<uint32_t> order;          // internal job slot -> user job index
-	std::vector<uint32_t> slot_of;        // user job -> internal slot
-	std::vector<unsigned long long> out_off;
-	std::vector<uint32_t> job_nv;
+	std::vector<unsigned long long> out_off;   // user job -> offset of its field
+	std::vector<uint32_t> job_nv;              // user job -> vertices
+	std::vector<uint32_t> slot_of;             // user job -> queue slot
 	gp::BMesh *d_meshes = nullptr;
 	gp::BJob *d_jobs = nullptr;
-	uint32_t *d_queue = nullptr, *d_level = nullptr, *d_sorted = nullptr, *d_limits = nullptr, *d_nlim = nullptr;
+	uint32_t *d_queue = nullptr;
+	float *d_out = nullptr;
+	// Per-workgroup scratch, sized for the current configuration.
+	uint32_t *d_level = nullptr, *d_sorted = nullptr, *d_limits = nullptr, *d_nlim = nullptr;
 	unsigned long long *d_key = nullptr;
-	float *d_dist = nullptr, *d_out = nullptr;
-	uint32_t njobs = 0, max_nv = 0, grid = 0;
+	float *d_dist = nullptr;
+	uint32_t njobs = 0, max_nv = 0, grid = 0, block = 0;
 	unsigned long long total_out = 0;
 	double tol = GP_PTP_TOL;
 	hipStream_t stream = nullptr;
 	hipGraphExec_t graph = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 
+	void free_scratch()
+	{
+		for (void *p : { (void *) d_level, (void *) d_sorted, (void *) d_limits, (void *) d_nlim, (void *) d_key,
+		                 (void *) d_dist })
+			(void) hipFree(p);
+		d_level = d_sorted = d_limits = d_nlim = nullptr;
+		d_key = nullptr;
+		d_dist = nullptr;
+		if (graph) (void) hipGraphExecDestroy(graph);
+		graph = nullptr;
+	}
 	~gp_batch()
 	{
+		free_scratch();
 		for (auto &t : tabs)
 			for (void *p : { (void *) t.coff, (void *) t.cid, (void *) t.cgeo, (void *) t.cgeo64 }) (void) hipFree(p);
-		for (void *p : { (void *) d_meshes, (void *) d_jobs, (void *) d_queue, (void *) d_level, (void *) d_sorted,
-		                 (void *) d_limits, (void *) d_nlim, (void *) d_key, (void *) d_dist, (void *) d_out })
-			(void) hipFree(p);
-		if (graph) (void) hipGraphExecDestroy(graph);
+		for (void *p : { (void *) d_meshes, (void *) d_jobs, (void *) d_queue, (void *) d_out }) (void) hipFree(p);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
 		if (stream) (void) hipStreamDestroy(stream);
@@ -177,11 +188,82 @@ static int up(V *&dst, const V *src, size_t n)
 	return GP_OK;
 }
 
+template <uint32_t NT>
+static int launch_shape(uint32_t njobs, uint32_t &grid, const void *&fn)
+{
+	int dev = 0, ncu = 0, occ = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch<NT>, NT, 0));
+	grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t) ncu * (uint32_t) std::max(1, occ)));
+	fn = (const void *) k_batch<NT>;
+	return GP_OK;
+}
+
+// Sizes the per-workgroup scratch for `block` threads per field and captures
+// the queue reset + persistent launch into the batch's graph.
+static int batch_configure(gp_batch *b, uint32_t block)
+{
+	b->free_scratch();
+	uint32_t grid = 0;
+	const void *fn = nullptr;
+	int rc;
+	switch (block) {
+	case 256: rc = launch_shape<256>(b->njobs, grid, fn); break;
+	case 512: rc = launch_shape<512>(b->njobs, grid, fn); break;
+	case 1024: rc = launch_shape<1024>(b->njobs, grid, fn); break;
+	default: return fail(GP_ERR_INVALID, "batch: block must be 256, 512 or 1024 (got %u)", block);
+	}
+	if (rc) return rc;
+	const size_t mv = b->max_nv, g = grid;
+	if (hipMalloc(&b->d_level, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_sorted, 4 * mv * g) != hipSuccess ||
+	    hipMalloc(&b->d_limits, 4 * (mv + 2) * g) != hipSuccess || hipMalloc(&b->d_nlim, 4 * g) != hipSuccess ||
+	    hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess) {
+		b->free_scratch();
+		return fail(GP_ERR_NOMEM, "batch: scratch allocation failed (%u workgroups x %u vertices)", grid, b->max_nv);
+	}
+	BArgs a{ b->d_meshes, b->d_jobs, b->njobs, b->max_nv, b->d_queue, b->d_level, b->d_sorted, b->d_limits, b->d_nlim,
+	         b->d_key, b->d_dist, b->d_out, b->tol };
+	void *kargs[] = { &a };
+	hipGraph_t gr = nullptr;
+	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+	k_batch_reset<<<1, 1, 0, b->stream>>>(b->d_queue);
+	const hipError_t le = hipLaunchKernel(fn, dim3(grid), dim3(block), kargs, 0, b->stream);
+	const hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+	if (le != hipSuccess || ce != hipSuccess) {
+		if (gr) (void) hipGraphDestroy(gr);
+		return fail(GP_ERR_HIP, "batch: capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+	}
+	const hipError_t e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(gr);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "batch: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	b->grid = grid;
+	b->block = block;
+	return GP_OK;
+}
+
+static int batch_launch(gp_batch *b, float *ms)
+{
+	GP_HIP(hipEventRecord(b->e0, b->stream));
+	GP_HIP(hipGraphLaunch(b->graph, b->stream));
+	GP_HIP(hipEventRecord(b->e1, b->stream));
+	GP_HIP(hipStreamSynchronize(b->stream));
+	GP_HIP(hipEventElapsedTime(ms, b->e0, b->e1));
+	return GP_OK;
+}
+
 int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh, const uint32_t *job_source,
-                 uint32_t njobs, double tol, gp_batch **out)
+                 uint32_t njobs, double tol, uint32_t block, gp_batch **out)
 {
 	*out = nullptr;
 	if (!meshes || !nmeshes || !job_mesh || !job_source || !njobs) return fail(GP_ERR_INVALID, "batch: empty input");
+	if (!(tol > 0)) return fail(GP_ERR_INVALID, "batch: tol must be positive");
+	for (uint32_t k = 0; k < nmeshes; k++)
+		if (!meshes[k]) return fail(GP_ERR_INVALID, "batch: mesh %u is null", k);
+	for (uint32_t q = 0; q < njobs; q++) {
+		if (job_mesh[q] >= nmeshes) return fail(GP_ERR_INVALID, "batch: job %u mesh out of range", q);
+		if (job_source[q] >= meshes[job_mesh[q]]->nv) return fail(GP_ERR_INVALID, "batch: job %u source out of range", q);
+	}
 	auto b = std::make_unique<gp_batch>();
 	b->njobs = njobs;
 	b->tol = tol;
@@ -189,33 +271,28 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 	b->tabs.resize(nmeshes);
 	for (uint32_t k = 0; k < nmeshes; k++) {
 		Che &m = *meshes[k];
+		if (m.nv >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "batch: mesh %u has too many vertices", k);
+		for (uint32_t v = 0; v < m.nv; v++)
+			if (m.fan_off[v + 1] - m.fan_off[v] > MAX_LINK)
+				return fail(GP_ERR_CAPACITY, "batch: mesh %u vertex %u: link over %u", k, v, MAX_LINK);
 		int rc = upload_mesh(m);
 		if (rc) return rc;
-		for (uint32_t v = 0; v < m.nv; v++)
-			if (m.fan_off[v + 1] - m.fan_off[v] > MAX_LINK) return fail(GP_ERR_CAPACITY, "batch: vertex link over %u", MAX_LINK);
 		Layout L;
-		rc = build_layout(m, nullptr, m.nv, false, L);
-		if (rc) return rc;
+		if ((rc = build_layout(m, nullptr, m.nv, false, L))) return rc;
 		auto &t = b->tabs[k];
-		std::vector<float4> cg(L.cgeo.size());
-		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
-		std::vector<double4> cg64(L.cgeo64.size());
-		std::memcpy(cg64.data(), L.cgeo64.data(), sizeof(double4) * cg64.size());
 		if ((rc = up(t.coff, L.coff.data(), L.coff.size())) || (rc = up(t.cid, L.cid.data(), L.cid.size())) ||
-		    (rc = up(t.cgeo, cg.data(), cg.size())) || (rc = up(t.cgeo64, cg64.data(), cg64.size())))
+		    (rc = up(t.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
+		    (rc = up(t.cgeo64, reinterpret_cast<const double4 *>(L.cgeo64.data()), L.cgeo64.size())))
 			return rc;
 		bm[k] = BMesh{ m.dev->fan_off, m.dev->fan, m.dev->closed, t.coff, t.cid, t.cgeo, t.cgeo64, nullptr,
 		               m.dev->h, m.dev->inv_h, m.nv, 0 };
 		b->max_nv = std::max(b->max_nv, m.nv);
 	}
-	// Longest jobs first.
-	b->order.resize(njobs);
-	std::iota(b->order.begin(), b->order.end(), 0u);
-	for (uint32_t q = 0; q < njobs; q++) {
-		if (job_mesh[q] >= nmeshes) return fail(GP_ERR_INVALID, "batch: job %u mesh out of range", q);
-		if (job_source[q] >= meshes[job_mesh[q]]->nv) return fail(GP_ERR_INVALID, "batch: job %u source out of range", q);
-	}
-	std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) {
+	// Queue order: longest fields first (work of a field grows ~ V^1.5), so
+	// the last jobs taken are the short ones and the grid drains evenly.
+	std::vector<uint32_t> order(njobs);
+	std::iota(order.begin(), order.end(), 0u);
+	std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
 		return meshes[job_mesh[x]]->nv > meshes[job_mesh[y]]->nv;
 	});
 	b->slot_of.resize(njobs);
@@ -224,75 +301,77 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 	std::vector<BJob> jobs(njobs);
 	unsigned long long off = 0;
 	for (uint32_t s = 0; s < njobs; s++) {
-		const uint32_t q = b->order[s];
+		const uint32_t q = order[s];
+		const uint32_t nv = meshes[job_mesh[q]]->nv;
 		b->slot_of[q] = s;
-		jobs[s] = BJob{ job_mesh[q], job_source[q], 0, 0, off, 0 };
 		b->out_off[q] = off;
-		b->job_nv[q] = meshes[job_mesh[q]]->nv;
-		off += meshes[job_mesh[q]]->nv;
+		b->job_nv[q] = nv;
+		jobs[s] = BJob{ job_mesh[q], job_source[q], 0, 0, off, 0 };
+		off += nv;
 	}
 	b->total_out = off;
 	int rc;
 	if ((rc = up(b->d_meshes, bm.data(), bm.size())) || (rc = up(b->d_jobs, jobs.data(), jobs.size()))) return rc;
-	int dev = 0, ncu = 0, occ = 0;
-	GP_HIP(hipGetDevice(&dev));
-	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch, BATCH_THREADS, 0));
-	b->grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t) ncu * (uint32_t) std::max(1, occ)));
-	const size_t mv = b->max_nv, g = b->grid;
-	if (hipMalloc(&b->d_queue, 4) != hipSuccess || hipMalloc(&b->d_level, 4 * mv * g) != hipSuccess ||
-	    hipMalloc(&b->d_sorted, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_limits, 4 * (mv + 2) * g) != hipSuccess ||
-	    hipMalloc(&b->d_nlim, 4 * g) != hipSuccess || hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess ||
-	    hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_out, 4 * std::max<size_t>(off, 1)) != hipSuccess)
-		return fail(GP_ERR_NOMEM, "batch: scratch allocation failed");
+	if (hipMalloc(&b->d_queue, 4) != hipSuccess || hipMalloc(&b->d_out, 4 * std::max<size_t>(off, 1)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "batch: output allocation failed (%llu values)", off);
 	GP_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
 	GP_HIP(hipEventCreate(&b->e0));
 	GP_HIP(hipEventCreate(&b->e1));
-	// One graph: queue reset + the persistent launch.
-	BArgs a{ b->d_meshes, b->d_jobs, njobs, b->max_nv, b->d_queue, b->d_level, b->d_sorted, b->d_limits, b->d_nlim,
-	         b->d_key, b->d_dist, b->d_out, tol };
-	hipGraph_t gr = nullptr;
-	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-	k_batch_reset<<<1, 1, 0, b->stream>>>(b->d_queue);
-	k_batch<<<b->grid, BATCH_THREADS, 0, b->stream>>>(a);
-	GP_HIP(hipStreamEndCapture(b->stream, &gr));
-	const hipError_t e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
-	(void) hipGraphDestroy(gr);
-	if (e != hipSuccess) return fail(GP_ERR_HIP, "batch: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	if (block) {
+		if ((rc = batch_configure(b.get(), block))) return rc;
+	} else {
+		// Autotune threads per field: one timed run of the whole batch per
+		// candidate after one warm-up run, keep the fastest.
+		float best = INFINITY;
+		uint32_t best_block = 1024;
+		for (uint32_t cand : { 256u, 512u, 1024u }) {
+			float ms = 0;
+			if ((rc = batch_configure(b.get(), cand)) || (rc = batch_launch(b.get(), &ms)) ||
+			    (rc = batch_launch(b.get(), &ms)))
+				return rc;
+			if (ms < best) { best = ms; best_block = cand; }
+		}
+		if (b->block != best_block && (rc = batch_configure(b.get(), best_block))) return rc;
+	}
 	*out = b.release();
 	return GP_OK;
 }
 
-int batch_run(gp_batch *b, double *ms, unsigned long long *sweeps, unsigned long long *relax)
+int batch_run(gp_batch *b, gp_batch_stats *st)
 {
 	if (!b) return fail(GP_ERR_INVALID, "batch is null");
-	GP_HIP(hipEventRecord(b->e0, b->stream));
-	GP_HIP(hipGraphLaunch(b->graph, b->stream));
-	GP_HIP(hipEventRecord(b->e1, b->stream));
-	GP_HIP(hipStreamSynchronize(b->stream));
-	float t = 0;
-	GP_HIP(hipEventElapsedTime(&t, b->e0, b->e1));
-	if (ms) *ms = t;
-	if (sweeps || relax) {
+	float ms = 0;
+	int rc = batch_launch(b, &ms);
+	if (rc) return rc;
+	if (st) {
 		std::vector<BJob> jobs(b->njobs);
 		GP_HIP(hipMemcpy(jobs.data(), b->d_jobs, sizeof(BJob) * b->njobs, hipMemcpyDeviceToHost));
 		unsigned long long s = 0, r = 0;
 		for (auto &j : jobs) { s += j.sweeps; r += j.relax; }
-		if (sweeps) *sweeps = s;
-		if (relax) *relax = r;
+		st->sweeps = s;
+		st->relaxations = r;
+		st->vertex_fields = b->total_out;
+		st->block = b->block;
+		st->grid = b->grid;
+		st->kernel_ms = ms;
 	}
 	return GP_OK;
 }
 
-int batch_result(gp_batch *b, uint32_t job, double *dist, uint32_t cap)
+int batch_result(const gp_batch *b, uint32_t job, double *dist, uint32_t cap, uint32_t *sweeps)
 {
 	if (!b || !dist) return fail(GP_ERR_INVALID, "null argument");
-	if (job >= b->njobs) return fail(GP_ERR_INVALID, "job out of range");
+	if (job >= b->njobs) return fail(GP_ERR_INVALID, "batch: job %u out of range", job);
 	const uint32_t n = b->job_nv[job];
-	if (cap < n) return fail(GP_ERR_CAPACITY, "result needs %u entries", n);
+	if (cap < n) return fail(GP_ERR_CAPACITY, "batch: result of job %u needs %u entries", job, n);
 	std::vector<float> tmp(n);
 	GP_HIP(hipMemcpy(tmp.data(), b->d_out + b->out_off[job], 4ull * n, hipMemcpyDeviceToHost));
 	for (uint32_t v = 0; v < n; v++) dist[v] = tmp[v];
+	if (sweeps) {
+		BJob j;
+		GP_HIP(hipMemcpy(&j, b->d_jobs + b->slot_of[job], sizeof(BJob), hipMemcpyDeviceToHost));
+		*sweeps = j.sweeps;
+	}
 	return GP_OK;
 }
 

@@ -39,9 +39,8 @@ void gp_ptp_opts_default(gp_ptp_opts *o)
 	o->coalesce = 1;
 	o->tol = GP_PTP_TOL;
 	o->launch_mode = GP_LAUNCH_GRAPH;
-	o->band_width = 4.0;
-	o->local_iters = 8;
-	o->judge_cap = 64;
+	o->band_width = 8.0;
+	o->local_iters = 32;
 }
 
 const char *gp_last_error(void) { return gp::last_error(); }
@@ -255,9 +254,17 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
                            const gp_ptp_opts &o, gp_plan **out)
 {
 	const gp::Che &m = mesh->m;
-	std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
-	int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
-	if (rc) return rc;
+	if (o.schedule == GP_SCHED_PATCHES) return gp::plan_create_patches(&mesh->m, sources, n_sources, nullptr, o, out);
+	if (o.schedule != GP_SCHED_RINGS && o.schedule != GP_SCHED_LEARNED_BAND && o.schedule != GP_SCHED_EUCLID_BAND &&
+	    o.schedule != GP_SCHED_RING_BAND)
+		return gp::fail(GP_ERR_INVALID, "unknown schedule %d", o.schedule);
+	std::vector<uint32_t> lev, sorted, lim;
+	if (o.schedule == GP_SCHED_RINGS || o.schedule == GP_SCHED_RING_BAND) {
+		lev.resize(m.nv);
+		sorted.resize(m.nv);
+		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
+		if (rc) return rc;
+	}
 	if (o.schedule == GP_SCHED_RINGS)
 		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
 	std::vector<double> d;
@@ -281,12 +288,7 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 		}
 		dhat = d.data();
 	}
-	std::vector<uint32_t> order, blim;
-	rc = gp::band_order(m, sources, n_sources, lev.data(), dhat, o.band_width * m.mean_edge, order, blim);
-	if (rc) return rc;
-	gp_ptp_opts ob = o;
-	ob.coalesce = 1;
-	return gp::plan_create_rings(&mesh->m, sources, n_sources, blim.data(), (uint32_t) blim.size(), order.data(), ob, out);
+	return gp::plan_create_patches(&mesh->m, sources, n_sources, dhat, o, out);
 }
 
 extern "C" {
@@ -389,6 +391,31 @@ int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 		o.schedule = GP_SCHED_LEARNED_BAND;
 		return plan_create_any(mesh, sources, n_sources, dhat.data(), o, out);
 	)
+}
+
+int gp_batch_create(gp_che *const *meshes, uint32_t n_meshes, const uint32_t *job_mesh, const uint32_t *job_source,
+                    uint32_t n_jobs, double tol, uint32_t block, gp_batch **out)
+{
+	GP_GUARD(
+		if (!meshes || !out) return gp::fail(GP_ERR_INVALID, "null argument");
+		*out = nullptr;
+		std::vector<gp::Che *> ms(n_meshes);
+		for (uint32_t k = 0; k < n_meshes; k++) {
+			if (!meshes[k]) return gp::fail(GP_ERR_INVALID, "batch: mesh %u is null", k);
+			ms[k] = &meshes[k]->m;
+		}
+		return gp::batch_create(ms.data(), n_meshes, job_mesh, job_source, n_jobs, tol, block, out);
+	)
+}
+
+int gp_batch_run(gp_batch *batch, gp_batch_stats *stats)
+{
+	GP_GUARD(return gp::batch_run(batch, stats);)
+}
+
+int gp_batch_result(const gp_batch *batch, uint32_t job, double *dist, uint32_t cap, uint32_t *sweeps)
+{
+	GP_GUARD(return gp::batch_result(batch, job, dist, cap, sweeps);)
 }
 
 }  // extern "C"

@@ -129,11 +129,27 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats);
 int plan_result(gp_plan *p, double *dist);
 int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n);
 
+// Patch / band schedules (patches.hip).  dhat == null: no band gating.
+struct PatchPlan;
+int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat, const gp_ptp_opts &opts,
+                        gp_plan **out);
+int patch_create(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat, const gp_ptp_opts &o,
+                 PatchPlan **out);
+int patch_run(PatchPlan *pp, gp_ptp_stats *stats);
+int patch_result(PatchPlan *pp, double *dist);
+int patch_trace(PatchPlan *pp, uint32_t *out, uint32_t cap, uint32_t *n);
+void patch_destroy(PatchPlan *pp);
+
 const char *last_error();
 void clear_error();
 
 // Learned band predictor (mlp.hip).
 constexpr uint32_t MLP_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1;
+int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh, const uint32_t *job_source,
+                 uint32_t njobs, double tol, uint32_t block, gp_batch **out);
+int batch_run(gp_batch *b, gp_batch_stats *st);
+int batch_result(const gp_batch *b, uint32_t job, double *dist, uint32_t cap, uint32_t *sweeps);
+
 int mlp_create(uint32_t seed, gp_mlp **out);
 int mlp_params(gp_mlp *m, float *get, const float *set, uint32_t n);
 int mlp_features(Che &mesh, const uint32_t *src, uint32_t ns, const uint32_t *levels, float *F_out, float *e_out);

@@ -104,7 +104,7 @@ int upload_mesh(Che &m)
 // launch writes it, so kernel boundaries are the only synchronisation.  The
 // captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
 struct Slot {
-	uint32_t i, j, d, iter, done, fresh, phase, judge;
+	uint32_t i, j, d, iter, done, fresh, pad0, pad1;
 };
 struct PState {
 	Slot slot[4];
@@ -125,11 +125,10 @@ struct KArgs {
 	void *dist0, *dist1;
 	PState *st;
 	uint32_t *trace;           // 8 words per sweep, or null
-	const uint16_t *lidx;      // band solves: slot neighbour's index in its tile, or LOCAL_OUT
 	double tol;
 	float h, inv_h;
 	uint32_t nl, max_iter;
-	uint32_t nr, local_iters, judge_cap;
+	uint32_t k2;               // 1: K/2 limit (the drivers); 0: the uncapped iter_error loop
 };
 
 // Relaxes the window [start, end) (positions); returns this thread's count of
@@ -195,7 +194,7 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	if (!prev.fresh) {
 		// The previous sweep's outcome: the reference loop tail.
 		const uint32_t nbad = st->bad[kp];
-		const uint32_t pi = prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
+		const uint32_t pi = a.k2 && prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
 		cur.i = pi + (nbad == 0 ? 1u : 0u);
 		cur.j = prev.j + (prev.j < a.nl - 1 ? 1u : 0u);
 		cur.d = prev.d ^ 1u;
@@ -221,195 +220,11 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 		if (cur.done) { st->final_d = cur.d; st->done = 1; }
 	}
 	if (cur.done) return;
-	const uint32_t i = cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
+	const uint32_t i = a.k2 && cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
 	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
 	const uint32_t bad = wave_sum(sweep_body<F64>(a, start, end, cend, od, nd));
-	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
-}
-
-// ---------------------------------------------------------------------------
-// Band schedule (learned / Euclidean / ring-group bands; no reference
-// counterpart).  Vertices are numbered by band, then by Morton code inside a
-// band, so a tile of TILE consecutive positions is a compact patch of one or
-// two bands.  A workgroup loads its tile's distances into LDS and runs up to
-// local_iters Jacobi iterations on it (out-of-tile neighbours held at their
-// values from the previous sweep), so the front crosses up to local_iters
-// hops per sweep instead of one.  The window [i, j) of bands grows by one band
-// per sweep; band i retires when it changed by less than tol and band i+1 was
-// relaxed with it.  When the window closes, judge sweeps relax the whole mesh
-// until no vertex changes by tol: a bad prediction costs sweeps, never
-// accuracy.
-
-constexpr uint32_t TILE = 256;
-
-__device__ __forceinline__ float lb_gather(uint32_t li, const float *sd, float tout)
-{
-	return li != LOCAL_OUT ? sd[li] : tout;
-}
-
-// Relaxes positions [ws, we) tile by tile; returns this thread's count of
-// non-converged vertices among [cs, ce).
-__device__ uint32_t lb_tiles(const KArgs &a, uint32_t ws, uint32_t we, uint32_t cs, uint32_t ce, const float *od,
-                             float *nd)
-{
-	__shared__ float sd[2][TILE];
-	if (ws >= we) return 0;
-	const uint32_t t0 = ws / TILE, t1 = (we + TILE - 1) / TILE;
-	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
-	const uint32_t x = blockIdx.x & 7u;
-	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
-	const uint32_t len = t1 - t0;
-	const uint32_t part = (len + ngrp - 1) / ngrp;
-	const uint32_t lo = t0 + x * part;
-	const uint32_t hi = min(t1, lo + part);
-	const uint32_t tid = threadIdx.x, lane = tid & 63u;
-	uint32_t bad = 0;
-	for (uint32_t tile = lo + (blockIdx.x >> 3); tile < hi; tile += nbx) {
-		const uint32_t pos = tile * TILE + tid;
-		const bool live = pos < a.nr;
-		const bool act = live && pos >= ws && pos < we;
-		uint32_t S = 0;
-		size_t base = 0;
-		if (live) {
-			const uint32_t c = pos >> 6, cb = a.coff[c];
-			S = a.coff[c + 1] - cb;
-			base = (size_t) cb * 64 + lane;
-		}
-		const float old = live ? od[pos] : (float) INFINITY;
-		const bool fast = S <= MAXS;
-		uint32_t id[MAXS], li[MAXS];
-		float4 g[MAXS];
-		float tout[MAXS];
-#pragma unroll
-		for (uint32_t k = 0; k < MAXS; k++) {
-			id[k] = SLOT_EMPTY; li[k] = LOCAL_OUT; tout[k] = (float) INFINITY;
-			if (act && fast && k < S) {
-				id[k] = a.cid[base + 64 * k];
-				li[k] = a.lidx[base + 64 * k];
-				g[k] = a.cgeo[base + 64 * k];
-			}
-		}
-#pragma unroll
-		for (uint32_t k = 0; k < MAXS; k++)
-			if (act && fast && k < S && li[k] == LOCAL_OUT) tout[k] = od[id[k] & ID30_MASK];
-		sd[0][tid] = old;
-		__syncthreads();
-		uint32_t b = 0;
-		for (uint32_t it = 0; it < a.local_iters; it++) {
-			const float cur = sd[b][tid];
-			float best = cur;
-			if (act && fast) {
-				float t[MAXS];
-#pragma unroll
-				for (uint32_t k = 0; k < MAXS; k++) t[k] = (k < S) ? lb_gather(li[k], sd[b], tout[k]) : (float) INFINITY;
-#pragma unroll
-				for (uint32_t k = 0; k + 1 < MAXS; k++) {
-					if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
-						const float p = (id[k] & F64_FLAG)
-						                    ? tri_flagged(a, base + 64 * k, t[k], t[k + 1])
-						                    : update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k],
-						                                           t[k + 1], a.h, a.inv_h);
-						if (p < best) best = p;
-					}
-				}
-			} else if (act) {
-				// Fans longer than MAXS: stream the slots.
-				uint32_t ida = a.cid[base];
-				uint32_t lia = a.lidx[base];
-				float4 ga = a.cgeo[base];
-				float ta = lia != LOCAL_OUT ? sd[b][lia] : od[ida & ID30_MASK];
-				for (uint32_t k = 1; k < S; k++) {
-					const uint32_t idb = a.cid[base + 64 * k];
-					if (idb & SLOT_EMPTY) break;
-					const uint32_t lib = a.lidx[base + 64 * k];
-					const float4 gb = a.cgeo[base + 64 * k];
-					const float tb = lib != LOCAL_OUT ? sd[b][lib] : od[idb & ID30_MASK];
-					const float p = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
-					                                 : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h,
-					                                                        a.inv_h);
-					if (p < best) best = p;
-					ida = idb; ga = gb; ta = tb;
-				}
-			}
-			sd[b ^ 1u][tid] = best;
-			const int changed = __syncthreads_or(best < cur);
-			b ^= 1u;
-			if (!changed) break;
-		}
-		if (act) {
-			const float v = sd[b][tid];
-			nd[pos] = v;
-			if (pos >= cs && pos < ce) bad += !(fabs((double) v - (double) old) / (double) old < a.tol);
-		}
-		__syncthreads();
-	}
-	return bad;
-}
-
-__device__ __forceinline__ void lb_window(const KArgs &a, const Slot &s, uint32_t &ws, uint32_t &we, uint32_t &cs,
-                                          uint32_t &ce)
-{
-	if (s.phase == 0) {
-		ws = a.limits[s.i]; we = a.limits[s.j]; cs = ws; ce = a.limits[s.i + 1];
-	} else {
-		ws = 0; we = a.nr; cs = 0; ce = a.nr;
-	}
-}
-
-// Band-schedule sweep k (k4 = k & 3).
-__global__ __launch_bounds__(256) void k_lb_sweep(KArgs a, uint32_t k4)
-{
-	PState *st = a.st;
-	const uint32_t kp = (k4 + 3u) & 3u;
-	const Slot prev = st->slot[kp];
-	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-	if (prev.done) {
-		if (lead) st->slot[k4] = prev;
-		return;
-	}
-	Slot cur = prev;
-	cur.fresh = 0;
-	if (!prev.fresh) {
-		const uint32_t nbad = st->bad[kp];
-		uint32_t ws, we, cs, ce;
-		lb_window(a, prev, ws, we, cs, ce);
-		if (prev.phase == 0) {
-			const bool last = prev.j == a.nl - 1;
-			const bool retire = nbad == 0 && (prev.i + 1 < prev.j || last);
-			cur.i = prev.i + (retire ? 1u : 0u);
-			cur.j = prev.j + (last ? 0u : 1u);
-			if (!(cur.i < cur.j) || !(prev.iter + 1 < a.max_iter)) { cur.phase = 1; cur.judge = 0; }
-		} else {
-			cur.judge = prev.judge + 1;
-			cur.done = nbad == 0 || cur.judge >= a.judge_cap;
-		}
-		cur.iter = prev.iter + 1;
-		cur.d = prev.d ^ 1u;
-		if (lead) {
-			if (a.trace) {
-				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
-				r[0] = prev.iter + 1; r[1] = prev.phase ? 0xFFFFFFFFu : prev.i; r[2] = prev.j; r[3] = ws; r[4] = we;
-				r[5] = ce - cs; r[6] = ce - cs - nbad; r[7] = prev.phase;
-			}
-			st->relax += we - ws;
-			st->cond += ce - cs;
-			if (a.fcum) st->f64 += a.fcum[we] - a.fcum[ws];
-			st->sweeps = cur.iter;
-		}
-	}
-	if (lead) {
-		st->slot[k4] = cur;
-		st->bad[(k4 + 1u) & 3u] = 0;
-		if (cur.done) { st->final_d = cur.d; st->done = 1; }
-	}
-	if (cur.done) return;
-	uint32_t ws, we, cs, ce;
-	lb_window(a, cur, ws, we, cs, ce);
-	const float *od = (const float *) (cur.d ? a.dist1 : a.dist0);
-	float *nd = (float *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = wave_sum(lb_tiles(a, ws, we, cs, ce, od, nd));
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 }
 
@@ -453,10 +268,9 @@ struct gp_plan {
 	gp::DeviceMesh *dm = nullptr;
 	gp_ptp_opts opts{};
 	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
-	uint32_t nchunks = 0, max_slots = 0, judge_cap = 0;
+	uint32_t nchunks = 0, max_slots = 0;
 	unsigned long long n_flagged = 0;
-	bool lb = false;
-	uint16_t *d_lidx = nullptr;
+	gp::PatchPlan *pp = nullptr;             // patch / band schedules
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
 	uint32_t *d_limits = nullptr, *d_coff = nullptr, *d_cid = nullptr;
 	unsigned long long *d_fcum = nullptr;
@@ -474,8 +288,9 @@ struct gp_plan {
 	{
 		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
 		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
-		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_lidx })
+		                 (void *) d_out, (void *) d_state, (void *) d_fcum })
 			(void) hipFree(p);
+		if (pp) gp::patch_destroy(pp);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
 		if (g_small) (void) hipGraphExecDestroy(g_small);
 		if (h_state) (void) hipHostFree(h_state);
@@ -537,31 +352,27 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	p->dm = use->dev;
 	p->n = use->nv;
 	const bool f64 = opts.precision == GP_PREC_F64;
-	p->lb = opts.schedule != GP_SCHED_RINGS;
-	if (p->lb && (f64 || !opts.coalesce)) return fail(GP_ERR_INVALID, "band schedules run in FP32 on a coalesced mesh");
-	if (p->lb && (opts.local_iters == 0 || opts.local_iters > 1024)) return fail(GP_ERR_INVALID, "local_iters out of range");
+	if (opts.schedule != GP_SCHED_RINGS) return fail(GP_ERR_INVALID, "plan_create_rings: ring schedule only");
 	Layout L;
-	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, p->lb ? TILE : 0);
+	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L);
 	if (rc) return rc;
 	p->nchunks = L.nchunks;
 	p->max_slots = L.max_slots;
 	p->n_flagged = L.n_flagged;
-	p->judge_cap = opts.judge_cap ? opts.judge_cap : 64;
-	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (p->lb ? 4 * nl + 64 : (nl << 1));
+	// The drivers cap the loop at 2*|limits| sweeps (src/geodesics_ptp.cpp:66-67);
+	// the uncapped loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248)
+	// has no cap, so a generous one stands in as a guard.
+	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (opts.uncapped ? 64 * nl + 64 : (nl << 1));
 	p->block = 256;
 	// One wave per chunk, at most as many workgroups as fit on the device at
 	// once (idle workgroups of small windows then cost no second dispatch round).
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	if (p->lb) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lb_sweep, 256, 0));
-	else if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true>, 256, 0));
+	if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true>, 256, 0));
 	else GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<false>, 256, 0));
 	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
-	if (p->lb) {
-		if ((rc = upload(p->d_lidx, L.lidx))) return rc;
-	}
 
 	const size_t es = f64 ? sizeof(double) : sizeof(float);
 	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
@@ -592,7 +403,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
-	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + p->judge_cap + 1)) != hipSuccess ||
+	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + 1)) != hipSuccess ||
 	    hipHostMalloc(&p->h_state, sizeof(PState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	*out = p.release();
@@ -606,8 +417,7 @@ static KArgs make_args(const gp_plan *p)
 	a.cx64 = p->d_cx64; a.fcum = p->d_fcum; a.order = p->d_order; a.dist0 = p->d_dist0; a.dist1 = p->d_dist1;
 	a.st = p->d_state; a.trace = p->d_trace;
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
-	a.nl = p->nl; a.max_iter = p->max_iter;
-	a.lidx = p->d_lidx; a.nr = p->nr; a.local_iters = p->opts.local_iters; a.judge_cap = p->judge_cap;
+	a.nl = p->nl; a.max_iter = p->max_iter; a.k2 = p->opts.uncapped ? 0u : 1u;
 	return a;
 }
 
@@ -616,8 +426,7 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 {
 	for (uint32_t q = 0; q < count; q++) {
 		const uint32_t k4 = (k0 + q) & 3u;
-		if (p->lb) k_lb_sweep<<<p->grid, p->block, 0, p->stream>>>(a, k4);
-		else if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		else k_sweep<false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 	}
 }
@@ -637,6 +446,7 @@ static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *o
 int plan_run(gp_plan *p, gp_ptp_stats *stats)
 {
 	if (!p) return fail(GP_ERR_INVALID, "plan is null");
+	if (p->pp) return patch_run(p->pp, stats);
 	const auto t0 = std::chrono::steady_clock::now();
 	const bool f64 = p->opts.precision == GP_PREC_F64;
 	const uint32_t done0 = p->nl < 3 ? 1u : 0u;
@@ -663,7 +473,7 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	// per sweep until |limits|-1, so at least nl-1-j sweeps remain: big graphs
 	// while that bound allows, small ones near the end.
 	uint32_t launched = 0, j_seen = 2;
-	const uint32_t cap = p->max_iter + (p->lb ? p->judge_cap : 0);
+	const uint32_t cap = p->max_iter;
 	while (!done0 && launched <= cap) {
 		const bool big = p->nl - 1 > j_seen + BIG;
 		const uint32_t cnt = big ? BIG : SMALL;
@@ -689,6 +499,8 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 		stats->relaxations = p->h_state->relax;
 		stats->cond_checks = p->h_state->cond;
 		stats->f64_updates = p->h_state->f64;
+		stats->patch_visits = 0;
+		stats->deferred = 0;
 		stats->kernel_ms = ms;
 		stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	}
@@ -698,6 +510,7 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 int plan_result(gp_plan *p, double *dist)
 {
 	if (!p || !dist) return fail(GP_ERR_INVALID, "plan_result: null argument");
+	if (p->pp) return patch_result(p->pp, dist);
 	if (!p->has_run) return fail(GP_ERR_INVALID, "plan_result: plan has not run");
 	const uint32_t d = p->h_state->final_d;
 	const bool last_read = p->opts.result_buffer == GP_RESULT_LAST_READ;
@@ -722,6 +535,7 @@ int plan_result(gp_plan *p, double *dist)
 
 int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n)
 {
+	if (p && p->pp) return patch_trace(p->pp, out, cap, n);
 	if (!p || !p->has_run) return fail(GP_ERR_INVALID, "plan_trace: plan has not run");
 	const uint32_t s = p->h_state->sweeps;
 	if (n) *n = s;
@@ -730,6 +544,21 @@ int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n)
 		GP_HIP(hipMemcpyAsync(out, p->d_trace, sizeof(uint32_t) * 8 * (size_t) c, hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipStreamSynchronize(p->stream));
 	}
+	return GP_OK;
+}
+
+int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat, const gp_ptp_opts &opts,
+                        gp_plan **out)
+{
+	*out = nullptr;
+	PatchPlan *pp = nullptr;
+	int rc = patch_create(mesh, src, ns, dhat, opts, &pp);
+	if (rc) return rc;
+	auto p = std::make_unique<gp_plan>();
+	p->pp = pp;
+	p->mesh = mesh;
+	p->opts = opts;
+	*out = p.release();
 	return GP_OK;
 }
 

@@ -19,6 +19,21 @@
 
 namespace gp {
 
+__global__ void k_rings_init(uint32_t *level, unsigned long long *key, uint32_t *sorted, uint32_t nv)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+		level[v] = GP_NIL;
+		key[v] = ~0ull;
+		sorted[v] = GP_NIL;
+	}
+}
+
+__global__ __launch_bounds__(BFS_THREADS) void k_rings_bfs(const uint32_t *fan_off, const uint32_t *fan,
+                                                           const uint8_t *closed, const BfsJob *jobs)
+{
+	bfs_block(fan_off, fan, closed, jobs[blockIdx.x]);
+}
+
 int toplesets_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels, uint32_t *sorted,
                   std::vector<uint32_t> &limits)
 {

@@ -24,7 +24,7 @@ __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v)
 
 // Exclusive prefix sum over the workgroup (wave64 shuffles + LDS for the
 // per-wave totals).  All threads must call it.
-__device__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_w, uint32_t &total)
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_w, uint32_t &total)
 {
 	const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 	uint32_t inc = x;
@@ -46,15 +46,6 @@ __device__ uint32_t block_exclusive_scan(uint32_t x, uint32_t *s_w, uint32_t &to
 	return ex;
 }
 
-__global__ void k_rings_init(uint32_t *level, unsigned long long *key, uint32_t *sorted, uint32_t nv)
-{
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
-		level[v] = GP_NIL;
-		key[v] = ~0ull;
-		sorted[v] = GP_NIL;
-	}
-}
-
 struct BfsJob {
 	const uint32_t *src;   // sources (distinct)
 	uint32_t ns;
@@ -66,7 +57,7 @@ struct BfsJob {
 };
 
 // One BFS run by the whole workgroup (blockDim.x threads, a multiple of 64).
-__device__ void bfs_block(const uint32_t *fan_off, const uint32_t *fan, const uint8_t *closed, const BfsJob &J)
+__device__ inline void bfs_block(const uint32_t *fan_off, const uint32_t *fan, const uint8_t *closed, const BfsJob &J)
 {
 	__shared__ uint32_t s_w[17];
 	const uint32_t tid = threadIdx.x;
@@ -139,11 +130,6 @@ __device__ void bfs_block(const uint32_t *fan_off, const uint32_t *fan, const ui
 	__syncthreads();
 }
 
-__global__ __launch_bounds__(BFS_THREADS) void k_rings_bfs(const uint32_t *fan_off, const uint32_t *fan,
-                                                           const uint8_t *closed, const BfsJob *jobs)
-{
-	bfs_block(fan_off, fan, closed, jobs[blockIdx.x]);
-}
 
 
 }  // namespace gp

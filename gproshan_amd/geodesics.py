@@ -24,7 +24,8 @@ from ._lib import GPError, NIL, PTP_TOL, check, lib, ptr
 
 __all__ = ["che", "geodesics", "parallel_toplesets_propagation_gpu",
            "parallel_toplesets_propagation_coalescence_gpu", "parallel_toplesets_propagation_cpu",
-           "parallel_toplesets_propagation_coalescence_cpu", "normalize_ptp", "GPError", "NIL", "PTP_TOL"]
+           "parallel_toplesets_propagation_coalescence_cpu", "normalize_ptp", "GPError", "NIL", "PTP_TOL",
+           "Plan", "Batch"]
 
 
 def _u32(a):
@@ -149,12 +150,12 @@ def normalize_ptp(dist):
 
 
 def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRITTEN, schedule=L.SCHED_RINGS,
-        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH):
+        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH, uncapped=False):
     """One PTP solve with explicit options; returns (dist, sorted, stats dict)."""
     src = _u32(sources)
     o = L.default_opts(precision=precision, result_buffer=result_buffer, schedule=schedule,
                        coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid,
-                       launch_mode=launch_mode)
+                       launch_mode=launch_mode, uncapped=int(uncapped))
     dist = np.zeros(mesh.n_vertices())
     srt = np.zeros(mesh.n_vertices(), np.uint32)
     st = L.PtpStats()
@@ -168,7 +169,7 @@ class Plan:
     def __init__(self, mesh, sources, dhat=None, **kw):
         self.mesh = mesh
         self._src = _u32(sources)
-        o = L.default_opts(**{k: (int(v) if k == "coalesce" else v) for k, v in kw.items()})
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped") else v) for k, v in kw.items()})
         h = C.c_void_p()
         if dhat is None:
             check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
@@ -200,6 +201,45 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             lib().gp_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Many single-source fields in one persistent launch (gp_batch_*).
+
+    jobs: sequence of (mesh index, source vertex).  block: threads per field
+    (256, 512, 1024) or 0 to autotune it on the batch itself.
+    """
+
+    def __init__(self, meshes, jobs, tol=L.PTP_TOL, block=0):
+        self.meshes = list(meshes)
+        jobs = np.asarray(jobs, dtype=np.int64).reshape(-1, 2)
+        self._jm = _u32(jobs[:, 0])
+        self._js = _u32(jobs[:, 1])
+        hs = (C.c_void_p * max(len(self.meshes), 1))(*[m.handle for m in self.meshes])
+        h = C.c_void_p()
+        check(lib().gp_batch_create(hs, len(self.meshes), ptr(self._jm), ptr(self._js), len(jobs), float(tol),
+                                    int(block), C.byref(h)))
+        self._h = h
+
+    def run(self):
+        st = L.BatchStats()
+        check(lib().gp_batch_run(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def result(self, job):
+        n = self.meshes[int(self._jm[job])].n_vertices()
+        d = np.zeros(n)
+        sw = C.c_uint32()
+        check(lib().gp_batch_result(self._h, int(job), ptr(d), n, C.byref(sw)))
+        return d, sw.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gp_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -3,8 +3,8 @@
 Data: `n_meshes` seeded meshes (height fields of 8 random sinusoids with a
 random amplitude scale, and displaced icospheres), vertex counts log-uniform
 in [10k, 300k]; one random source per mesh.  Labels are GPU distance fields
-from the Euclidean-band schedule run to tol 1e-5 (it converges to the fixed
-point of the relaxation); features are computed on the GPU
+from the patch schedule run to tol 1e-6 (the fixed point of the
+relaxation); features are computed on the GPU
 (gp_mlp_features); `samples_per_mesh` vertices are drawn per mesh.  The MLP
 is trained with the library's BF16-MFMA gradient kernel and Adam
 (gp_mlp_train).  The parameters are saved as a float32 .npy.
@@ -46,7 +46,7 @@ def make_mesh(rng: np.random.Generator):
 
 
 def label_field(mesh, src):
-    plan = P.Plan(mesh, src, schedule=L.SCHED_EUCLID_BAND, band_width=4.0, local_iters=8, tol=1e-5)
+    plan = P.Plan(mesh, src, schedule=L.SCHED_PATCHES, tol=1e-6)
     plan.run()
     return plan.result()
 

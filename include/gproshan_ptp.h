@@ -53,17 +53,22 @@ enum gp_result_buffer {
 	GP_RESULT_LAST_READ = 1
 };
 
-/* Band schedule.  GP_SCHED_RINGS is the reference's; the band schedules have
- * no reference counterpart: vertices are grouped into bands of predicted
- * distance, relaxed tile by tile with local iterations, and finished by judge
- * sweeps over the whole mesh until nothing changes by tol (they converge to
- * the fixed point of the relaxation, not to the reference's truncated
- * schedule). */
+/* Schedule.  GP_SCHED_RINGS is the reference's window schedule over BFS
+ * rings.  The others have no reference counterpart and run the same FP32
+ * relaxation to its fixed point instead (patches.hip): vertices in Morton
+ * order, patches of `block` consecutive vertices iterated in LDS up to
+ * local_iters times, an active list of patches woken by changed neighbours;
+ * the band schedules also gate the list by a predicted distance: a launch
+ * processes the listed patches whose smallest predicted distance is within
+ * band_width mean edge lengths of the list's smallest, so the front sweeps
+ * the mesh in predicted order.  They return the fixed point to about tol
+ * (relative), not the reference's truncated schedule. */
 enum gp_schedule {
 	GP_SCHED_RINGS = 0,        /* toplesets (BFS rings), the reference schedule                     */
-	GP_SCHED_LEARNED_BAND = 1, /* bands of the learned distance predictor (gp_mlp_*)                */
-	GP_SCHED_EUCLID_BAND = 2,  /* bands of the Euclidean distance to the nearest source (ablation)  */
-	GP_SCHED_RING_BAND = 3     /* bands of band_width consecutive BFS rings (ablation: no predictor) */
+	GP_SCHED_LEARNED_BAND = 1, /* patches gated by the learned distance predictor (gp_mlp_*)        */
+	GP_SCHED_EUCLID_BAND = 2,  /* patches gated by the Euclidean distance to the nearest source     */
+	GP_SCHED_RING_BAND = 3,    /* patches gated by BFS ring level * mean edge length                */
+	GP_SCHED_PATCHES = 4       /* patches, no gating (ablation: no predictor)                       */
 };
 
 /* How the sweeps are launched. */
@@ -77,27 +82,34 @@ typedef struct gp_ptp_opts {
 	int result_buffer;  /* gp_result_buffer                                      */
 	int schedule;       /* gp_schedule                                           */
 	int coalesce;       /* 1: renumber vertices by rings before the sweeps       */
-	double tol;         /* relative-change tolerance of the ring test (PTP_TOL)  */
+	double tol;         /* relative-change tolerance (PTP_TOL for the ring test) */
 	uint32_t max_sweeps;/* 0: the reference's 2*|limits| cap                     */
-	uint32_t block;     /* threads per workgroup (0: autotuned/default)          */
+	uint32_t block;     /* patch schedules: patch size 256 or 512 (0: 256)       */
 	uint32_t grid;      /* workgroups (0: autotuned/default)                     */
 	int launch_mode;    /* gp_launch_mode                                        */
-	double band_width;  /* band schedules: band width in mean edge lengths       */
-	uint32_t local_iters;/* band schedules: Jacobi iterations per tile per sweep */
-	uint32_t judge_cap; /* band schedules: maximum judge sweeps                  */
+	double band_width;  /* band schedules: gate width in mean edge lengths       */
+	uint32_t local_iters;/* patch schedules: iteration cap per patch visit       */
+	int uncapped;       /* ring schedule: 1 = the reference's uncapped iter_error
+	                       loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248):
+	                       no K/2 freeze, no 2*|limits| cap; it runs to the
+	                       relaxation's fixed point as tol -> 0               */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {
-	uint32_t sweeps;          /* sweeps run                                           */
-	uint32_t n_limits;        /* |limits| of the band structure used                  */
-	uint64_t relaxations;     /* sum over sweeps of the window size (vertex relaxations) */
-	uint64_t cond_checks;     /* sum over sweeps of the ring-i size (convergence checks) */
-	uint64_t f64_updates;     /* FP64-fallback triangle updates (GP_PREC_F32)         */
+	uint32_t sweeps;          /* sweeps run (patch schedules: non-empty launches)     */
+	uint32_t n_limits;        /* |limits| (patch schedules: number of patches)        */
+	uint64_t relaxations;     /* vertex relaxations (window sizes; patch schedules:
+	                             patch size x iterations per visit)                   */
+	uint64_t cond_checks;     /* convergence checks (ring i sizes; patch visits x size) */
+	uint64_t f64_updates;     /* FP64-fallback triangle updates (ring schedule, F32)  */
+	uint64_t patch_visits;    /* patch schedules: patches processed                   */
+	uint64_t deferred;        /* band schedules: patch entries carried over           */
 	double kernel_ms;         /* device time of the sweep loop (hipEvent)             */
 	double total_ms;          /* host wall time of the whole call                     */
 } gp_ptp_stats;
 
-/* Fill opts with the defaults (FP32, last written, rings, coalesce, PTP_TOL). */
+/* Fill opts with the defaults (FP32, last written, rings, coalesce, PTP_TOL;
+ * patch schedules: block 256, local_iters 32, band_width 8). */
 void gp_ptp_opts_default(gp_ptp_opts *opts);
 
 /* Text of the last error on this thread ("" if none). */
@@ -240,6 +252,33 @@ int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t 
 /* Learned-band plan: gp_mlp_predict, then gp_plan_create_bands. */
 int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, gp_mlp *mlp,
                            const gp_ptp_opts *opts, gp_plan **out);
+
+/* ---- batched single-source fields (no reference counterpart) ------------ */
+
+/* Many independent one-source fields in one persistent launch: job q is the
+ * field of job_source[q] on meshes[job_mesh[q]].  It replaces a host loop of
+ * single-source calls, as in the reference's FPS and benchmark harness
+ * (src/sampling.cpp:69, src/test_geodesics_ptp.cpp:310), each running the
+ * window schedule of parallel_toplesets_propagation_gpu
+ * (src/cuda/geodesics_ptp.cu:20-84) in FP32 with the FP64 fallback on
+ * skinny/obtuse triangles, rings built on the GPU, last-written buffer.
+ * block: threads per field (256, 512 or 1024), or 0 to autotune it by timing
+ * the whole batch once per candidate. */
+typedef struct gp_batch gp_batch;
+typedef struct gp_batch_stats {
+	uint64_t sweeps;         /* summed over the fields                         */
+	uint64_t relaxations;
+	uint64_t vertex_fields;  /* sum of the fields' vertex counts               */
+	uint32_t block, grid;    /* launch shape in use                            */
+	double kernel_ms;        /* device time of the launch                      */
+} gp_batch_stats;
+
+int gp_batch_create(gp_che *const *meshes, uint32_t n_meshes, const uint32_t *job_mesh,
+                    const uint32_t *job_source, uint32_t n_jobs, double tol, uint32_t block, gp_batch **out);
+int gp_batch_run(gp_batch *batch, gp_batch_stats *stats);
+/* Field of job q (cap >= its mesh's vertex count); *sweeps may be NULL. */
+int gp_batch_result(const gp_batch *batch, uint32_t job, double *dist, uint32_t cap, uint32_t *sweeps);
+void gp_batch_destroy(gp_batch *batch);
 
 #ifdef __cplusplus
 }

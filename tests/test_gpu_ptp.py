@@ -72,6 +72,28 @@ def test_f32_matches_oracle(case, oracle, launch):
     assert mx <= 1e-4 or not same_sched
 
 
+@pytest.mark.parametrize("tol", [1e-3, 1e-5])
+def test_uncapped_matches_oracle(case, oracle, tol):
+    # The reference's uncapped loop (no K/2 freeze, no sweep cap,
+    # src/cuda/test_geodesics_ptp_coalescence.cu:223-248): FP64 bit-exact with
+    # the same sweep count, FP32 within 1e-5 when the schedules agree; at
+    # tol 1e-5 it lands within 1e-4 of the relaxation's fixed point.
+    name, xyz, f, src, M, C = case
+    ref, nsw = M.ptp(src, oracle.GPU_COAL | oracle.UNCAPPED, tol=tol)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F64, tol=tol, uncapped=True)
+    mx, _ = rel_err(d, ref)
+    print(f"[uncapped f64] {name} tol {tol:.0e} sweeps {st['sweeps']}/{nsw} max {mx:.3e}", flush=True)
+    assert st["sweeps"] == nsw
+    assert mx <= 1e-12
+    d32, _, st32 = P.ptp(C, src, precision=L.PREC_F32, tol=tol, uncapped=True)
+    mx32, _ = rel_err(d32, ref)
+    print(f"[uncapped f32] {name} tol {tol:.0e} sweeps {st32['sweeps']}/{nsw} max {mx32:.3e}", flush=True)
+    assert mx32 <= (1e-5 if st32["sweeps"] == nsw else 1e-3)
+    if tol <= 1e-5:
+        mxf, _ = rel_err(d, M.fixed_point(src))
+        assert mxf <= 1e-4
+
+
 def test_frontend_ptp_gpu_dispatch(case, oracle):
     name, xyz, f, src, M, C = case
     g = P.geodesics(C, src, P.geodesics.PTP_GPU, precision=L.PREC_F64)

@@ -103,6 +103,14 @@ def test_error_paths():
     # empty source list: toplesets returns no rings, as the reference does
     top, srt, lim = C.compute_toplesets([])
     assert len(lim) == 0
+    # batch jobs are validated before any device work
+    for jobs, status in (([(1, 0)], L.ERR_INVALID), ([(0, len(xyz))], L.ERR_INVALID)):
+        with pytest.raises(P.GPError) as e:
+            P.Batch([C], jobs)
+        assert e.value.status == status
+    with pytest.raises(P.GPError) as e:
+        P.Batch([C], [(0, 0)], tol=0.0)
+    assert e.value.status == L.ERR_INVALID
 
 
 def test_device_probe_reports_missing_gpu():
