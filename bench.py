@@ -12,11 +12,12 @@ Rows:
   * plain_ref_f32 / _f64: the reference window schedule (K/2 freeze,
     PTP_TOL 1e-3, 2*|limits| cap) -- parity with the FP64 oracle of the same
     schedule; it stops ~1e-2 short of the relaxation's fixed point;
-  * plain_uncapped_f32: the reference's uncapped loop at tol 1e-5, the plain
+  * plain_uncapped_f32: the reference's uncapped loop at tol 1e-6, the plain
     ring schedule that reaches the fixed point within 1e-4 ("plain at bar");
-  * patches: the patch schedule (no predictor) at tol 1e-6;
-  * band_euclid / band_learned: the patch schedule gated by the Euclidean or
-    the learned distance prediction (band_learned when a trained model exists).
+  * band_euclid / band_learned: bands of the Euclidean or the learned
+    distance prediction in place of the rings in the same uncapped loop at
+    tol 1e-6 (band_learned when a trained model exists);
+  * patches: the patch schedule (LDS-iterated patches, active list) at tol 1e-6.
 value = ms per field of band_learned when it meets the 1e-4 bar against the
 FP64 fixed point, else of the fastest row that does; the JSON carries every
 row with its errors against both FP64 references.
@@ -77,9 +78,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle error checks")
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline work")
     ap.add_argument("--model", default=os.path.join(ROOT, "gproshan_amd", "models", "band_mlp.npy"))
-    ap.add_argument("--band-width", type=float, default=8.0)
-    ap.add_argument("--local-iters", type=int, default=32)
-    ap.add_argument("--patch", type=int, default=256, help="patch size (256 or 512)")
+    ap.add_argument("--band-width", type=float, default=1.0, help="band width in mean edge lengths")
+    ap.add_argument("--local-iters", type=int, default=4, help="patch schedule: iterations per visit")
+    ap.add_argument("--patch", type=int, default=512, help="patch schedule: patch size (256 or 512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -102,13 +103,13 @@ def main():
 
     rows = {}
     patch_kw = dict(block=args.patch, local_iters=args.local_iters, tol=1e-6)
-    band_kw = dict(band_width=args.band_width, **patch_kw)
+    band_kw = dict(band_width=args.band_width, tol=1e-6)
     configs = [
         ("plain_ref_f32", dict(precision=L.PREC_F32)),
         ("plain_ref_f64", dict(precision=L.PREC_F64)),
-        ("plain_uncapped_f32", dict(precision=L.PREC_F32, tol=1e-5, uncapped=True)),
-        ("patches", dict(schedule=L.SCHED_PATCHES, **patch_kw)),
+        ("plain_uncapped_f32", dict(precision=L.PREC_F32, tol=1e-6, uncapped=True)),
         ("band_euclid", dict(schedule=L.SCHED_EUCLID_BAND, **band_kw)),
+        ("patches", dict(schedule=L.SCHED_PATCHES, **patch_kw)),
     ]
     for name, kw in configs:
         t = time.time()
@@ -167,7 +168,7 @@ def main():
         headline = min(ok, key=lambda n: rows[n]["kernel_ms"])
     h = rows[headline]
     st = h["st"]
-    patch_row = headline in ("patches",) or headline.startswith("band_")
+    patch_row = headline == "patches"
     # Algorithmic bytes: B_relax per vertex relaxation that reads the vertex's
     # tables from memory (window vertices per sweep; for the patch schedules,
     # patch vertices per visit -- the local iterations run in LDS).

@@ -26,7 +26,7 @@ LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
 # Every symbol include/gproshan_ptp.h declares.
 EXPORTS = [
     "gp_ptp_opts_default", "gp_last_error", "gp_device_check",
-    "gp_che_create", "gp_che_destroy", "gp_che_sizes", "gp_che_tables", "gp_che_link",
+    "gp_che_create", "gp_che_destroy", "gp_che_sizes", "gp_che_tables", "gp_che_link", "gp_che_mean_edge",
     "gp_compute_toplesets", "gp_compute_toplesets_gpu",
     "gp_parallel_toplesets_propagation_gpu", "gp_parallel_toplesets_propagation_coalescence_gpu",
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
     L.gp_che_sizes.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(i32)]
     L.gp_che_tables.argtypes = [P, P, P, P]
     L.gp_che_link.argtypes = [P, u32, P, u32, C.POINTER(u32)]
+    L.gp_che_mean_edge.argtypes = [P, C.POINTER(C.c_double)]
     L.gp_compute_toplesets.argtypes = [P, P, u32, P, P, P, u32, C.POINTER(u32)]
     L.gp_compute_toplesets_gpu.argtypes = [P, P, u32, P, P, P, u32, C.POINTER(u32)]
     drv = [P, P, u32, P, u32, P, P, P, C.POINTER(C.c_double)]

@@ -39,8 +39,8 @@ void gp_ptp_opts_default(gp_ptp_opts *o)
 	o->coalesce = 1;
 	o->tol = GP_PTP_TOL;
 	o->launch_mode = GP_LAUNCH_GRAPH;
-	o->band_width = 8.0;
-	o->local_iters = 32;
+	o->band_width = 0.0;
+	o->local_iters = 8;
 }
 
 const char *gp_last_error(void) { return gp::last_error(); }
@@ -107,6 +107,15 @@ int gp_che_link(const gp_che *mesh, uint32_t v, uint32_t *he_out, uint32_t cap, 
 	}
 	*n = c;
 	return c > cap ? gp::fail(GP_ERR_CAPACITY, "link larger than cap") : GP_OK;
+}
+
+int gp_che_mean_edge(const gp_che *mesh, double *h)
+{
+	GP_GUARD(
+		if (!mesh || !h) return gp::fail(GP_ERR_INVALID, "null argument");
+		*h = mesh->m.mean_edge;
+		return GP_OK;
+	)
 }
 
 int gp_compute_toplesets(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t *toplesets,
@@ -254,17 +263,17 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
                            const gp_ptp_opts &o, gp_plan **out)
 {
 	const gp::Che &m = mesh->m;
-	if (o.schedule == GP_SCHED_PATCHES) return gp::plan_create_patches(&mesh->m, sources, n_sources, nullptr, o, out);
+	if (o.schedule == GP_SCHED_PATCHES) {
+		gp_ptp_opts op = o;
+		if (!(op.band_width > 0)) op.band_width = 128.0;
+		return gp::plan_create_patches(&mesh->m, sources, n_sources, dhat, op, out);
+	}
 	if (o.schedule != GP_SCHED_RINGS && o.schedule != GP_SCHED_LEARNED_BAND && o.schedule != GP_SCHED_EUCLID_BAND &&
 	    o.schedule != GP_SCHED_RING_BAND)
 		return gp::fail(GP_ERR_INVALID, "unknown schedule %d", o.schedule);
-	std::vector<uint32_t> lev, sorted, lim;
-	if (o.schedule == GP_SCHED_RINGS || o.schedule == GP_SCHED_RING_BAND) {
-		lev.resize(m.nv);
-		sorted.resize(m.nv);
-		int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
-		if (rc) return rc;
-	}
+	std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
+	int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
+	if (rc) return rc;
 	if (o.schedule == GP_SCHED_RINGS)
 		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
 	std::vector<double> d;
@@ -288,7 +297,18 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 		}
 		dhat = d.data();
 	}
-	return gp::plan_create_patches(&mesh->m, sources, n_sources, dhat, o, out);
+	// Bands of predicted distance take the place of the rings in the
+	// reference's uncapped window loop; the vertices of a band are in Morton
+	// order (the coalesced numbering of the plan).
+	const double width = (o.band_width > 0 ? o.band_width : 1.0) * m.mean_edge;
+	std::vector<uint32_t> order, blim;
+	rc = gp::band_order(m, sources, n_sources, lev.data(), dhat, width, order, blim);
+	if (rc) return rc;
+	gp_ptp_opts ob = o;
+	ob.schedule = GP_SCHED_RINGS;
+	ob.coalesce = 1;
+	ob.uncapped = 1;
+	return gp::plan_create_rings(&mesh->m, sources, n_sources, blim.data(), (uint32_t) blim.size(), order.data(), ob, out);
 }
 
 extern "C" {

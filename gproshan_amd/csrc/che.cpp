@@ -146,9 +146,12 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	}
 	for (uint32_t v = 0; v < nv; v++) m.fan_off[v + 1] += m.fan_off[v];
 	m.fan.assign(m.fan_off[nv], NIL);
-	double esum = 0;
-	uint64_t ecnt = 0;
-	#pragma omp parallel for schedule(static) reduction(+: esum, ecnt)
+	// Mean edge length: per-vertex sums added in vertex order, so the value
+	// (it sets the FP32 unit h and the band widths) does not depend on the
+	// thread count.
+	std::vector<double> vsum(nv, 0.0);
+	std::vector<uint32_t> vcnt(nv, 0);
+	#pragma omp parallel for schedule(static)
 	for (int64_t v = 0; v < (int64_t) nv; v++) {
 		const uint32_t stop = m.evt[v];
 		if (stop == NIL) continue;
@@ -163,11 +166,14 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 		const double *p = &m.gt[3 * (size_t) v];
 		for (uint32_t k = 0; k <= c; k++) {
 			const double *q = &m.gt[3 * (size_t) L[k]];
-			esum += std::sqrt((q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) +
-			                  (q[2] - p[2]) * (q[2] - p[2]));
-			ecnt++;
+			vsum[v] += std::sqrt((q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) +
+			                     (q[2] - p[2]) * (q[2] - p[2]));
+			vcnt[v]++;
 		}
 	}
+	double esum = 0;
+	uint64_t ecnt = 0;
+	for (uint32_t v = 0; v < nv; v++) { esum += vsum[v]; ecnt += vcnt[v]; }
 	m.mean_edge = (ecnt && esum > 0) ? esum / (double) ecnt : 1.0;
 	return GP_OK;
 }

@@ -14,9 +14,8 @@
 // the next list; a patch that hit the iteration cap re-appends itself.  The
 // launch that finds its list empty marks the solve done.
 //
-// Band gating (the band schedules): each patch carries a key, the smallest
-// predicted distance dhat of its vertices (Euclidean, ring level * h, or the
-// learned predictor).  A launch processes only the listed patches whose key
+// Gating by a prediction (when the caller gives one): each patch carries a
+// key, the smallest predicted distance dhat of its vertices.  A launch processes only the listed patches whose key
 // is within band_width * h of the smallest key on the list and carries the
 // others over, so the front moves through the mesh in predicted-distance
 // order and fewer patches are recomputed after a later, shorter arrival.
@@ -47,7 +46,7 @@ struct PatchState {
 	uint32_t count[4];
 	uint32_t kmin[4];       // float bits of the smallest key on each list
 	uint32_t done, launches, pad0, pad1;
-	unsigned long long visits, updates, deferred, appends;
+	unsigned long long visits, updates, deferred, pad2;
 };
 
 struct PArgs {
@@ -60,6 +59,7 @@ struct PArgs {
 	float *dist;
 	uint32_t *list;        // [4][npatch]
 	uint32_t *mark;        // [4][npatch]
+	uint8_t *defer;        // [npatch] launches a listed patch has waited
 	PatchState *st;
 	uint32_t *trace;       // [max_launch][8]
 	float h, inv_h;
@@ -73,7 +73,6 @@ __device__ __forceinline__ void p_append(const PArgs &a, uint32_t slot, uint32_t
 		const uint32_t at = atomicAdd(&a.st->count[slot], 1u);
 		a.list[(size_t) slot * a.npatch + at] = q;
 		atomicMin(&a.st->kmin[slot], __float_as_uint(a.pkey[q]));
-		atomicAdd(&a.st->appends, 1ull);
 	}
 }
 
@@ -81,16 +80,21 @@ __device__ __forceinline__ void p_append(const PArgs &a, uint32_t slot, uint32_t
 // patch (LOCAL_OUT if outside), bit 16 the slot is empty, bit 17 triangle
 // (k, k+1) takes the FP64 update.
 constexpr uint32_t PK_EMPTY = 1u << 16, PK_F64 = 1u << 17;
+constexpr uint32_t FCACHE = 3;     // FP64 triangles per vertex held in LDS
+constexpr uint32_t WAKE_TAB = 64;  // LDS set of the patches a visit wakes
+constexpr uint32_t MAX_DEFER = 8;  // a listed patch waits at most this many launches
 
 template <uint32_t P>
 __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 {
 	__shared__ float sd[2][P];
-	// FP64 geometry of each thread's first FP64-flagged triangle (loaded once
-	// per visit; later flagged triangles of the same vertex, rare, read it
-	// from global memory).
-	__shared__ double fq0[P], fq1[P], fg[P], fdet[P], fS[P];
-	__shared__ uint32_t s_cap;
+	// FP64 geometry of each thread's first FCACHE FP64-flagged triangles,
+	// loaded once per visit, and their inputs of the current iteration (any
+	// further flagged triangle of a vertex -- rare -- reads global memory).
+	__shared__ double fq0[FCACHE][P], fq1[FCACHE][P], fg[FCACHE][P], fdet[FCACHE][P], fS[FCACHE][P];
+	__shared__ float fta[FCACHE][P], ftb[FCACHE][P];
+	__shared__ uint32_t s_wake[WAKE_TAB];
+	__shared__ uint32_t s_cap, s_entry[2];
 	PatchState *st = a.st;
 	const uint32_t cur = k4, nxt = (k4 + 1u) & 3u, rst = (k4 + 2u) & 3u;
 	const uint32_t count = st->count[cur];
@@ -114,12 +118,25 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const float h = a.h, inv_h = a.inv_h;
 	for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-		const uint32_t p = a.list[(size_t) cur * a.npatch + idx];
-		if (tid == 0) a.mark[(size_t) cur * a.npatch + p] = 0;
-		if (!(a.pkey[p] <= T)) {             // outside the band: carry over
-			if (tid == 0) { p_append(a, nxt, p); atomicAdd(&st->deferred, 1ull); }
-			continue;
+		// Thread 0 takes the entry and decides for the workgroup: a patch
+		// outside the band is carried over (at most MAX_DEFER times running).
+		if (tid == 0) {
+			const uint32_t q = a.list[(size_t) cur * a.npatch + idx];
+			a.mark[(size_t) cur * a.npatch + q] = 0;
+			const bool go = a.pkey[q] <= T || a.defer[q] >= MAX_DEFER;
+			if (!go) {
+				a.defer[q]++;
+				p_append(a, nxt, q);
+				atomicAdd(&st->deferred, 1ull);
+			}
+			s_entry[0] = q;
+			s_entry[1] = go ? 1u : 0u;
 		}
+		__syncthreads();
+		const uint32_t p = s_entry[0];
+		const bool go = s_entry[1] != 0;
+		__syncthreads();
+		if (!go) continue;
 		const uint32_t pos = p * P + tid;
 		const bool live = pos < a.n;
 		uint32_t S = 0;
@@ -134,7 +151,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 		uint32_t pk[MAXS];
 		float4 g[MAXS];
 		float tout[MAXS];
-		uint32_t fk = MAXS;                  // slot of the cached FP64 triangle
+		uint32_t fks = 0xFFFu, nf = 0;       // cached FP64 slots, 4 bits each (0xF: none)
 #pragma unroll
 		for (uint32_t k = 0; k < MAXS; k++) {
 			pk[k] = LOCAL_OUT | PK_EMPTY;
@@ -145,16 +162,21 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 				pk[k] = li | ((id & SLOT_EMPTY) ? PK_EMPTY : 0u) | ((id & F64_FLAG) ? PK_F64 : 0u);
 				g[k] = a.cgeo[base + 64 * k];
 				if (li == LOCAL_OUT && !(id & SLOT_EMPTY)) tout[k] = a.dist[id & ID30_MASK];
-				if ((id & F64_FLAG) && fk == MAXS) fk = k;
+				if ((id & F64_FLAG) && nf < FCACHE) {
+					fks = (fks & ~(0xFu << (4 * nf))) | (k << (4 * nf));
+					nf++;
+				}
 			}
 		}
-		if (fk != MAXS) {
-			const size_t at = base + 64 * (size_t) fk;
+#pragma unroll 1
+		for (uint32_t c = 0; c < nf; c++) {
+			const size_t at = base + 64 * (size_t) ((fks >> (4 * c)) & 0xFu);
 			const double4 gd = a.cgeo64[at];
-			fq0[tid] = gd.x; fg[tid] = gd.y; fdet[tid] = gd.z; fS[tid] = gd.w;
-			fq1[tid] = a.cgeo64[at + 64].x;
+			fq0[c][tid] = gd.x; fg[c][tid] = gd.y; fdet[c][tid] = gd.z; fS[c][tid] = gd.w;
+			fq1[c][tid] = a.cgeo64[at + 64].x;
 		}
 		sd[0][tid] = v0;
+		if (tid < WAKE_TAB) s_wake[tid] = 0;
 		if (tid == 0) s_cap = 0;
 		__syncthreads();
 		uint32_t b = 0, it = 0;
@@ -163,10 +185,8 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 			float best = curv;
 			if (fast) {
 				// FP32 triangles inline; the FP64-flagged ones after the slot
-				// loop (the cached one from LDS, any other from global memory),
-				// so the FP64 code is emitted once, not per unrolled slot.
+				// loop, so the FP64 code is emitted once, not per unrolled slot.
 				float ta = (pk[0] & 0xFFFFu) != LOCAL_OUT ? sd[b][pk[0] & 0xFFFFu] : tout[0];
-				float fa = (float) INFINITY, fb = (float) INFINITY;
 				uint32_t fmore = 0;
 #pragma unroll
 				for (uint32_t k = 0; k + 1 < MAXS; k++) {
@@ -176,17 +196,23 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 						if (!(pk[k] & PK_F64)) {
 							const float q = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, ta, tb, h, inv_h);
 							if (q < best) best = q;
-						} else if (k == fk) {
-							fa = ta; fb = tb;
+						} else if ((fks & 0xFu) == k) {
+							fta[0][tid] = ta; ftb[0][tid] = tb;
+						} else if (((fks >> 4) & 0xFu) == k) {
+							fta[1][tid] = ta; ftb[1][tid] = tb;
+						} else if (((fks >> 8) & 0xFu) == k) {
+							fta[2][tid] = ta; ftb[2][tid] = tb;
 						} else {
 							fmore |= 1u << k;
 						}
 						ta = tb;
 					}
 				}
-				if (fk != MAXS) {
-					const float q = (float) update_stable<double>(fq0[tid], fq1[tid], fg[tid], fdet[tid], fS[tid], (double) fa,
-					                                              (double) fb, (double) h, (double) inv_h);
+#pragma unroll 1
+				for (uint32_t c = 0; c < nf; c++) {
+					const float q = (float) update_stable<double>(fq0[c][tid], fq1[c][tid], fg[c][tid], fdet[c][tid],
+					                                              fS[c][tid], (double) fta[c][tid], (double) ftb[c][tid],
+					                                              (double) h, (double) inv_h);
 					if (q < best) best = q;
 				}
 				while (fmore) {
@@ -224,16 +250,27 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 		}
 		const float v = sd[b][tid];
 		if (live && v < v0) a.dist[pos] = v;
-		// Wake the patches across the boundary of every vertex that moved.
+		// Patches across the boundary of every vertex that moved: collected
+		// into an LDS set first, so each is appended once per visit.
 		if (live && v < v0 && (!(v0 < INFINITY) || (v0 - v) > a.tol * v)) {
 			for (uint32_t k = 0; k < S; k++) {
 				const uint32_t idk = a.cid[base + 64 * k];
-				if (!(idk & SLOT_EMPTY) && a.lidx[base + 64 * k] == LOCAL_OUT) p_append(a, nxt, (idk & ID30_MASK) / P);
+				if ((idk & SLOT_EMPTY) || a.lidx[base + 64 * k] != LOCAL_OUT) continue;
+				const uint32_t q = (idk & ID30_MASK) / P;
+				uint32_t hsh = q & (WAKE_TAB - 1u), probe = 0;
+				for (; probe < WAKE_TAB; probe++) {
+					const uint32_t old = atomicCAS(&s_wake[hsh], 0u, q + 1u);
+					if (old == 0u || old == q + 1u) break;
+					hsh = (hsh + 1u) & (WAKE_TAB - 1u);
+				}
+				if (probe == WAKE_TAB) p_append(a, nxt, q);   // table full: append directly
 			}
 		}
 		if (it == a.local_iters && tid == 0) s_cap = 1;
 		__syncthreads();
+		if (tid < WAKE_TAB && s_wake[tid]) p_append(a, nxt, s_wake[tid] - 1u);
 		if (tid == 0) {
+			a.defer[p] = 0;
 			if (s_cap) p_append(a, nxt, p);
 			atomicAdd(&st->visits, 1ull);
 			atomicAdd(&st->updates, (unsigned long long) P * (it + (it < a.local_iters ? 1u : 0u)));
@@ -241,11 +278,13 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 	}
 }
 
-__global__ void k_patch_init(float *dist, uint32_t n, uint32_t *mark, uint32_t nmark, PatchState *st)
+__global__ void k_patch_init(float *dist, uint32_t n, uint32_t *mark, uint32_t nmark, uint8_t *defer, uint32_t np,
+                             PatchState *st)
 {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	for (uint32_t v = t; v < n; v += gridDim.x * blockDim.x) dist[v] = INFINITY;
 	for (uint32_t q = t; q < nmark; q += gridDim.x * blockDim.x) mark[q] = 0;
+	for (uint32_t q = t; q < np; q += gridDim.x * blockDim.x) defer[q] = 0;
 	if (t == 0) {
 		uint32_t *w = (uint32_t *) st;
 		for (uint32_t q = 0; q < sizeof(PatchState) / 4; q++) w[q] = 0;
@@ -286,6 +325,7 @@ struct PatchPlan {
 	uint32_t *d_coff = nullptr, *d_cid = nullptr, *d_list = nullptr, *d_mark = nullptr, *d_trace = nullptr,
 	         *d_src = nullptr;
 	uint16_t *d_lidx = nullptr;
+	uint8_t *d_defer = nullptr;
 	float4 *d_cgeo = nullptr;
 	double4 *d_cgeo64 = nullptr;
 	float *d_pkey = nullptr, *d_dist = nullptr;
@@ -300,7 +340,7 @@ struct PatchPlan {
 	{
 		for (void *p : { (void *) d_coff, (void *) d_cid, (void *) d_list, (void *) d_mark, (void *) d_trace,
 		                 (void *) d_src, (void *) d_lidx, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_pkey,
-		                 (void *) d_dist, (void *) d_out, (void *) d_state })
+		                 (void *) d_dist, (void *) d_out, (void *) d_state, (void *) d_defer })
 			(void) hipFree(p);
 		if (h_state) (void) hipHostFree(h_state);
 		if (graph) (void) hipGraphExecDestroy(graph);
@@ -430,7 +470,7 @@ int patch_create(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat
 		return rc;
 	const uint32_t trace_rows = std::min<uint32_t>(pp->max_launch, 1u << 20);
 	if (hipMalloc(&pp->d_list, 4ull * 4 * pp->npatch) != hipSuccess ||
-	    hipMalloc(&pp->d_mark, 4ull * 4 * pp->npatch) != hipSuccess ||
+	    hipMalloc(&pp->d_mark, 4ull * 4 * pp->npatch) != hipSuccess || hipMalloc(&pp->d_defer, pp->npatch) != hipSuccess ||
 	    hipMalloc(&pp->d_trace, 32ull * trace_rows) != hipSuccess || hipMalloc(&pp->d_dist, 4ull * pp->n) != hipSuccess ||
 	    hipMalloc(&pp->d_out, 8ull * pp->n) != hipSuccess || hipMalloc(&pp->d_state, sizeof(PatchState)) != hipSuccess ||
 	    hipHostMalloc(&pp->h_state, sizeof(PatchState)) != hipSuccess)
@@ -448,6 +488,7 @@ static PArgs patch_args(const PatchPlan *pp)
 	PArgs a{};
 	a.coff = pp->d_coff; a.cid = pp->d_cid; a.lidx = pp->d_lidx; a.cgeo = pp->d_cgeo; a.cgeo64 = pp->d_cgeo64;
 	a.cx64 = nullptr; a.pkey = pp->d_pkey; a.dist = pp->d_dist; a.list = pp->d_list; a.mark = pp->d_mark;
+	a.defer = pp->d_defer;
 	a.st = pp->d_state; a.trace = pp->d_trace;
 	a.h = pp->ren->dev->h; a.inv_h = pp->ren->dev->inv_h;
 	a.tol = (float) pp->opts.tol; a.width = pp->width;
@@ -482,7 +523,8 @@ int patch_run(PatchPlan *pp, gp_ptp_stats *stats)
 		if (e != hipSuccess) return fail(GP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
 	}
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (pp->n + 255) / 256));
-	k_patch_init<<<ig, 256, 0, pp->stream>>>(pp->d_dist, pp->n, pp->d_mark, 4 * pp->npatch, pp->d_state);
+	k_patch_init<<<ig, 256, 0, pp->stream>>>(pp->d_dist, pp->n, pp->d_mark, 4 * pp->npatch, pp->d_defer, pp->npatch,
+	                                         pp->d_state);
 	k_patch_seed<<<1, 64, 0, pp->stream>>>(a, pp->d_src, pp->ns, pp->P);
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipEventRecord(pp->e0, pp->stream));

@@ -99,6 +99,11 @@ class che:
         check(lib().gp_che_link(self._h, v, ptr(out), 256, C.byref(n)))
         return out[:n.value].copy()
 
+    def mean_edge(self):
+        h = C.c_double()
+        check(lib().gp_che_mean_edge(self._h, C.byref(h)))
+        return h.value
+
     def compute_toplesets(self, sources, gpu: bool = False):
         """Returns (toplesets u32[V], sorted u32[V], limits u32[R+1])."""
         src = _u32(sources)

@@ -54,21 +54,23 @@ enum gp_result_buffer {
 };
 
 /* Schedule.  GP_SCHED_RINGS is the reference's window schedule over BFS
- * rings.  The others have no reference counterpart and run the same FP32
- * relaxation to its fixed point instead (patches.hip): vertices in Morton
- * order, patches of `block` consecutive vertices iterated in LDS up to
- * local_iters times, an active list of patches woken by changed neighbours;
- * the band schedules also gate the list by a predicted distance: a launch
- * processes the listed patches whose smallest predicted distance is within
- * band_width mean edge lengths of the list's smallest, so the front sweeps
- * the mesh in predicted order.  They return the fixed point to about tol
- * (relative), not the reference's truncated schedule. */
+ * rings (gp_ptp_opts.uncapped selects its uncapped iter_error loop).  The
+ * band schedules have no reference counterpart: a predicted distance dhat
+ * groups the vertices into bands of band_width mean edge lengths (sources
+ * first, Morton order inside a band), and the bands take the place of the
+ * rings in the uncapped window loop -- the prediction only reorders the
+ * propagation.  GP_SCHED_PATCHES runs the same FP32 relaxation to its fixed
+ * point by Morton-ordered patches of `block` vertices iterated in LDS (up to
+ * local_iters times per visit) with an active list of patches woken by
+ * changed neighbours; given a dhat (gp_plan_create_bands) it only processes
+ * the listed patches whose smallest prediction is within band_width mean edge
+ * lengths of the list's smallest.  The patch schedule runs in FP32. */
 enum gp_schedule {
 	GP_SCHED_RINGS = 0,        /* toplesets (BFS rings), the reference schedule                     */
-	GP_SCHED_LEARNED_BAND = 1, /* patches gated by the learned distance predictor (gp_mlp_*)        */
-	GP_SCHED_EUCLID_BAND = 2,  /* patches gated by the Euclidean distance to the nearest source     */
-	GP_SCHED_RING_BAND = 3,    /* patches gated by BFS ring level * mean edge length                */
-	GP_SCHED_PATCHES = 4       /* patches, no gating (ablation: no predictor)                       */
+	GP_SCHED_LEARNED_BAND = 1, /* bands of the learned distance predictor (gp_mlp_*)                */
+	GP_SCHED_EUCLID_BAND = 2,  /* bands of the Euclidean distance to the nearest source (ablation)  */
+	GP_SCHED_RING_BAND = 3,    /* bands of BFS level * mean edge length (ablation: no predictor)    */
+	GP_SCHED_PATCHES = 4       /* patch solver (optionally gated by a caller's dhat)                */
 };
 
 /* How the sweeps are launched. */
@@ -87,7 +89,8 @@ typedef struct gp_ptp_opts {
 	uint32_t block;     /* patch schedules: patch size 256 or 512 (0: 256)       */
 	uint32_t grid;      /* workgroups (0: autotuned/default)                     */
 	int launch_mode;    /* gp_launch_mode                                        */
-	double band_width;  /* band schedules: gate width in mean edge lengths       */
+	double band_width;  /* band width / patch gate in mean edge lengths (0: 1 for
+	                       bands, 128 for the patch gate)                       */
 	uint32_t local_iters;/* patch schedules: iteration cap per patch visit       */
 	int uncapped;       /* ring schedule: 1 = the reference's uncapped iter_error
 	                       loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248):
@@ -109,7 +112,7 @@ typedef struct gp_ptp_stats {
 } gp_ptp_stats;
 
 /* Fill opts with the defaults (FP32, last written, rings, coalesce, PTP_TOL;
- * patch schedules: block 256, local_iters 32, band_width 8). */
+ * band_width 0 (schedule default), patches: block 256, local_iters 8). */
 void gp_ptp_opts_default(gp_ptp_opts *opts);
 
 /* Text of the last error on this thread ("" if none). */
@@ -141,6 +144,10 @@ int gp_che_tables(const gp_che *mesh, uint32_t *vt, uint32_t *ot, uint32_t *evt)
 
 /* che::link (src/che.cpp:102-112): half-edge ids of the link of v. */
 int gp_che_link(const gp_che *mesh, uint32_t v, uint32_t *he_out, uint32_t cap, uint32_t *n);
+
+/* Mean edge length over the fans (no reference counterpart): the unit h of
+ * the FP32 solve and of band_width. */
+int gp_che_mean_edge(const gp_che *mesh, double *h);
 
 /* ---- toplesets ----------------------------------------------------------- */
 

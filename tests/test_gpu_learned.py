@@ -1,0 +1,79 @@
+"""What the trained band predictor gains over the method without it.
+
+The committed model (gproshan_amd/models/band_mlp.npy, trained on the GPU by
+gproshan_amd/train.py on 2,000 seeded meshes) is evaluated on held-out meshes
+(seeds the training never drew):
+  * prediction: the network's distance estimate is closer to the converged
+    field than its own Euclidean input feature (mean |log(dhat / d)|);
+  * schedule: bands of the learned prediction, in place of rings in the
+    uncapped window loop, reach the same fixed point (<= 1e-4 vs the oracle)
+    with fewer vertex relaxations than bands of the Euclidean distance of the
+    same width.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from gproshan_amd import _lib as L
+from gproshan_amd import geodesics as P
+from gproshan_amd import learned as Lr
+from gproshan_amd import meshgen as G
+
+pytestmark = pytest.mark.gpu
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gproshan_amd", "models",
+                     "band_mlp.npy")
+
+HELD_OUT = [
+    ("hf320_s9001", lambda: G.height_field(320, 9001)),
+    ("rect200x450_s9002", lambda: G.height_field_rect(200, 450, 9002)),
+    ("ico6_s9003", lambda: G.icosphere(6, 0.01, 9003)),
+]
+
+
+@pytest.fixture(scope="module")
+def net():
+    assert os.path.exists(MODEL), "trained model missing"
+    return Lr.MLP(params=np.load(MODEL, allow_pickle=False))
+
+
+@pytest.fixture(scope="module", params=HELD_OUT, ids=[c[0] for c in HELD_OUT])
+def case(request, oracle):
+    name, gen = request.param
+    xyz, f = gen()
+    src = G.draw_sources(np.random.default_rng(77), len(xyz), 1)
+    M = oracle.Mesh(xyz, f)
+    print(f"\n[learned] {name} V={len(xyz)} source={src.tolist()}", flush=True)
+    return name, src, P.che(xyz, f), M.fixed_point(src)
+
+
+def test_prediction_beats_euclidean(case, net):
+    name, src, C, fp = case
+    dhat, ms = net.predict(C, src)
+    _, e = Lr.features(C, src)
+    m = np.isfinite(fp) & (fp > 0) & (e > 0)
+    err_net = float(np.mean(np.abs(np.log(dhat[m] / fp[m]))))
+    err_euc = float(np.mean(np.abs(np.log(e[m] / fp[m]))))
+    print(f"[learned] {name} mean |log(dhat/d)| network {err_net:.4e} euclidean {err_euc:.4e} "
+          f"(features + inference {ms:.3f} ms)", flush=True)
+    assert err_net < err_euc
+
+
+def test_learned_bands_save_relaxations(case, net):
+    name, src, C, fp = case
+    dhat, _ = net.predict(C, src)
+    kw = dict(band_width=1.0, tol=1e-6)
+    out = {}
+    for tag, plan in (("learned", P.Plan(C, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, **kw)),
+                      ("euclid", P.Plan(C, src, schedule=L.SCHED_EUCLID_BAND, **kw)),
+                      ("rings", P.Plan(C, src, tol=1e-6, uncapped=True))):
+        st = plan.run()
+        mx, _ = rel_err(plan.result(), fp)
+        plan.close()
+        out[tag] = st
+        print(f"[learned] {name} {tag}: sweeps {st['sweeps']} relaxations {st['relaxations']} max err {mx:.2e} "
+              f"kernel {st['kernel_ms']:.2f} ms", flush=True)
+        assert mx <= 1e-4
+    assert out["learned"]["relaxations"] < out["euclid"]["relaxations"]
