@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--band-width", type=float, default=1.0, help="band width in mean edge lengths")
     ap.add_argument("--local-iters", type=int, default=4, help="patch schedule: iterations per visit")
     ap.add_argument("--patch", type=int, default=512, help="patch schedule: patch size (256 or 512)")
+    ap.add_argument("--stress", action="store_true", help="precision stress case instead (2048^2 1:100 grid, 4 sources)")
+    ap.add_argument("--batch", action="store_true", help="batched case instead (256 meshes, one launch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -89,6 +91,10 @@ def main():
     from gproshan_amd import _lib as L
     from gproshan_amd import geodesics as P
     from gproshan_amd import meshgen as G
+    if args.stress:
+        return stress(args, L, P, G)
+    if args.batch:
+        return batch(args, L, P, G)
 
     t = time.time()
     xyz, faces = G.height_field(args.n, args.seed)
@@ -205,6 +211,100 @@ def main():
     }
     if cpu:
         out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def stress(args, L, P, G):
+    """2048 x 2048 grid stretched 1:100 in x, jitter U(-0.4, 0.4) cells, 4
+    sources: FP32 with the FP64 fallback vs all-FP64 GPU vs the FP64 oracle
+    (the reference schedule; the oracle runs on the host cores)."""
+    t = time.time()
+    xyz, faces = G.anisotropic_grid(2048, 100.0, 0.4, args.seed)
+    src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), 4)
+    mesh = P.che(xyz, faces)
+    log(f"[stress] V={mesh.n_vertices()} sources={src.tolist()} ({time.time() - t:.1f}s)")
+    rows = {}
+    for name, prec in (("f32_fallback", L.PREC_F32), ("f64", L.PREC_F64)):
+        plan = P.Plan(mesh, src, precision=prec)
+        rows[name] = time_plan(plan, args.steps, args.warmup)
+        plan.close()
+        log(f"[stress] {name}: {rows[name]['kernel_ms']:.2f} ms, sweeps {rows[name]['st']['sweeps']}")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    om = O.Mesh(xyz, faces)
+    t0 = time.perf_counter()
+    ref, nsw = om.ptp(src, O.GPU_COAL, n_threads=os.cpu_count() or 1)
+    cpu_s = time.perf_counter() - t0
+    out = {"metric": "C5 precision stress: max relative error vs the FP64 oracle", "unit": "relative error",
+           "config": {"workload": "anisotropic grid 2048x2048 (4194304 vertices), stretch 1:100, jitter 0.4, "
+                                  f"4 sources, seed {args.seed}", "sources": src.tolist()},
+           "oracle": {"sweeps": nsw, "seconds": round(cpu_s, 2), "cores": os.cpu_count()}, "rows": {}}
+    for name, r in rows.items():
+        mx, mn = rel_err(r["dist"], ref)
+        st = r["st"]
+        out["rows"][name] = {"kernel_ms": round(r["kernel_ms"], 3), "sweeps": st["sweeps"], "max_rel_err": mx,
+                             "mean_rel_err": mn, "f64_updates": st["f64_updates"], "relaxations": st["relaxations"]}
+        log(f"[stress] {name}: max {mx:.3e} mean {mn:.3e} vs FP64 oracle ({nsw} sweeps, {cpu_s:.1f}s on the host)")
+    print(json.dumps(out), flush=True)
+
+
+def batch(args, L, P, G):
+    """256 meshes (grid height fields and displaced icospheres), vertex counts
+    log-uniform in [2k, 200k], U{1..64} sources each; every (mesh, source)
+    pair is its own one-source field, all in one persistent launch."""
+    import math
+    rng = np.random.default_rng(args.seed)
+    t = time.time()
+    meshes, raw, jobs = [], [], []
+    for k in range(256):
+        nv = math.exp(rng.uniform(math.log(2000), math.log(200000)))
+        if rng.random() < 0.75:
+            rows = max(16, int(round(math.sqrt(nv) * rng.uniform(0.8, 1.25))))
+            cols = max(16, int(round(nv / rows)))
+            xyz, f = G.height_field_rect(rows, cols, seed=int(rng.integers(1 << 31)))
+        else:
+            level = int(min(range(4, 8), key=lambda l: abs(math.log(10 * 4 ** l + 2) - math.log(nv))))
+            xyz, f = G.icosphere(level, 0.01, seed=int(rng.integers(1 << 31)))
+        meshes.append(P.che(xyz, f))
+        raw.append((xyz, f))
+        for s in G.draw_sources(rng, len(xyz), int(rng.integers(1, 65))):
+            jobs.append((k, int(s)))
+    nvf = sum(meshes[m].n_vertices() for m, _ in jobs)
+    log(f"[batch] {len(meshes)} meshes, {len(jobs)} fields, {nvf} vertex-fields ({time.time() - t:.1f}s)")
+    t = time.time()
+    b = P.Batch(meshes, jobs, block=0)
+    log(f"[batch] created and autotuned in {time.time() - t:.1f}s")
+    for _ in range(args.warmup):
+        b.run()
+    ms = []
+    for _ in range(args.steps):
+        st = b.run()
+        ms.append(st["kernel_ms"])
+    kms = float(np.mean(ms))
+    gbs = st["relaxations"] * B_RELAX_F32 / (kms * 1e-3) / 1e9
+    log(f"[batch] {kms:.2f} ms per launch, block {st['block']}, grid {st['grid']}, sweeps {st['sweeps']}, "
+        f"relaxations {st['relaxations']}, {gbs:.1f} GB/s algorithmic")
+    # Spot-check the smallest fields against the FP64 oracle.
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    small = sorted(range(len(jobs)), key=lambda q: meshes[jobs[q][0]].n_vertices())[:8]
+    worst = 0.0
+    for q in small:
+        m, s = jobs[q]
+        ref, _ = O.Mesh(*raw[m]).ptp(np.array([s], np.uint32), O.GPU_PLAIN)
+        d, _ = b.result(q)
+        worst = max(worst, rel_err(d, ref)[0])
+    log(f"[batch] max relative error of the 8 smallest fields vs the FP64 oracle: {worst:.2e}")
+    out = {"metric": "C3 batched PTP: ms per launch of all fields", "value": round(kms, 3), "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": f"256 meshes 2k-200k vertices, {len(jobs)} one-source fields, seed {args.seed}",
+                      "vertex_fields": nvf, "block": st["block"], "grid": st["grid"]},
+           "fields_per_s": round(len(jobs) / (kms * 1e-3), 1),
+           "vertex_fields_per_s": round(nvf / (kms * 1e-3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None, "kernel": f"gp::k_batch<{st['block']}u>"},
+           "spot_check_max_rel_err": worst}
     print(json.dumps(out), flush=True)
 
 
