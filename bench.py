@@ -75,6 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1024, help="grid side")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--sources", type=int, default=1, help="sources of the one field (1 or 16 in the workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle error checks")
     ap.add_argument("--cpu-budget", type=float, default=30.0, help="seconds of CPU baseline work")
     ap.add_argument("--model", default=os.path.join(ROOT, "gproshan_amd", "models", "band_mlp.npy"))
@@ -83,10 +84,23 @@ def main():
     ap.add_argument("--patch", type=int, default=512, help="patch schedule: patch size (256 or 512)")
     ap.add_argument("--stress", action="store_true", help="precision stress case instead (2048^2 1:100 grid, 4 sources)")
     ap.add_argument("--batch", action="store_true", help="batched case instead (256 meshes, one launch)")
+    ap.add_argument("--only", default="", help="comma-separated row names to run (default: all)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 256 MiB five times (to scale FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 64 << 20                                  # 64 Mi floats = 256 MiB each way
+        x = torch.ones(n, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
 
     from gproshan_amd import _lib as L
     from gproshan_amd import geodesics as P
@@ -98,7 +112,7 @@ def main():
 
     t = time.time()
     xyz, faces = G.height_field(args.n, args.seed)
-    src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), 1)
+    src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), args.sources)
     mesh = P.che(xyz, faces)
     log(f"[bench] mesh V={mesh.n_vertices()} F={mesh.n_faces()} source={src.tolist()} ({time.time() - t:.1f}s)")
 
@@ -117,7 +131,10 @@ def main():
         ("band_euclid", dict(schedule=L.SCHED_EUCLID_BAND, **band_kw)),
         ("patches", dict(schedule=L.SCHED_PATCHES, **patch_kw)),
     ]
+    only = set(filter(None, args.only.split(",")))
     for name, kw in configs:
+        if only and name not in only:
+            continue
         t = time.time()
         plan = P.Plan(mesh, src, **kw)
         rows[name] = time_plan(plan, args.steps, args.warmup)
@@ -125,7 +142,7 @@ def main():
         st = rows[name]["st"]
         log(f"[bench] {name}: {rows[name]['kernel_ms']:.3f} ms/field, sweeps {st['sweeps']}, "
             f"relaxations {st['relaxations']}, visits {st['patch_visits']} (plan {time.time() - t:.1f}s)")
-    if net is not None:
+    if net is not None and (not only or "band_learned" in only):
         dhat, pred_ms = net.predict(mesh, src)
         plan = P.Plan(mesh, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, **band_kw)
         rows["band_learned"] = time_plan(plan, args.steps, args.warmup)
@@ -167,7 +184,7 @@ def main():
     # Headline: learned bands if they meet the bar, else the fastest row that does.
     def meets(name):
         return name in errs and errs[name]["max_vs_fixed_point"] <= ERR_BAR
-    candidates = [n for n in rows if n.startswith("band_") or n == "patches" or n == "plain_uncapped_f32"]
+    candidates = [n for n in rows if n.startswith("band_") or n == "patches" or n == "plain_uncapped_f32"] or list(rows)
     headline = "band_learned" if ("band_learned" in rows and (args.no_cpu or meets("band_learned"))) else None
     if headline is None:
         ok = [n for n in candidates if meets(n)] or candidates
@@ -194,14 +211,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"height field {args.n}x{args.n} ({mesh.n_vertices()} vertices), 1 source, seed "
+        "config": {"workload": f"height field {args.n}x{args.n} ({mesh.n_vertices()} vertices), {args.sources} "
+                               f"source(s), seed "
                                f"{args.seed}; value = {headline}", "sources": src.tolist(),
                    "band_width": args.band_width, "local_iters": args.local_iters, "patch": args.patch},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
                      "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false>"},
-        "speedup_vs_plain_at_bar": round(rows["plain_uncapped_f32"]["kernel_ms"] / h["kernel_ms"], 3),
-        "speedup_vs_plain_ref": round(rows["plain_ref_f32"]["kernel_ms"] / h["kernel_ms"], 3),
+        "speedup_vs_plain_at_bar": (round(rows["plain_uncapped_f32"]["kernel_ms"] / h["kernel_ms"], 3)
+                                    if "plain_uncapped_f32" in rows else None),
+        "speedup_vs_plain_ref": (round(rows["plain_ref_f32"]["kernel_ms"] / h["kernel_ms"], 3)
+                                 if "plain_ref_f32" in rows else None),
         "rows": {n: {"kernel_ms": round(r["kernel_ms"], 4), "kernel_ms_min": round(r["kernel_ms_min"], 4),
                      "sweeps": r["st"]["sweeps"], "relaxations": r["st"]["relaxations"],
                      "f64_updates": r["st"]["f64_updates"], "patch_visits": r["st"]["patch_visits"],

@@ -6,10 +6,14 @@
 // runs it entirely by itself: the ring build (the same BFS as the toplesets
 // kernel), then the reference window schedule (K/2 band limit, PTP_TOL test,
 // 2*|limits| cap, last-written buffer) with __syncthreads as the only barrier,
-// relaxing vertices through the mesh's ELL slot layout (original numbering,
-// sorted[] indirection as in the reference's multi-source driver,
-// src/cuda/geodesics_ptp.cu:174-299).  Every workgroup leaves the loop when the
-// queue is exhausted, so the launch always drains.
+// relaxing vertices through sorted[] (as the reference's non-coalesced
+// driver does, src/cuda/geodesics_ptp.cu:174-299).  A field's window is in
+// BFS order, not memory order, so each mesh is stored vertex-major (a vertex's
+// slots are contiguous) and renumbered in Morton order (a window's vertices
+// share cache lines); the renumbering keeps every fan's order, hence the
+// rings and the arithmetic, and the output is scattered back.  Every
+// workgroup leaves the loop when the queue is exhausted, so the launch always
+// drains.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +37,7 @@ struct BMesh {
 	const float4 *cgeo;
 	const double4 *cgeo64;
 	const double4 *cx64;
+	const uint32_t *perm;   // Morton position -> original vertex id
 	float h, inv_h;
 	uint32_t nv, pad;
 };
@@ -100,9 +105,9 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 			uint32_t bad = 0;
 			for (uint32_t pos = start + tid; pos < end; pos += nt) {
 				const uint32_t v = sorted[pos];
-				const uint32_t c = v >> 6, cb = M.coff[c];
+				const uint32_t cb = M.coff[v];
 				const float old = od[v];
-				const float best = relax_lane<false>(M, (size_t) cb * 64 + (v & 63u), M.coff[c + 1] - cb, od, old);
+				const float best = relax_lane<false, 1>(M, (size_t) cb, M.coff[v + 1] - cb, od, old);
 				nd[v] = best;
 				if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
 			}
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 		}
 		const float *res = d ? d1 : d0;            // the buffer the last sweep wrote
 		float *out = a.out + a.jobs[job].out;
-		for (uint32_t v = tid; v < nv; v += nt) out[v] = res[v];
+		for (uint32_t v = tid; v < nv; v += nt) out[M.perm[v]] = res[v];
 		if (tid == 0) { a.jobs[job].sweeps = iter; a.jobs[job].relax = relax; }
 		__syncthreads();
 	}
@@ -132,7 +137,9 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 
 struct gp_batch {
 	struct MeshTables {
-		uint32_t *coff = nullptr, *cid = nullptr;
+		std::unique_ptr<gp::Che> ren;   // Morton-numbered copy of the mesh
+		std::vector<uint32_t> inv;      // original -> Morton numbering
+		uint32_t *coff = nullptr, *cid = nullptr, *perm = nullptr;
 		float4 *cgeo = nullptr;
 		double4 *cgeo64 = nullptr;
 	};
@@ -170,7 +177,8 @@ struct gp_batch {
 	{
 		free_scratch();
 		for (auto &t : tabs)
-			for (void *p : { (void *) t.coff, (void *) t.cid, (void *) t.cgeo, (void *) t.cgeo64 }) (void) hipFree(p);
+			for (void *p : { (void *) t.coff, (void *) t.cid, (void *) t.cgeo, (void *) t.cgeo64, (void *) t.perm })
+				(void) hipFree(p);
 		for (void *p : { (void *) d_meshes, (void *) d_jobs, (void *) d_queue, (void *) d_out }) (void) hipFree(p);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
@@ -270,21 +278,28 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 	std::vector<BMesh> bm(nmeshes);
 	b->tabs.resize(nmeshes);
 	for (uint32_t k = 0; k < nmeshes; k++) {
-		Che &m = *meshes[k];
-		if (m.nv >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "batch: mesh %u has too many vertices", k);
-		for (uint32_t v = 0; v < m.nv; v++)
-			if (m.fan_off[v + 1] - m.fan_off[v] > MAX_LINK)
+		const Che &m0 = *meshes[k];
+		if (m0.nv >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "batch: mesh %u has too many vertices", k);
+		for (uint32_t v = 0; v < m0.nv; v++)
+			if (m0.fan_off[v + 1] - m0.fan_off[v] > MAX_LINK)
 				return fail(GP_ERR_CAPACITY, "batch: mesh %u vertex %u: link over %u", k, v, MAX_LINK);
-		int rc = upload_mesh(m);
-		if (rc) return rc;
-		Layout L;
-		if ((rc = build_layout(m, nullptr, m.nv, false, L))) return rc;
 		auto &t = b->tabs[k];
+		std::vector<uint32_t> order;
+		morton_order(m0, order);
+		t.ren = std::make_unique<Che>();
+		int rc = coalesce(m0, order.data(), m0.nv, *t.ren, t.inv);
+		if (rc) return rc;
+		Che &m = *t.ren;
+		m.mean_edge = m0.mean_edge;      // the same h as the mesh itself
+		if ((rc = upload_mesh(m))) return rc;
+		Layout L;
+		if ((rc = build_layout(m, nullptr, m.nv, false, L, 0, 1))) return rc;
 		if ((rc = up(t.coff, L.coff.data(), L.coff.size())) || (rc = up(t.cid, L.cid.data(), L.cid.size())) ||
 		    (rc = up(t.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
-		    (rc = up(t.cgeo64, reinterpret_cast<const double4 *>(L.cgeo64.data()), L.cgeo64.size())))
+		    (rc = up(t.cgeo64, reinterpret_cast<const double4 *>(L.cgeo64.data()), L.cgeo64.size())) ||
+		    (rc = up(t.perm, order.data(), order.size())))
 			return rc;
-		bm[k] = BMesh{ m.dev->fan_off, m.dev->fan, m.dev->closed, t.coff, t.cid, t.cgeo, t.cgeo64, nullptr,
+		bm[k] = BMesh{ m.dev->fan_off, m.dev->fan, m.dev->closed, t.coff, t.cid, t.cgeo, t.cgeo64, nullptr, t.perm,
 		               m.dev->h, m.dev->inv_h, m.nv, 0 };
 		b->max_nv = std::max(b->max_nv, m.nv);
 	}
@@ -306,7 +321,7 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 		b->slot_of[q] = s;
 		b->out_off[q] = off;
 		b->job_nv[q] = nv;
-		jobs[s] = BJob{ job_mesh[q], job_source[q], 0, 0, off, 0 };
+		jobs[s] = BJob{ job_mesh[q], b->tabs[job_mesh[q]].inv[job_source[q]], 0, 0, off, 0 };
 		off += nv;
 	}
 	b->total_out = off;

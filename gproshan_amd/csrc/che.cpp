@@ -178,18 +178,19 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	return GP_OK;
 }
 
-int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile)
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W)
 {
 	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.9;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
-	L.nchunks = (nr + 63) / 64;
+	if (W != 64 && W != 1) return fail(GP_ERR_INVALID, "layout: chunk width must be 64 or 1");
+	L.nchunks = (nr + W - 1) / W;
 	L.coff.assign((size_t) L.nchunks + 1, 0);
 	L.fcum.assign((size_t) nr + 1, 0);
 	L.max_slots = 0;
 	L.n_flagged = 0;
 	for (uint32_t c = 0; c < L.nchunks; c++) {
 		uint32_t S = 0;
-		for (uint32_t pos = 64 * c; pos < std::min(nr, 64 * c + 64); pos++) {
+		for (uint32_t pos = W * c; pos < std::min(nr, W * c + W); pos++) {
 			const uint32_t v = order ? order[pos] : pos;
 			if (v >= m.nv) return fail(GP_ERR_INVALID, "layout: vertex %u out of range", v);
 			if (v >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "layout: vertex ids must stay below 2^30");
@@ -198,7 +199,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		L.coff[c + 1] = L.coff[c] + S;
 		L.max_slots = std::max(L.max_slots, S);
 	}
-	const size_t lanes = (size_t) L.coff[L.nchunks] * 64;
+	const size_t lanes = (size_t) L.coff[L.nchunks] * W;
 	L.cid.assign(std::max<size_t>(lanes, 1), 0);
 	if (f64) {
 		L.cx64.assign(std::max<size_t>(lanes, 1), Geo64{0, 0, 0, 0});
@@ -212,15 +213,15 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 	#pragma omp parallel for schedule(dynamic, 64)
 	for (int64_t pp = 0; pp < (int64_t) nr; pp++) {
 		const uint32_t pos = (uint32_t) pp;
-		const uint32_t c = pos / 64, lane = pos % 64;
+		const uint32_t c = pos / W, lane = pos % W;
 		const uint32_t S = L.coff[c + 1] - L.coff[c];
-		const size_t base = (size_t) L.coff[c] * 64 + lane;
+		const size_t base = (size_t) L.coff[c] * W + lane;
 		const uint32_t v = order ? order[pos] : pos;
 		const uint32_t s0 = m.fan_off[v], n = m.fan_off[v + 1] - s0;
 		const double *pv = &m.gt[3 * (size_t) v];
 		uint32_t nflag = 0;
 		for (uint32_t k = 0; k < S; k++) {
-			const size_t at = base + 64 * (size_t) k;
+			const size_t at = base + W * (size_t) k;
 			if (k >= n) { L.cid[at] = v | SLOT_EMPTY; continue; }
 			const uint32_t a = m.fan[s0 + k];
 			const double *pa = &m.gt[3 * (size_t) a];
@@ -263,10 +264,10 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		L.lidx.assign(L.cid.size(), LOCAL_OUT);
 		#pragma omp parallel for schedule(static)
 		for (int64_t pp = 0; pp < (int64_t) nr; pp++) {
-			const uint32_t pos = (uint32_t) pp, c = pos / 64, lane = pos % 64;
+			const uint32_t pos = (uint32_t) pp, c = pos / W, lane = pos % W;
 			const uint32_t S = L.coff[c + 1] - L.coff[c];
 			for (uint32_t k = 0; k < S; k++) {
-				const size_t at = (size_t) (L.coff[c] + k) * 64 + lane;
+				const size_t at = (size_t) (L.coff[c] + k) * W + lane;
 				const uint32_t id = L.cid[at];
 				if (id & SLOT_EMPTY) continue;
 				const uint32_t u = id & ID30_MASK;
@@ -285,6 +286,41 @@ static inline uint32_t spread10(uint32_t x)
 	x = (x | (x << 4)) & 0x030C30C3;
 	x = (x | (x << 2)) & 0x09249249;
 	return x;
+}
+
+static inline uint64_t spread21(uint64_t x)
+{
+	x &= 0x1FFFFF;
+	x = (x | (x << 32)) & 0x1F00000000FFFFull;
+	x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+	x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+	x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+	x = (x | (x << 2)) & 0x1249249249249249ull;
+	return x;
+}
+
+void morton_order(const Che &m, std::vector<uint32_t> &order)
+{
+	double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+	for (uint32_t v = 0; v < m.nv; v++)
+		for (int k = 0; k < 3; k++) {
+			lo[k] = std::min(lo[k], m.gt[3 * (size_t) v + k]);
+			hi[k] = std::max(hi[k], m.gt[3 * (size_t) v + k]);
+		}
+	const double ext = std::max({ hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300 });
+	const double sc = 2097151.0 / ext;
+	std::vector<std::pair<uint64_t, uint32_t>> keys(m.nv);
+	#pragma omp parallel for schedule(static)
+	for (int64_t vv = 0; vv < (int64_t) m.nv; vv++) {
+		const uint32_t v = (uint32_t) vv;
+		const double *p = &m.gt[3 * (size_t) v];
+		uint64_t c = 0;
+		for (int k = 0; k < 3; k++) c |= spread21((uint64_t) std::min(2097151.0, (p[k] - lo[k]) * sc)) << k;
+		keys[v] = { c, v };
+	}
+	std::sort(keys.begin(), keys.end());
+	order.resize(m.nv);
+	for (uint32_t q = 0; q < m.nv; q++) order[q] = keys[q].second;
 }
 
 int band_order(const Che &m, const uint32_t *src, uint32_t ns, const uint32_t *levels, const double *dhat,

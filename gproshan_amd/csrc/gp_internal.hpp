@@ -92,7 +92,15 @@ struct Layout {
 	std::vector<uint16_t> lidx;            // tile > 0: slot neighbour's index in the tile, or LOCAL_OUT
 };
 // tile > 0 also fills lidx (the mesh must be numbered by positions: order == null).
-int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile = 0);
+// W = 64: chunks of 64 positions, slot k of lane l at (coff[c] + k) * 64 + l
+// (one wave per chunk, coalesced); W = 1: vertex-major, slot k of position p
+// at coff[p] + k (one thread walks a vertex's slots in a few cache lines).
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile = 0,
+                 uint32_t W = 64);
+
+// All vertices in Morton order of their positions, one scale for the three
+// axes (a flat axis takes few bits), so consecutive vertices are close.
+void morton_order(const Che &m, std::vector<uint32_t> &order);
 
 // Band order for the band schedules: sources first (band 0), then reached
 // vertices by band 1 + floor(dhat / width), inside a band by the Morton code

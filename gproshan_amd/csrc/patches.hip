@@ -94,7 +94,8 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 	__shared__ double fq0[FCACHE][P], fq1[FCACHE][P], fg[FCACHE][P], fdet[FCACHE][P], fS[FCACHE][P];
 	__shared__ float fta[FCACHE][P], ftb[FCACHE][P];
 	__shared__ uint32_t s_wake[WAKE_TAB];
-	__shared__ uint32_t s_cap, s_entry[2];
+	__shared__ uint8_t chg[2][P];        // value dropped in the iteration that wrote sd[b]
+	__shared__ uint32_t s_cap, s_entry[2], s_upd;
 	PatchState *st = a.st;
 	const uint32_t cur = k4, nxt = (k4 + 1u) & 3u, rst = (k4 + 2u) & 3u;
 	const uint32_t count = st->count[cur];
@@ -177,13 +178,29 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 		}
 		sd[0][tid] = v0;
 		if (tid < WAKE_TAB) s_wake[tid] = 0;
-		if (tid == 0) s_cap = 0;
+		if (tid == 0) { s_cap = 0; s_upd = 0; }
 		__syncthreads();
 		uint32_t b = 0, it = 0;
 		for (; it < a.local_iters; it++) {
 			const float curv = sd[b][tid];
 			float best = curv;
-			if (fast) {
+			// A vertex is recomputed only when an in-patch neighbour dropped in
+			// the last iteration (out-of-patch values are fixed during the
+			// visit): otherwise its inputs are those of its last evaluation,
+			// whose result it already holds, so the skip is exact.
+			bool need = it == 0 || !fast;
+			if (!need) {
+#pragma unroll
+				for (uint32_t k = 0; k < MAXS; k++) {
+					const uint32_t lk = pk[k] & 0xFFFFu;
+					if (k < S && lk != LOCAL_OUT && chg[b][lk]) need = true;
+				}
+			}
+			{
+				const unsigned long long bal = __ballot(need && live);
+				if (lane == 0 && bal) atomicAdd(&s_upd, (uint32_t) __popcll(bal));
+			}
+			if (need && fast) {
 				// FP32 triangles inline; the FP64-flagged ones after the slot
 				// loop, so the FP64 code is emitted once, not per unrolled slot.
 				float ta = (pk[0] & 0xFFFFu) != LOCAL_OUT ? sd[b][pk[0] & 0xFFFFu] : tout[0];
@@ -225,7 +242,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 					const float q = tri_flagged(a, at, tka, tkb);
 					if (q < best) best = q;
 				}
-			} else {
+			} else if (!fast) {
 				// Fans longer than MAXS: stream the slots.
 				uint32_t ida = a.cid[base];
 				uint32_t lia = a.lidx[base];
@@ -244,6 +261,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 				}
 			}
 			sd[b ^ 1u][tid] = best;
+			chg[b ^ 1u][tid] = best < curv ? 1 : 0;
 			const int moved = __syncthreads_or(best < curv && (curv - best) > a.tol * best);
 			b ^= 1u;
 			if (!moved) break;
@@ -273,7 +291,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 			a.defer[p] = 0;
 			if (s_cap) p_append(a, nxt, p);
 			atomicAdd(&st->visits, 1ull);
-			atomicAdd(&st->updates, (unsigned long long) P * (it + (it < a.local_iters ? 1u : 0u)));
+			atomicAdd(&st->updates, (unsigned long long) s_upd);
 		}
 	}
 }
@@ -358,43 +376,6 @@ static int pupload(V *&dst, const V *src, size_t n)
 	return GP_OK;
 }
 
-static inline uint64_t spread21(uint64_t x)
-{
-	x &= 0x1FFFFF;
-	x = (x | (x << 32)) & 0x1F00000000FFFFull;
-	x = (x | (x << 16)) & 0x1F0000FF0000FFull;
-	x = (x | (x << 8)) & 0x100F00F00F00F00Full;
-	x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
-	x = (x | (x << 2)) & 0x1249249249249249ull;
-	return x;
-}
-
-// All vertices in Morton order of their positions, one scale for the three
-// axes so that patches are compact in space (a flat axis takes few bits).
-static void morton_order(const Che &m, std::vector<uint32_t> &order)
-{
-	double lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
-	for (uint32_t v = 0; v < m.nv; v++)
-		for (int k = 0; k < 3; k++) {
-			lo[k] = std::min(lo[k], m.gt[3 * (size_t) v + k]);
-			hi[k] = std::max(hi[k], m.gt[3 * (size_t) v + k]);
-		}
-	const double ext = std::max({ hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300 });
-	const double sc = 2097151.0 / ext;
-	std::vector<std::pair<uint64_t, uint32_t>> keys(m.nv);
-	#pragma omp parallel for schedule(static)
-	for (int64_t vv = 0; vv < (int64_t) m.nv; vv++) {
-		const uint32_t v = (uint32_t) vv;
-		const double *p = &m.gt[3 * (size_t) v];
-		uint64_t c = 0;
-		for (int k = 0; k < 3; k++) c |= spread21((uint64_t) std::min(2097151.0, (p[k] - lo[k]) * sc)) << k;
-		keys[v] = { c, v };
-	}
-	std::sort(keys.begin(), keys.end());
-	order.resize(m.nv);
-	for (uint32_t q = 0; q < m.nv; q++) order[q] = keys[q].second;
-}
-
 template <uint32_t P>
 static int occupancy(int &occ)
 {
@@ -429,6 +410,7 @@ int patch_create(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat
 	pp->ren = std::make_unique<Che>();
 	int rc = coalesce(*mesh, order.data(), mesh->nv, *pp->ren, pp->inv);
 	if (rc) return rc;
+	pp->ren->mean_edge = mesh->mean_edge;   // the same h as the mesh itself
 	Che &m = *pp->ren;
 	pp->n = m.nv;
 	rc = upload_mesh(m);

@@ -12,12 +12,14 @@ namespace gp {
 
 constexpr uint32_t MAXS = 10;  // register-resident slots; longer fans take the streaming loop
 
-// FP64 update of triangle (k, k+1) of the lane at slot base (flagged triangles).
-template <typename A>
+// FP64 update of triangle (k, k+1) of the lane at slot base (flagged
+// triangles).  ST is the distance between a lane's consecutive slots: 64 in
+// the chunked layout, 1 in the vertex-major one.
+template <uint32_t ST = 64, typename A>
 __device__ __forceinline__ float tri_flagged(const A &a, size_t at, float ta, float tb)
 {
 	const double4 gd = a.cgeo64[at];
-	const double q1 = a.cgeo64[at + 64].x;
+	const double q1 = a.cgeo64[at + ST].x;
 	return (float) update_stable<double>(gd.x, q1, gd.y, gd.z, gd.w, (double) ta, (double) tb, (double) a.h,
 	                                     (double) a.inv_h);
 }
@@ -33,7 +35,7 @@ __device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, 
 // flagged triangles are skipped in the slot loop and done afterwards in a
 // loop that runs as many times as the most-flagged lane of the wave needs,
 // so the FP64 code runs ~1-2 times per wave instead of once per slot.
-template <bool F64, typename A>
+template <bool F64, uint32_t ST = 64, typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
 relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
            typename std::conditional<F64, double, float>::type best)
@@ -50,7 +52,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 #pragma unroll
 		for (uint32_t k = 0; k < MAXS; k++) {
 			id[k] = SLOT_EMPTY;
-			if (k < S) { id[k] = a.cid[base + 64 * k]; g[k] = geo[base + 64 * k]; }
+			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
 		}
 #pragma unroll
 		for (uint32_t k = 0; k < MAXS; k++)
@@ -77,15 +79,15 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 		G ga = geo[base];
 		T ta = od[ida & ID30_MASK];
 		for (uint32_t k = 1; k < S; k++) {
-			const uint32_t idb = a.cid[base + 64 * k];
+			const uint32_t idb = a.cid[base + ST * k];
 			if (idb & SLOT_EMPTY) break;
-			const G gb = geo[base + 64 * k];
+			const G gb = geo[base + ST * k];
 			const T tb = od[idb & ID30_MASK];
 			if constexpr (F64) {
 				const T p = tri_f64(ga, gb, ta, tb);
 				if (p < best) best = p;
 			} else {
-				const T p = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
+				const T p = (ida & F64_FLAG) ? tri_flagged<ST>(a, base + ST * (k - 1), ta, tb)
 				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
 				if (p < best) best = p;
 			}
@@ -96,10 +98,10 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 		while (fmask) {
 			const uint32_t k = __builtin_ctz(fmask);
 			fmask &= fmask - 1u;
-			const size_t at = base + 64 * (size_t) k;
+			const size_t at = base + ST * (size_t) k;
 			const float ta = od[a.cid[at] & ID30_MASK];
-			const float tb = od[a.cid[at + 64] & ID30_MASK];
-			const float p = tri_flagged(a, at, ta, tb);
+			const float tb = od[a.cid[at + ST] & ID30_MASK];
+			const float p = tri_flagged<ST>(a, at, ta, tb);
 			if (p < best) best = p;
 		}
 	}

@@ -13,6 +13,8 @@ CASES = [
     ("ico5", lambda: G.icosphere(5, 0.01, 0), [0]),
     ("hf128_4src", lambda: G.height_field(128, 1), [777, 5000, 9000, 16000]),
     ("hf256", lambda: G.height_field(256, 2), [32896]),
+    ("hf256_16src", lambda: G.height_field(256, 7),
+     G.draw_sources(np.random.default_rng(7), 256 * 256, 16).tolist()),
 ]
 
 
