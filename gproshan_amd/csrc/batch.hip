@@ -107,7 +107,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 				const uint32_t v = sorted[pos];
 				const uint32_t cb = M.coff[v];
 				const float old = od[v];
-				const float best = relax_lane<false, 1>(M, (size_t) cb, M.coff[v + 1] - cb, od, old);
+				const float best = relax_lane<false, 1, false>(M, (size_t) cb, M.coff[v + 1] - cb, od, old);
 				nd[v] = best;
 				if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
 			}

@@ -35,7 +35,7 @@ __device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, 
 // flagged triangles are skipped in the slot loop and done afterwards in a
 // loop that runs as many times as the most-flagged lane of the wave needs,
 // so the FP64 code runs ~1-2 times per wave instead of once per slot.
-template <bool F64, uint32_t ST = 64, typename A>
+template <bool F64, uint32_t ST = 64, bool PF = true, typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
 relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
            typename std::conditional<F64, double, float>::type best)
@@ -54,9 +54,27 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 			id[k] = SLOT_EMPTY;
 			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
 		}
+		// FP32 (PF): the FP64 geometry of the first flagged triangle is
+		// requested with the neighbour distances (both depend only on the slot
+		// ids), so it costs no extra memory round trip; later flagged
+		// triangles of the lane (rare) are deferred to the loop below.  PF =
+		// false (register-tight callers) defers them all.
+		uint32_t k0 = MAXS;
+		double4 gd0 = { 0, 0, 0, 0 };
+		double q10 = 0;
+		if constexpr (!F64 && PF) {
+#pragma unroll
+			for (uint32_t k = MAXS; k-- > 0;)
+				if (k + 1 < S && (id[k] & F64_FLAG)) k0 = k;
+			if (k0 != MAXS) {
+				gd0 = a.cgeo64[base + ST * (size_t) k0];
+				q10 = a.cgeo64[base + ST * (size_t) (k0 + 1)].x;
+			}
+		}
 #pragma unroll
 		for (uint32_t k = 0; k < MAXS; k++)
 			t[k] = (k < S) ? od[id[k] & ID30_MASK] : (T) INFINITY;
+		T fa = (T) INFINITY, fb = (T) INFINITY;
 #pragma unroll
 		for (uint32_t k = 0; k + 1 < MAXS; k++) {
 			if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
@@ -65,13 +83,21 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 					if (p < best) best = p;
 				} else {
 					if (id[k] & F64_FLAG) {
-						fmask |= 1u << k;
+						if (k == k0) { fa = t[k]; fb = t[k + 1]; }
+						else fmask |= 1u << k;
 					} else {
 						const T p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1],
 						                                 a.h, a.inv_h);
 						if (p < best) best = p;
 					}
 				}
+			}
+		}
+		if constexpr (!F64) {
+			if (k0 != MAXS) {
+				const T p = (T) update_stable<double>(gd0.x, q10, gd0.y, gd0.z, gd0.w, (double) fa, (double) fb,
+				                                      (double) a.h, (double) a.inv_h);
+				if (p < best) best = p;
 			}
 		}
 	} else {
