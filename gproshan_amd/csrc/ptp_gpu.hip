@@ -278,9 +278,9 @@ struct gp_plan {
 	double4 *d_cgeo64 = nullptr, *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
 	double *d_out = nullptr;
-	gp::PState *d_state = nullptr, *h_state = nullptr;
+	gp::PState *d_state = nullptr, *h_state = nullptr;   // h_state: 2 pinned snapshots
 	hipStream_t stream = nullptr;
-	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = { nullptr, nullptr };
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	bool has_run = false;
 
@@ -296,6 +296,8 @@ struct gp_plan {
 		if (h_state) (void) hipHostFree(h_state);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
+		for (hipEvent_t e : ev)
+			if (e) (void) hipEventDestroy(e);
 		if (stream) (void) hipStreamDestroy(stream);
 	}
 };
@@ -378,6 +380,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
 	GP_HIP(hipEventCreate(&p->e0));
 	GP_HIP(hipEventCreate(&p->e1));
+	GP_HIP(hipEventCreate(&p->ev[0]));
+	GP_HIP(hipEventCreate(&p->ev[1]));
 	std::vector<uint32_t> lim(limits, limits + nl);
 	if ((rc = upload(p->d_limits, lim)) || (rc = upload(p->d_src, srcm)) || (rc = upload(p->d_coff, L.coff)) ||
 	    (rc = upload(p->d_cid, L.cid)))
@@ -404,7 +408,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
 	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + 1)) != hipSuccess ||
-	    hipHostMalloc(&p->h_state, sizeof(PState)) != hipSuccess)
+	    hipHostMalloc(&p->h_state, 2 * sizeof(PState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	*out = p.release();
 	return GP_OK;
@@ -468,30 +472,56 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	}
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipEventRecord(p->e0, p->stream));
-	// Replays until the device reports done.  One sweep more than the sweep
-	// count runs (it records the last sweep's outcome).  j grows by one ring
-	// per sweep until |limits|-1, so at least nl-1-j sweeps remain: big graphs
-	// while that bound allows, small ones near the end.
+	// Replays until the device reports done, two replays in flight: after
+	// each replay the state is copied to its pinned snapshot and an event is
+	// recorded; the host waits for the older replay's event while the newer
+	// one runs, so the GPU does not idle on the poll.  The device time ends
+	// at the event of the replay whose state says done (a replay queued
+	// behind it only meets done slots and returns at once).  One sweep more
+	// than the sweep count runs (it records the last sweep's outcome).  j
+	// grows by one ring per sweep until |limits|-1, so at least nl-1-j sweeps
+	// remain: big graphs while that bound allows, small ones near the end.
 	uint32_t launched = 0, j_seen = 2;
 	const uint32_t cap = p->max_iter;
-	while (!done0 && launched <= cap) {
-		const bool big = p->nl - 1 > j_seen + BIG;
+	auto replay = [&](uint32_t s) -> int {
+		const bool big = p->nl - 1 > j_seen + BIG + (launched ? BIG : 0);
 		const uint32_t cnt = big ? BIG : SMALL;
 		if (graph) GP_HIP(hipGraphLaunch(big ? p->g_big : p->g_small, p->stream));
 		else enqueue_sweeps(p, a, launched, cnt);
 		launched += cnt;
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
-		GP_HIP(hipStreamSynchronize(p->stream));
-		if (p->h_state->done) break;
-		j_seen = p->h_state->slot[(launched - 1) & 3u].j;
+		GP_HIP(hipMemcpyAsync(p->h_state + s, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipEventRecord(p->ev[s], p->stream));
+		return GP_OK;
+	};
+	uint32_t cur = 0;
+	bool done = done0 != 0;
+	if (!done) {
+		int rc = replay(0);
+		if (rc) return rc;
+		for (;;) {
+			// Queue the next replay (if the cap allows), then wait for the
+			// older one; the replay queued here is the one checked next.
+			const bool queued = launched <= cap;
+			if (queued && (rc = replay(cur ^ 1u))) return rc;
+			GP_HIP(hipEventSynchronize(p->ev[cur]));
+			if (p->h_state[cur].done) { done = true; break; }
+			if (!queued) break;
+			// Graphs hold a multiple of 4 sweeps: slot 3 is a replay's last sweep.
+			j_seen = p->h_state[cur].slot[3].j;
+			cur ^= 1u;
+		}
 	}
-	if (!done0 && !p->h_state->done) return fail(GP_ERR_KERNEL, "ptp: schedule did not finish in %u sweeps", launched);
-	GP_HIP(hipEventRecord(p->e1, p->stream));
-	GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
 	GP_HIP(hipStreamSynchronize(p->stream));
+	if (!done) return fail(GP_ERR_KERNEL, "ptp: schedule did not finish in %u sweeps", launched);
+	if (done0) {
+		GP_HIP(hipEventRecord(p->e1, p->stream));
+		GP_HIP(hipMemcpyAsync(p->h_state + cur, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipStreamSynchronize(p->stream));
+	}
+	if (cur != 0) std::memcpy(p->h_state, p->h_state + cur, sizeof(PState));
 	float ms = 0;
-	GP_HIP(hipEventElapsedTime(&ms, p->e0, p->e1));
+	GP_HIP(hipEventElapsedTime(&ms, p->e0, done0 ? p->e1 : p->ev[cur]));
 	p->has_run = true;
 	if (stats) {
 		stats->sweeps = p->h_state->sweeps;
