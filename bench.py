@@ -39,6 +39,12 @@ B_RELAX_F32 = 72   # bytes per vertex relaxation: 2*4 dist + 3*4 xyz + 4 EVT + 8
 B_RELAX_F64 = 92
 PEAK_HBM_GBS = 8000.0
 ERR_BAR = 1e-4
+# HBM bytes per dispatch of the headline kernel on the default workload, from
+# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, 1,176
+# dispatches of one band_learned run) scaled by what the same counters read for
+# a 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
+# profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources).
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): (43687.9 * 2.0 + 1516.6 * 1.0) * 1024}
 
 
 def log(*a):
@@ -197,7 +203,11 @@ def main():
     # patch vertices per visit -- the local iterations run in LDS).
     loads = st["cond_checks"] if patch_row else st["relaxations"]
     bytes_alg = loads * B_RELAX_F32 + (0 if patch_row else st["cond_checks"] * 8)
-    achieved = bytes_alg / (h["kernel_ms"] * 1e-3) / 1e9
+    launches = max(1, st["sweeps"])
+    avg_launch_s = h["kernel_ms"] * 1e-3 / launches          # hipEvent time / launches
+    achieved = bytes_alg / launches / avg_launch_s / 1e9
+    default_opts = args.band_width == 1.0 and args.seed == 0
+    traffic = MEASURED_TRAFFIC.get((headline, args.n, args.sources)) if default_opts else None
     out = {
         "metric": "ms per 1M-vertex distance field, learned-band vs plain PTP at <=1e-4 err; GB/s",
         "value": round(h["kernel_ms"], 4),
@@ -216,7 +226,10 @@ def main():
                                f"{args.seed}; value = {headline}", "sources": src.tolist(),
                    "band_width": args.band_width, "local_iters": args.local_iters, "patch": args.patch},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 5),
+                     "traffic": round(traffic) if traffic else None,
+                     "algorithmic_bytes_per_launch": round(bytes_alg / launches),
+                     "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false>"},
         "speedup_vs_plain_at_bar": (round(rows["plain_uncapped_f32"]["kernel_ms"] / h["kernel_ms"], 3)
                                     if "plain_uncapped_f32" in rows else None),
@@ -243,10 +256,11 @@ def stress(args, L, P, G):
     src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), 4)
     mesh = P.che(xyz, faces)
     log(f"[stress] V={mesh.n_vertices()} sources={src.tolist()} ({time.time() - t:.1f}s)")
-    rows = {}
+    rows, traces = {}, {}
     for name, prec in (("f32_fallback", L.PREC_F32), ("f64", L.PREC_F64)):
         plan = P.Plan(mesh, src, precision=prec)
         rows[name] = time_plan(plan, args.steps, args.warmup)
+        traces[name] = plan.trace()
         plan.close()
         log(f"[stress] {name}: {rows[name]['kernel_ms']:.2f} ms, sweeps {rows[name]['st']['sweeps']}")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -255,7 +269,14 @@ def stress(args, L, P, G):
     t0 = time.perf_counter()
     ref, nsw = om.ptp(src, O.GPU_COAL, n_threads=os.cpu_count() or 1)
     cpu_s = time.perf_counter() - t0
+    # Where the FP32 run's schedule first departs from the FP64 run's (a ring
+    # test flipped by rounding moves values by about PTP_TOL).
+    t32, t64 = traces["f32_fallback"][:, 1:7], traces["f64"][:, 1:7]
+    n = min(len(t32), len(t64))
+    diff = np.flatnonzero(np.any(t32[:n] != t64[:n], axis=1))
+    first_diff = int(diff[0]) + 1 if len(diff) else (None if len(t32) == len(t64) else n + 1)
     out = {"metric": "C5 precision stress: max relative error vs the FP64 oracle", "unit": "relative error",
+           "f32_schedule_matches_f64": first_diff is None, "f32_first_differing_sweep": first_diff,
            "config": {"workload": "anisotropic grid 2048x2048 (4194304 vertices), stretch 1:100, jitter 0.4, "
                                   f"4 sources, seed {args.seed}", "sources": src.tolist()},
            "oracle": {"sweeps": nsw, "seconds": round(cpu_s, 2), "cores": os.cpu_count()}, "rows": {}}
