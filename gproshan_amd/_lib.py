@@ -35,6 +35,7 @@ EXPORTS = [
     "gp_mlp_create", "gp_mlp_destroy", "gp_mlp_params", "gp_mlp_features", "gp_mlp_forward", "gp_mlp_train",
     "gp_mlp_predict", "gp_plan_create_learned",
     "gp_batch_create", "gp_batch_run", "gp_batch_result", "gp_batch_destroy",
+    "gp_farthest_point_sampling_ptp_gpu",
 ]
 MLP_NPARAM = 6337
 MLP_NFEAT = 10
@@ -123,6 +124,8 @@ def lib() -> C.CDLL:
     L.gp_batch_result.argtypes = [P, u32, P, u32, C.POINTER(u32)]
     L.gp_batch_destroy.argtypes = [P]
     L.gp_batch_destroy.restype = None
+    L.gp_farthest_point_sampling_ptp_gpu.argtypes = [P, P, u32, u32, C.c_uint64, C.c_double, C.c_int,
+                                                     C.POINTER(u32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 

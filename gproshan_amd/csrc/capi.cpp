@@ -438,4 +438,16 @@ int gp_batch_result(const gp_batch *batch, uint32_t job, double *dist, uint32_t 
 	GP_GUARD(return gp::batch_result(batch, job, dist, cap, sweeps);)
 }
 
+int gp_farthest_point_sampling_ptp_gpu(gp_che *mesh, uint32_t *samples, uint32_t n_init, uint32_t cap,
+                                       uint64_t n, double radio, int precision, uint32_t *n_out,
+                                       double *radio_out, double *seconds)
+{
+	GP_GUARD(
+		if (!mesh || !samples) return gp::fail(GP_ERR_INVALID, "null argument");
+		if (precision != GP_PREC_F32 && precision != GP_PREC_F64) return gp::fail(GP_ERR_INVALID, "bad precision");
+		return gp::fps_gpu(mesh->m, samples, n_init, cap, n, radio, precision == GP_PREC_F64, n_out, radio_out,
+		                   seconds);
+	)
+}
+
 }  // extern "C"

@@ -1,0 +1,367 @@
+// Farthest-point sampling over GPU PTP (farthest_point_sampling_ptp_gpu,
+// src/cuda/geodesics_ptp.cu:87-172), device-resident.
+//
+// Each round runs the reference's multi-source GPU driver on the current
+// samples -- rings from the samples (che::compute_toplesets order), the window
+// schedule through sorted[] (src/cuda/geodesics_ptp.cu:174-232, last-written
+// buffer) -- takes the first vertex of largest distance (cuBLAS I?amax
+// semantics: smallest index among the maxima) and appends it.  Here the rings
+// are built on the GPU by the same one-workgroup BFS as the toplesets kernel,
+// the samples, rings, schedule state and distances stay in HBM, and the host
+// only polls the per-round done flag: no per-round upload, download or host
+// BFS.  The mesh keeps its numbering; its slots are stored vertex-major (a
+// window vertex reads its fan from consecutive addresses).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <vector>
+
+#include "device.hpp"
+#include "rings_dev.hpp"
+#include "sweep_dev.hpp"
+
+namespace gp {
+
+struct FSlot {
+	uint32_t i, j, d, iter, done, fresh, pad0, pad1;
+};
+struct FState {
+	FSlot slot[4];
+	uint32_t bad[4];
+	uint32_t done, sweeps, final_d, nl;
+	unsigned long long best;   // ordered bits of the round's largest distance
+	uint32_t arg, stop, pad2, pad3;
+};
+
+struct FArgs {
+	const uint32_t *coff, *cid;
+	const float4 *cgeo;
+	const double4 *cgeo64;
+	const double4 *cx64;
+	float h, inv_h;
+	const uint32_t *sorted, *limits, *nl;   // rings of the current samples
+	void *dist0, *dist1;
+	FState *st;
+	double tol;
+};
+
+// Sweep k of a round: the reference loop tail on the previous state, then the
+// window relaxation (as the single-field sweep kernel, with |limits| read from
+// the device because the rings are rebuilt every round).
+template <bool F64>
+__global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
+{
+	using T = typename std::conditional<F64, double, float>::type;
+	FState *st = a.st;
+	const uint32_t nl = *a.nl, max_iter = nl << 1;
+	const uint32_t kp = (k4 + 3u) & 3u;
+	const FSlot prev = st->slot[kp];
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if (prev.done) {
+		if (lead) st->slot[k4] = prev;
+		return;
+	}
+	FSlot cur = prev;
+	cur.fresh = 0;
+	if (!prev.fresh) {
+		const uint32_t nbad = st->bad[kp];
+		const uint32_t pi = prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
+		cur.i = pi + (nbad == 0 ? 1u : 0u);
+		cur.j = prev.j + (prev.j < nl - 1 ? 1u : 0u);
+		cur.d = prev.d ^ 1u;
+		cur.iter = prev.iter + 1;
+		cur.done = !(cur.i < cur.j) || !(cur.iter < max_iter);
+		if (lead) st->sweeps = cur.iter;
+	}
+	if (lead) {
+		st->slot[k4] = cur;
+		st->bad[(k4 + 1u) & 3u] = 0;
+		if (cur.done) { st->final_d = cur.d; st->done = 1; }
+	}
+	if (cur.done) return;
+	const uint32_t i = cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
+	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
+	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
+	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
+	// XCD-aware split of the window: workgroups b % 8 == x take the x-th
+	// contiguous eighth of the positions.
+	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
+	const uint32_t x = blockIdx.x & 7u;
+	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
+	const uint32_t len = end - start, part = (len + ngrp - 1) / ngrp;
+	const uint32_t lo = start + x * part, hi = min(end, lo + part);
+	uint32_t bad = 0;
+	for (uint32_t pos = lo + (blockIdx.x >> 3) * blockDim.x + threadIdx.x; pos < hi; pos += nbx * blockDim.x) {
+		const uint32_t v = a.sorted[pos];
+		const uint32_t cb = a.coff[v];
+		const T old = od[v];
+		const T best = relax_lane<F64, 1, false>(a, (size_t) cb, a.coff[v + 1] - cb, od, old);
+		nd[v] = best;
+		if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+	}
+	for (uint32_t o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
+}
+
+// Round set-up: ring-builder scratch and both distance buffers reset.
+template <typename T>
+__global__ void k_fps_prep(T *d0, T *d1, uint32_t *level, unsigned long long *key, uint32_t *sorted, uint32_t n)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+		d0[v] = (T) INFINITY; d1[v] = (T) INFINITY;
+		level[v] = GP_NIL; key[v] = ~0ull; sorted[v] = GP_NIL;
+	}
+}
+
+template <typename T>
+__global__ void k_fps_sources(T *d0, T *d1, const BfsJob *job)
+{
+	for (uint32_t q = threadIdx.x; q < job->ns; q += blockDim.x) { d0[job->src[q]] = (T) 0; d1[job->src[q]] = (T) 0; }
+}
+
+__global__ __launch_bounds__(BFS_THREADS) void k_fps_bfs(const uint32_t *fan_off, const uint32_t *fan,
+                                                         const uint8_t *closed, const BfsJob *job)
+{
+	bfs_block(fan_off, fan, closed, *job);
+}
+
+__global__ void k_fps_state_init(FState *st, const uint32_t *nl)
+{
+	uint32_t *w = (uint32_t *) st;
+	for (uint32_t q = 0; q < sizeof(FState) / 4; q++) w[q] = 0;
+	const uint32_t n = *nl;
+	st->nl = n;
+	st->slot[3].i = 1; st->slot[3].j = 2; st->slot[3].fresh = 1;
+	st->slot[3].done = n < 3 ? 1u : 0u;     // only the sources: no sweep
+	st->done = st->slot[3].done;
+}
+
+__device__ __forceinline__ unsigned long long ordered_bits(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned long long ordered_bits(double x) { return (unsigned long long) __double_as_longlong(x); }
+
+// Largest |distance| of the buffer the last sweep wrote (distances are >= 0 or
+// +inf, so their IEEE bit patterns order like the values).
+template <typename T>
+__global__ void k_fps_max(const void *d0, const void *d1, uint32_t n, FState *st)
+{
+	const T *d = (const T *) (st->final_d ? d1 : d0);
+	unsigned long long m = 0;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+		const unsigned long long b = ordered_bits(fabs(d[v]));
+		m = b > m ? b : m;
+	}
+	for (uint32_t o = 32; o > 0; o >>= 1) {
+		const unsigned long long y = __shfl_xor(m, o);
+		m = y > m ? y : m;
+	}
+	if ((threadIdx.x & 63u) == 0) atomicMax(&st->best, m);
+}
+
+// First vertex holding that maximum (the smallest index, as I?amax returns).
+template <typename T>
+__global__ void k_fps_arg(const void *d0, const void *d1, uint32_t n, FState *st)
+{
+	const T *d = (const T *) (st->final_d ? d1 : d0);
+	const unsigned long long best = st->best;
+	uint32_t a = GP_NIL;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+		if (ordered_bits(fabs(d[v])) == best) { a = v; break; }
+	for (uint32_t o = 32; o > 0; o >>= 1) a = min(a, (uint32_t) __shfl_xor(a, o));
+	if ((threadIdx.x & 63u) == 0 && a != GP_NIL) atomicMin(&st->arg, a);
+}
+
+__global__ void k_fps_arg_init(FState *st) { st->arg = GP_NIL; st->best = 0; }
+
+// Appends the new sample.  A maximum of 0 means every vertex is already a
+// sample (the reference would append a duplicate and overrun its rings): stop.
+template <typename T>
+__global__ void k_fps_append(BfsJob *job, uint32_t *samples, FState *st)
+{
+	const unsigned long long zero = ordered_bits((T) 0);
+	if (st->best == zero || st->arg == GP_NIL) { st->stop = 1; return; }
+	samples[job->ns] = st->arg;
+	job->ns = job->ns + 1;
+}
+
+// ---------------------------------------------------------------------------
+
+struct FpsRun {
+	uint32_t *coff = nullptr, *cid = nullptr, *samples = nullptr, *level = nullptr, *sorted = nullptr,
+	         *limits = nullptr, *nl = nullptr;
+	unsigned long long *key = nullptr;
+	float4 *cgeo = nullptr;
+	double4 *cgeo64 = nullptr, *cx64 = nullptr;
+	void *d0 = nullptr, *d1 = nullptr;
+	FState *st = nullptr, *h_st = nullptr;
+	BfsJob *job = nullptr;
+	hipStream_t stream = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipGraphExec_t g_big = nullptr, g_small = nullptr;
+	~FpsRun()
+	{
+		for (void *p : { (void *) coff, (void *) cid, (void *) samples, (void *) level, (void *) sorted, (void *) limits,
+		                 (void *) nl, (void *) key, (void *) cgeo, (void *) cgeo64, (void *) cx64, d0, d1, (void *) st,
+		                 (void *) job })
+			(void) hipFree(p);
+		if (h_st) (void) hipHostFree(h_st);
+		if (g_big) (void) hipGraphExecDestroy(g_big);
+		if (g_small) (void) hipGraphExecDestroy(g_small);
+		if (e0) (void) hipEventDestroy(e0);
+		if (e1) (void) hipEventDestroy(e1);
+		if (stream) (void) hipStreamDestroy(stream);
+	}
+};
+
+template <typename V>
+static int fup(V *&dst, const V *src, size_t n)
+{
+	if (hipMalloc(&dst, sizeof(V) * std::max<size_t>(n, 1)) != hipSuccess) return fail(GP_ERR_NOMEM, "fps: allocation failed");
+	if (n) GP_HIP(hipMemcpy(dst, src, sizeof(V) * n, hipMemcpyHostToDevice));
+	return GP_OK;
+}
+
+template <bool F64>
+static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio,
+                    uint32_t *n_out, double *radio_out, double *seconds)
+{
+	using T = typename std::conditional<F64, double, float>::type;
+	const uint32_t nv = m.nv;
+	for (uint32_t v = 0; v < nv; v++)
+		if (m.fan_off[v + 1] - m.fan_off[v] > MAX_LINK) return fail(GP_ERR_CAPACITY, "fps: vertex %u: link over %u", v, MAX_LINK);
+	// The reference's sample budget (src/cuda/geodesics_ptp.cu:123-126).
+	if (n >= nv) n = nv >> 1;
+	const uint64_t rounds = n > n_init ? n - n_init : 0;
+	if (n_init + rounds > cap) return fail(GP_ERR_CAPACITY, "fps: samples needs room for %llu entries",
+	                                       (unsigned long long) (n_init + rounds));
+	int rc = upload_mesh(m);
+	if (rc) return rc;
+	FpsRun r;
+	Layout L;
+	if ((rc = build_layout(m, nullptr, nv, F64, L, 0, 1))) return rc;
+	auto as_d4 = [](const std::vector<Geo64> &v) {
+		std::vector<double4> o(v.size());
+		std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
+		return o;
+	};
+	if ((rc = fup(r.coff, L.coff.data(), L.coff.size())) || (rc = fup(r.cid, L.cid.data(), L.cid.size())))
+		return rc;
+	if (F64) {
+		const auto x = as_d4(L.cx64);
+		if ((rc = fup(r.cx64, x.data(), x.size()))) return rc;
+	} else {
+		const auto g64 = as_d4(L.cgeo64);
+		if ((rc = fup(r.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
+		    (rc = fup(r.cgeo64, g64.data(), g64.size())))
+			return rc;
+	}
+	if ((rc = fup(r.samples, samples, cap))) return rc;
+	if (hipMalloc(&r.level, 4ull * nv) != hipSuccess || hipMalloc(&r.sorted, 4ull * nv) != hipSuccess ||
+	    hipMalloc(&r.limits, 4ull * (nv + 2)) != hipSuccess || hipMalloc(&r.nl, 4) != hipSuccess ||
+	    hipMalloc(&r.key, 8ull * nv) != hipSuccess || hipMalloc(&r.d0, sizeof(T) * nv) != hipSuccess ||
+	    hipMalloc(&r.d1, sizeof(T) * nv) != hipSuccess || hipMalloc(&r.st, sizeof(FState)) != hipSuccess ||
+	    hipMalloc(&r.job, sizeof(BfsJob)) != hipSuccess || hipHostMalloc(&r.h_st, sizeof(FState)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "fps: device allocation failed");
+	const BfsJob job{ r.samples, n_init, r.level, r.key, r.sorted, r.limits, r.nl };
+	GP_HIP(hipMemcpy(r.job, &job, sizeof(job), hipMemcpyHostToDevice));
+	GP_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+	GP_HIP(hipEventCreate(&r.e0));
+	GP_HIP(hipEventCreate(&r.e1));
+	FArgs a{};
+	a.coff = r.coff; a.cid = r.cid; a.cgeo = r.cgeo; a.cgeo64 = r.cgeo64; a.cx64 = r.cx64;
+	a.h = m.dev->h; a.inv_h = m.dev->inv_h;
+	a.sorted = r.sorted; a.limits = r.limits; a.nl = r.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
+	a.tol = GP_PTP_TOL;
+	int dev = 0, ncu = 0, occ = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fps_sweep<F64>, 256, 0));
+	const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>((uint32_t) ncu * (uint32_t) std::max(1, occ),
+	                                                               (nv + 255) / 256));
+	constexpr uint32_t BIG = 64, SMALL = 8;
+	auto capture = [&](uint32_t count, hipGraphExec_t *out) -> int {
+		hipGraph_t g = nullptr;
+		GP_HIP(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
+		for (uint32_t q = 0; q < count; q++) k_fps_sweep<F64><<<grid, 256, 0, r.stream>>>(a, q & 3u);
+		GP_HIP(hipStreamEndCapture(r.stream, &g));
+		const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+		(void) hipGraphDestroy(g);
+		if (e != hipSuccess) return fail(GP_ERR_HIP, "fps: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+		return GP_OK;
+	};
+	if ((rc = capture(BIG, &r.g_big)) || (rc = capture(SMALL, &r.g_small))) return rc;
+	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(2048, (nv + 255) / 256));
+	const auto t0 = std::chrono::steady_clock::now();
+	GP_HIP(hipEventRecord(r.e0, r.stream));
+	double max_dist = INFINITY;
+	uint32_t count = n_init;
+	for (uint64_t left = rounds; left-- > 0 && max_dist > radio;) {
+		k_fps_prep<T><<<ig, 256, 0, r.stream>>>((T *) r.d0, (T *) r.d1, r.level, r.key, r.sorted, nv);
+		k_fps_sources<T><<<1, 256, 0, r.stream>>>((T *) r.d0, (T *) r.d1, r.job);
+		k_fps_bfs<<<1, BFS_THREADS, 0, r.stream>>>(m.dev->fan_off, m.dev->fan, m.dev->closed, r.job);
+		k_fps_state_init<<<1, 1, 0, r.stream>>>(r.st, r.nl);
+		GP_HIP(hipGetLastError());
+		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
+		GP_HIP(hipStreamSynchronize(r.stream));
+		const uint32_t nl = r.h_st->nl;
+		if (nl < 2 || nl > nv + 2) return fail(GP_ERR_KERNEL, "fps: bad ring count %u", nl);
+		// Sweeps: graph replays until the device reports done (at most
+		// 2*|limits| sweeps plus the replay that records the last outcome).
+		uint32_t launched = 0, j_seen = 2;
+		while (!r.h_st->done) {
+			if (launched > 2 * nl + BIG) return fail(GP_ERR_KERNEL, "fps: schedule did not finish");
+			const bool big = nl - 1 > j_seen + BIG;
+			GP_HIP(hipGraphLaunch(big ? r.g_big : r.g_small, r.stream));
+			launched += big ? BIG : SMALL;
+			GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
+			GP_HIP(hipStreamSynchronize(r.stream));
+			j_seen = r.h_st->slot[3].j;
+		}
+		k_fps_arg_init<<<1, 1, 0, r.stream>>>(r.st);
+		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
+		k_fps_arg<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
+		k_fps_append<T><<<1, 1, 0, r.stream>>>(r.job, r.samples, r.st);
+		GP_HIP(hipGetLastError());
+		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
+		GP_HIP(hipStreamSynchronize(r.stream));
+		if (r.h_st->stop) break;
+		count++;
+		// max_dist is read when a radius is given or in the last round
+		// (src/cuda/geodesics_ptp.cu:145-146).
+		if (radio > 0 || left == 0) {
+			const unsigned long long b = r.h_st->best;
+			if (F64) { double x; std::memcpy(&x, &b, 8); max_dist = x; }
+			else { const uint32_t u = (uint32_t) b; float x; std::memcpy(&x, &u, 4); max_dist = x; }
+		}
+	}
+	GP_HIP(hipEventRecord(r.e1, r.stream));
+	GP_HIP(hipStreamSynchronize(r.stream));
+	float ms = 0;
+	GP_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+	GP_HIP(hipMemcpy(samples, r.samples, 4ull * count, hipMemcpyDeviceToHost));
+	if (n_out) *n_out = count;
+	if (radio_out) *radio_out = max_dist;
+	if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	(void) ms;
+	return GP_OK;
+}
+
+int fps_gpu(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio, bool f64,
+            uint32_t *n_out, double *radio_out, double *seconds)
+{
+	if (!samples || !n_init) return fail(GP_ERR_INVALID, "fps: at least one initial sample is needed");
+	std::vector<uint8_t> seen(m.nv, 0);
+	for (uint32_t q = 0; q < n_init; q++) {
+		if (samples[q] >= m.nv) return fail(GP_ERR_INVALID, "fps: sample %u out of range", samples[q]);
+		if (seen[samples[q]]) return fail(GP_ERR_DUPLICATE_SOURCE, "fps: sample %u listed twice", samples[q]);
+		seen[samples[q]] = 1;
+	}
+	return f64 ? fps_impl<true>(m, samples, n_init, cap, n, radio, n_out, radio_out, seconds)
+	           : fps_impl<false>(m, samples, n_init, cap, n, radio, n_out, radio_out, seconds);
+}
+
+}  // namespace gp

@@ -153,6 +153,9 @@ void clear_error();
 
 // Learned band predictor (mlp.hip).
 constexpr uint32_t MLP_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1;
+int fps_gpu(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio, bool f64,
+            uint32_t *n_out, double *radio_out, double *seconds);
+
 int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh, const uint32_t *job_source,
                  uint32_t njobs, double tol, uint32_t block, gp_batch **out);
 int batch_run(gp_batch *b, gp_batch_stats *st);

@@ -25,7 +25,7 @@ from ._lib import GPError, NIL, PTP_TOL, check, lib, ptr
 __all__ = ["che", "geodesics", "parallel_toplesets_propagation_gpu",
            "parallel_toplesets_propagation_coalescence_gpu", "parallel_toplesets_propagation_cpu",
            "parallel_toplesets_propagation_coalescence_cpu", "normalize_ptp", "GPError", "NIL", "PTP_TOL",
-           "Plan", "Batch"]
+           "Plan", "Batch", "farthest_point_sampling_ptp_gpu"]
 
 
 def _u32(a):
@@ -146,6 +146,20 @@ def parallel_toplesets_propagation_cpu(mesh, sources, toplesets):
 
 def parallel_toplesets_propagation_coalescence_cpu(mesh, sources, toplesets):
     return _driver(lib().gp_parallel_toplesets_propagation_coalescence_cpu, mesh, sources, toplesets, False)
+
+
+def farthest_point_sampling_ptp_gpu(mesh, samples, n, radio=0.0, precision=L.PREC_F64):
+    """farthest_point_sampling_ptp_gpu (src/cuda/geodesics_ptp.cu:87-172):
+    returns (samples, max distance, seconds); ``samples`` are the initial ones."""
+    init = _u32(samples)
+    nv = mesh.n_vertices()
+    cap = max(len(init), min(int(n), nv) + 1)
+    buf = np.zeros(cap, np.uint32)
+    buf[:len(init)] = init
+    cnt, r, secs = C.c_uint32(), C.c_double(), C.c_double()
+    check(lib().gp_farthest_point_sampling_ptp_gpu(mesh.handle, ptr(buf), len(init), cap, int(n), float(radio),
+                                                   int(precision), C.byref(cnt), C.byref(r), C.byref(secs)))
+    return buf[:cnt.value].copy(), r.value, secs.value
 
 
 def normalize_ptp(dist):

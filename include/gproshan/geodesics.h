@@ -9,6 +9,7 @@
  *   parallel_toplesets_propagation_{gpu,coalescence_gpu,cpu,coalescence_cpu}
  *                                                         include/geodesics_ptp.h:34-40
  *   normalize_ptp                                         include/geodesics_ptp.h:46
+ *   farthest_point_sampling_ptp_gpu                       include/geodesics_ptp.h:42
  *   class geodesics (PTP_CPU / PTP_GPU options)           include/geodesics.h:18-72
  * Differences: errors throw gproshan::ptp_error (the reference asserts or
  * reads out of bounds); FM and the heat method are other algorithms and
@@ -17,6 +18,7 @@
 #ifndef GPROSHAN_GEODESICS_H
 #define GPROSHAN_GEODESICS_H
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -155,6 +157,23 @@ inline void parallel_toplesets_propagation_coalescence_cpu(const ptp_out_t &out,
 }
 
 inline void normalize_ptp(distance_t *dist, const size_t &n) { check(gp_normalize_ptp(dist, (uint32_t) n)); }
+
+// farthest_point_sampling_ptp_gpu (include/geodesics_ptp.h:42): appends to
+// samples, returns the last read maximum distance, time_fps in seconds.
+inline distance_t farthest_point_sampling_ptp_gpu(che *mesh, std::vector<index_t> &samples, double &time_fps,
+                                                  size_t n, distance_t radio = 0)
+{
+	const size_t nv = mesh->n_vertices();
+	const size_t want = n >= nv ? nv / 2 : n;
+	const size_t n0 = samples.size();
+	samples.resize(std::max(n0, want));
+	uint32_t cnt = 0;
+	double r = 0;
+	check(gp_farthest_point_sampling_ptp_gpu(mesh->handle(), samples.data(), (uint32_t) n0, (uint32_t) samples.size(),
+	                                         (uint64_t) n, radio, GP_PREC_F64, &cnt, &r, &time_fps));
+	samples.resize(cnt);
+	return r;
+}
 
 class geodesics {
   public:

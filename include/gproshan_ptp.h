@@ -260,6 +260,25 @@ int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t 
 int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, gp_mlp *mlp,
                            const gp_ptp_opts *opts, gp_plan **out);
 
+/* ---- farthest-point sampling ------------------------------------------- */
+
+/* farthest_point_sampling_ptp_gpu (src/cuda/geodesics_ptp.cu:87-172,
+ * include/geodesics_ptp.h:42): samples[cap] holds n_init distinct samples
+ * (at least one) and receives the new ones; n is the total wanted (n >=
+ * n_vertices means n_vertices / 2); rounds run while samples are wanted and
+ * the last read maximum exceeds radio (read every round when radio > 0, else
+ * in the last round), each appending the first vertex of largest distance of
+ * the multi-source field of the current samples (plain GPU driver, PTP_TOL,
+ * last-written buffer).  *n_out: samples now in the array; *radio_out: the
+ * returned max distance (the reference's return value; INF if no round read
+ * it); *seconds: wall time of the rounds.  precision GP_PREC_F64 reproduces
+ * the reference bit for bit; GP_PREC_F32 is the FP32 solve with fallback.
+ * Differences: n <= n_init adds nothing (the reference's size_t wraps), and a
+ * round whose maximum is 0 stops instead of appending a duplicate. */
+int gp_farthest_point_sampling_ptp_gpu(gp_che *mesh, uint32_t *samples, uint32_t n_init, uint32_t cap,
+                                       uint64_t n, double radio, int precision, uint32_t *n_out,
+                                       double *radio_out, double *seconds);
+
 /* ---- batched single-source fields (no reference counterpart) ------------ */
 
 /* Many independent one-source fields in one persistent launch: job q is the

@@ -114,6 +114,17 @@ int or_ptp_f32(const or_che *m, const uint32_t *src, uint32_t ns,
 int or_ptp_fixed_point(const or_che *m, const uint32_t *src, uint32_t ns,
                        double rel_eps, double *dist, uint64_t *n_relax);
 
+/* farthest_point_sampling_ptp_gpu (src/cuda/geodesics_ptp.cu:87-172): the
+ * sample budget n (>= n_vertices -> n_vertices / 2), then rounds of
+ * compute_toplesets(samples) + the GPU plain driver (last-written buffer,
+ * PTP_TOL) + the first index of the largest |distance| (cuBLAS I?amax), while
+ * rounds remain and the last read maximum exceeds radio; the maximum is read
+ * when radio > 0 or in the last round.  samples[cap] holds n_init distinct
+ * samples and receives the new ones; a round whose maximum is 0 stops (the
+ * reference would append a duplicate).  f32 != 0 runs the SINGLE_P arithmetic. */
+int or_fps(const or_che *m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio,
+           int f32, uint32_t *n_out, double *radio_out);
+
 /* compute_error (src/test_geodesics_ptp.cpp:362-371): mean relative error
  * in percent over n - s vertices. */
 double or_compute_error(const double *dist, const double *exact, uint32_t n, uint32_t s);

@@ -64,6 +64,8 @@ def lib():
                              C.POINTER(u32)]
         L.or_ptp_f32.argtypes = L.or_ptp.argtypes
         L.or_ptp_fixed_point.argtypes = [C.POINTER(_Che), P, u32, C.c_double, P, C.POINTER(C.c_uint64)]
+        L.or_fps.argtypes = [C.POINTER(_Che), P, u32, u32, C.c_uint64, C.c_double, C.c_int, C.POINTER(u32),
+                             C.POINTER(C.c_double)]
         L.or_compute_error.argtypes = [P, P, u32, u32]
         L.or_compute_error.restype = C.c_double
         L.or_normalize_ptp.argtypes = [P, u32]
@@ -148,6 +150,19 @@ class Mesh:
         if trace:
             return dist, nsw.value, tr[:min(nsw.value, cap)].copy()
         return dist, nsw.value
+
+    def fps(self, samples, n, radio=0.0, f32=False):
+        """farthest_point_sampling_ptp_gpu: (samples, radio)."""
+        init = np.ascontiguousarray(samples, dtype=np.uint32)
+        cap = max(len(init), min(n, self.nv) + 1)
+        buf = np.zeros(cap, np.uint32)
+        buf[:len(init)] = init
+        cnt = C.c_uint32(0)
+        r = C.c_double(0)
+        rc = lib().or_fps(C.byref(self._c), _p(buf), len(init), cap, n, radio, int(f32), C.byref(cnt), C.byref(r))
+        if rc != 0:
+            raise ValueError(f"or_fps failed: {rc}")
+        return buf[:cnt.value].copy(), r.value
 
     def fixed_point(self, sources, rel_eps=1e-15):
         src = np.ascontiguousarray(sources, dtype=np.uint32)

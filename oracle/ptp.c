@@ -115,6 +115,50 @@ int or_ptp_fixed_point(const or_che *m, const uint32_t *src, uint32_t ns,
 	return OR_OK;
 }
 
+int or_fps(const or_che *m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio,
+           int f32, uint32_t *n_out, double *radio_out)
+{
+	const uint32_t nv = m->nv;
+	*n_out = n_init;
+	*radio_out = INFINITY;
+	if (n_init == 0) return OR_EINVAL;
+	if (n >= nv) n = nv >> 1;                                  /* :123 */
+	const uint64_t rounds = n > n_init ? n - n_init : 0;      /* :125 (without the size_t wrap) */
+	if (n_init + rounds > cap) return OR_ECAPACITY;
+	uint32_t *levels = (uint32_t *) malloc(sizeof(uint32_t) * nv);
+	uint32_t *sorted = (uint32_t *) malloc(sizeof(uint32_t) * nv);
+	uint32_t *limits = (uint32_t *) malloc(sizeof(uint32_t) * ((size_t) nv + 2));
+	double *d64 = (double *) malloc(sizeof(double) * nv);
+	float *d32 = (float *) malloc(sizeof(float) * nv);
+	int rc = OR_OK;
+	if (!levels || !sorted || !limits || !d64 || !d32) { rc = OR_ENOMEM; goto out; }
+	uint32_t count = n_init;
+	double max_dist = INFINITY;
+	for (uint64_t left = rounds; left-- > 0 && max_dist > radio;) {     /* :132 */
+		uint32_t nl = 0, nsw = 0;
+		rc = or_toplesets(m, samples, count, OR_NIL, levels, sorted, limits, nv + 2, &nl);
+		if (rc) goto out;
+		if (f32) rc = or_ptp_f32(m, samples, count, limits, nl, sorted, 2 /* GPU_PLAIN */, 1e-3, 0, d32, NULL, 0, &nsw);
+		else rc = or_ptp(m, samples, count, limits, nl, sorted, 2 /* GPU_PLAIN */, 1e-3, 0, d64, NULL, 0, &nsw);
+		if (rc) goto out;
+		/* First index of the largest |value| (:140-142). */
+		uint32_t f = 0;
+		double best = -1;
+		for (uint32_t v = 0; v < nv; v++) {
+			const double x = f32 ? fabs((double) d32[v]) : fabs(d64[v]);
+			if (x > best) { best = x; f = v; }
+		}
+		if (best == 0) break;
+		if (radio > 0 || left == 0) max_dist = best;                   /* :144-145 */
+		samples[count++] = f;                                          /* :147 */
+	}
+	*n_out = count;
+	*radio_out = max_dist;
+out:
+	free(levels); free(sorted); free(limits); free(d64); free(d32);
+	return rc;
+}
+
 double or_compute_error(const double *dist, const double *exact, uint32_t n, uint32_t s)
 {
 	double e = 0;

@@ -53,6 +53,14 @@ int main(int argc, char **argv)
 		for (index_t v = 0; v < mesh.n_vertices(); v++) d = std::fmax(d, std::fabs(ext[v] - cpu[v]) / std::fmax(cpu[v], 1e-30));
 		std::printf("ptp_gpu vs ptp_cpu max rel diff %.3e\n", d);
 		ok &= d < 2e-3;   // last-written (GPU) vs last-read (CPU) buffer, FP32 vs FP64
+
+		// Farthest-point sampling from the corner: the first sample added is
+		// the opposite corner of the unit square.
+		std::vector<index_t> samples = { 0 };
+		double t = 0;
+		const distance_t r = farthest_point_sampling_ptp_gpu(&mesh, samples, t, 8);
+		std::printf("fps %zu samples, second %u, radio %.4f\n", samples.size(), samples[1], r);
+		ok &= samples.size() == 8 && samples[1] == n * n - 1 && r > 0 && r < 1;
 	}
 	std::printf(ok ? "OK\n" : "FAILED\n");
 	return ok ? 0 : 1;
