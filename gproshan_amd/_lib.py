@@ -32,6 +32,7 @@ EXPORTS = [
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
     "gp_normalize_ptp", "gp_ptp",
     "gp_plan_create", "gp_plan_create_bands", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
+    "gp_plan_clusters",
     "gp_mlp_create", "gp_mlp_destroy", "gp_mlp_params", "gp_mlp_features", "gp_mlp_forward", "gp_mlp_train",
     "gp_mlp_predict", "gp_plan_create_learned",
     "gp_batch_create", "gp_batch_run", "gp_batch_result", "gp_batch_destroy",
@@ -51,7 +52,7 @@ class PtpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
                 ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int),
-                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int)]
+                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int), ("clusters", C.c_int)]
 
 
 class PtpStats(C.Structure):
@@ -108,6 +109,7 @@ def lib() -> C.CDLL:
     L.gp_plan_run.argtypes = [P, C.POINTER(PtpStats)]
     L.gp_plan_result.argtypes = [P, P]
     L.gp_plan_trace.argtypes = [P, P, u32, C.POINTER(u32)]
+    L.gp_plan_clusters.argtypes = [P, P]
     L.gp_plan_destroy.argtypes = [P]
     L.gp_plan_destroy.restype = None
     L.gp_mlp_create.argtypes = [u32, C.POINTER(P)]

@@ -155,10 +155,10 @@ static int ptp_gpu_driver(gp_che *mesh, const uint32_t *sources, uint32_t ns, co
                           const uint32_t *sorted, double *dist, uint32_t *clusters, double *seconds, int coalesce)
 {
 	if (!mesh || !dist) return gp::fail(GP_ERR_INVALID, "null argument");
-	if (clusters) return gp::fail(GP_ERR_INVALID, "clusters output is not supported");
 	gp_ptp_opts o;
 	gp_ptp_opts_default(&o);
 	o.coalesce = coalesce;
+	o.clusters = clusters ? 1 : 0;
 	gp_plan *p = nullptr;
 	int rc = gp::plan_create_rings(&mesh->m, sources, ns, limits, nl, sorted, o, &p);
 	if (rc) return rc;
@@ -167,6 +167,7 @@ static int ptp_gpu_driver(gp_che *mesh, const uint32_t *sources, uint32_t ns, co
 	rc = gp::plan_run(p, &st);
 	if (rc) return rc;
 	rc = gp::plan_result(p, dist);
+	if (!rc && clusters) rc = gp::plan_clusters(p, clusters);
 	if (seconds) *seconds = st.kernel_ms * 1e-3;
 	return rc;
 }
@@ -336,6 +337,7 @@ int gp_plan_create_bands(gp_che *mesh, const uint32_t *sources, uint32_t n_sourc
 }
 
 int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats) { GP_GUARD(return gp::plan_run(plan, stats);) }
+int gp_plan_clusters(gp_plan *plan, uint32_t *clusters) { GP_GUARD(return gp::plan_clusters(plan, clusters);) }
 int gp_plan_result(gp_plan *plan, double *dist) { GP_GUARD(return gp::plan_result(plan, dist);) }
 int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap, uint32_t *n)
 {

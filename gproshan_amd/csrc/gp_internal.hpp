@@ -135,6 +135,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
                       const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
 int plan_result(gp_plan *p, double *dist);
+int plan_clusters(gp_plan *p, uint32_t *clusters);
 int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n);
 
 // Patch / band schedules (patches.hip).  dhat == null: no band gating.

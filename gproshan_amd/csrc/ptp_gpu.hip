@@ -129,11 +129,12 @@ struct KArgs {
 	float h, inv_h;
 	uint32_t nl, max_iter;
 	uint32_t k2;               // 1: K/2 limit (the drivers); 0: the uncapped iter_error loop
+	uint32_t *cl0, *cl1;       // cluster labels (double-buffered like the distances), or null
 };
 
 // Relaxes the window [start, end) (positions); returns this thread's count of
 // non-converged vertices among positions [start, cend).
-template <bool F64>
+template <bool F64, bool CL>
 __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
                                                const typename std::conditional<F64, double, float>::type *od,
                                                typename std::conditional<F64, double, float>::type *nd)
@@ -161,8 +162,24 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 		if (pos >= start && pos < end) {
 			const uint32_t v = a.order ? a.order[pos] : pos;
 			const T old = od[v];
-			const T best = relax_lane<F64>(a, (size_t) cb * 64 + lane, S, od, old);
+			const size_t base = (size_t) cb * 64 + lane;
+			uint32_t bk = NO_TRI;
+			const T best = relax_lane<F64, 64, true, CL>(a, base, S, od, old, &bk);
 			nd[v] = best;
+			if constexpr (CL) {
+				// The label of the nearer of the winning triangle's two other
+				// vertices (old values; ties to the first fan entry), as the
+				// reference's GPU relax does (src/cuda/geodesics_ptp.cu:266-276).
+				const uint32_t *ocl = od == (const T *) a.dist0 ? a.cl0 : a.cl1;
+				uint32_t *ncl = od == (const T *) a.dist0 ? a.cl1 : a.cl0;
+				uint32_t c = ocl[v];
+				if (bk != NO_TRI) {
+					const uint32_t ia = a.cid[base + 64 * (size_t) bk] & ID30_MASK;
+					const uint32_t ib = a.cid[base + 64 * (size_t) (bk + 1)] & ID30_MASK;
+					c = od[ib] < od[ia] ? ocl[ib] : ocl[ia];
+				}
+				ncl[v] = c;
+			}
 			if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
 		}
 	}
@@ -176,7 +193,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 }
 
 // Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
-template <bool F64>
+template <bool F64, bool CL>
 __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 {
 	using T = typename std::conditional<F64, double, float>::type;
@@ -224,7 +241,7 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = wave_sum(sweep_body<F64>(a, start, end, cend, od, nd));
+	const uint32_t bad = wave_sum(sweep_body<F64, CL>(a, start, end, cend, od, nd));
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 }
 
@@ -247,6 +264,18 @@ __global__ void k_sources(T *d0, T *d1, const uint32_t *src, uint32_t ns)
 {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t < ns) { d0[src[t]] = (T) 0; d1[src[t]] = (T) 0; }
+}
+
+// Cluster labels: NIL everywhere, then source q gets q + 1
+// (src/cuda/geodesics_ptp.cu:186-195; the reference leaves the others unset).
+__global__ void k_cl_init(uint32_t *c0, uint32_t *c1, uint32_t n)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) c0[v] = c1[v] = GP_NIL;
+}
+__global__ void k_cl_sources(uint32_t *c0, uint32_t *c1, const uint32_t *src, uint32_t ns)
+{
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < ns) c0[src[t]] = c1[src[t]] = t + 1;
 }
 
 template <typename T>
@@ -277,6 +306,7 @@ struct gp_plan {
 	float4 *d_cgeo = nullptr;
 	double4 *d_cgeo64 = nullptr, *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
+	uint32_t *d_cl0 = nullptr, *d_cl1 = nullptr;   // cluster labels (opts.clusters)
 	double *d_out = nullptr;
 	gp::PState *d_state = nullptr, *h_state = nullptr;   // h_state: 2 pinned snapshots
 	hipStream_t stream = nullptr;
@@ -288,7 +318,7 @@ struct gp_plan {
 	{
 		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
 		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
-		                 (void *) d_out, (void *) d_state, (void *) d_fcum })
+		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_cl0, (void *) d_cl1 })
 			(void) hipFree(p);
 		if (pp) gp::patch_destroy(pp);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
@@ -371,8 +401,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true>, 256, 0));
-	else GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<false>, 256, 0));
+	if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true, false>, 256, 0));
+	else GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<false, false>, 256, 0));
 	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
 
@@ -404,6 +434,9 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		std::vector<uint32_t> ord(sorted, sorted + nr);
 		if ((rc = upload(p->d_order, ord))) return rc;
 	}
+	if (opts.clusters &&
+	    (hipMalloc(&p->d_cl0, 4ull * p->n) != hipSuccess || hipMalloc(&p->d_cl1, 4ull * p->n) != hipSuccess))
+		return fail(GP_ERR_NOMEM, "plan: cluster allocation failed");
 	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
 	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
@@ -422,6 +455,7 @@ static KArgs make_args(const gp_plan *p)
 	a.st = p->d_state; a.trace = p->d_trace;
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
 	a.nl = p->nl; a.max_iter = p->max_iter; a.k2 = p->opts.uncapped ? 0u : 1u;
+	a.cl0 = p->d_cl0; a.cl1 = p->d_cl1;
 	return a;
 }
 
@@ -430,8 +464,14 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 {
 	for (uint32_t q = 0; q < count; q++) {
 		const uint32_t k4 = (k0 + q) & 3u;
-		if (p->opts.precision == GP_PREC_F64) k_sweep<true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-		else k_sweep<false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		const bool cl = a.cl0 != nullptr;
+		if (p->opts.precision == GP_PREC_F64) {
+			if (cl) k_sweep<true, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else k_sweep<true, false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		} else {
+			if (cl) k_sweep<false, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else k_sweep<false, false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+		}
 	}
 }
 
@@ -469,6 +509,10 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	} else {
 		k_init<float><<<ig, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->n, p->d_state, done0);
 		k_sources<float><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->d_src, p->ns);
+	}
+	if (p->d_cl0) {
+		k_cl_init<<<ig, 256, 0, p->stream>>>(p->d_cl0, p->d_cl1, p->n);
+		k_cl_sources<<<(p->ns + 255) / 256, 256, 0, p->stream>>>(p->d_cl0, p->d_cl1, p->d_src, p->ns);
 	}
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipEventRecord(p->e0, p->stream));
@@ -559,6 +603,25 @@ int plan_result(gp_plan *p, double *dist)
 	} else {
 		GP_HIP(hipMemcpyAsync(dist, p->d_out, sizeof(double) * p->n, hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipStreamSynchronize(p->stream));
+	}
+	return GP_OK;
+}
+
+int plan_clusters(gp_plan *p, uint32_t *out)
+{
+	if (!p || !out) return fail(GP_ERR_INVALID, "plan_clusters: null argument");
+	if (p->pp || !p->d_cl0) return fail(GP_ERR_INVALID, "plan_clusters: the plan was made without clusters");
+	if (!p->has_run) return fail(GP_ERR_INVALID, "plan_clusters: plan has not run");
+	const uint32_t d = p->h_state->final_d;
+	const bool last_read = p->opts.result_buffer == GP_RESULT_LAST_READ;
+	const uint32_t *buf = (last_read ? !d : d) ? p->d_cl1 : p->d_cl0;
+	std::vector<uint32_t> tmp(p->n);
+	GP_HIP(hipMemcpyAsync(tmp.data(), buf, 4ull * p->n, hipMemcpyDeviceToHost, p->stream));
+	GP_HIP(hipStreamSynchronize(p->stream));
+	if (p->coal) {
+		for (uint32_t v = 0; v < p->nv_orig; v++) out[v] = p->inv[v] != NIL ? tmp[p->inv[v]] : GP_NIL;
+	} else {
+		std::memcpy(out, tmp.data(), 4ull * p->n);
 	}
 	return GP_OK;
 }

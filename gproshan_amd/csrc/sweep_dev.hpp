@@ -35,13 +35,32 @@ __device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, 
 // flagged triangles are skipped in the slot loop and done afterwards in a
 // loop that runs as many times as the most-flagged lane of the wave needs,
 // so the FP64 code runs ~1-2 times per wave instead of once per slot.
-template <bool F64, uint32_t ST = 64, bool PF = true, typename A>
+//
+// CL: also report in *bk the triangle (slot k of triangle (k, k+1)) that set
+// the new value -- the first in fan order among those reaching the minimum,
+// NO_TRI if the value did not drop -- for the cluster label.
+constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
+
+template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
 relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
-           typename std::conditional<F64, double, float>::type best)
+           typename std::conditional<F64, double, float>::type best, uint32_t *bk_out = nullptr)
 {
 	using T = typename std::conditional<F64, double, float>::type;
 	using G = typename std::conditional<F64, double4, float4>::type;
+	const T old = best;
+	uint32_t bk = NO_TRI;
+	// Strictly lower wins; an equal value from an earlier triangle (evaluated
+	// later because FP64 triangles are deferred) takes the label, as the
+	// in-order scan of the reference would have kept it.
+	auto take = [&](T p, uint32_t k) {
+		if (p < best) {
+			best = p;
+			if constexpr (CL) bk = k;
+		} else if constexpr (CL) {
+			if (p == best && p < old && k < bk) bk = k;
+		}
+	};
 	const G *geo;
 	if constexpr (F64) geo = a.cx64; else geo = a.cgeo;
 	uint32_t fmask = 0;
@@ -79,25 +98,22 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 		for (uint32_t k = 0; k + 1 < MAXS; k++) {
 			if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
 				if constexpr (F64) {
-					const T p = tri_f64(g[k], g[k + 1], t[k], t[k + 1]);
-					if (p < best) best = p;
+					take(tri_f64(g[k], g[k + 1], t[k], t[k + 1]), k);
 				} else {
 					if (id[k] & F64_FLAG) {
 						if (k == k0) { fa = t[k]; fb = t[k + 1]; }
 						else fmask |= 1u << k;
 					} else {
-						const T p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1],
-						                                 a.h, a.inv_h);
-						if (p < best) best = p;
+						take(update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1], a.h,
+						                          a.inv_h), k);
 					}
 				}
 			}
 		}
 		if constexpr (!F64) {
 			if (k0 != MAXS) {
-				const T p = (T) update_stable<double>(gd0.x, q10, gd0.y, gd0.z, gd0.w, (double) fa, (double) fb,
-				                                      (double) a.h, (double) a.inv_h);
-				if (p < best) best = p;
+				take((T) update_stable<double>(gd0.x, q10, gd0.y, gd0.z, gd0.w, (double) fa, (double) fb,
+				                               (double) a.h, (double) a.inv_h), k0);
 			}
 		}
 	} else {
@@ -110,12 +126,11 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 			const G gb = geo[base + ST * k];
 			const T tb = od[idb & ID30_MASK];
 			if constexpr (F64) {
-				const T p = tri_f64(ga, gb, ta, tb);
-				if (p < best) best = p;
+				take(tri_f64(ga, gb, ta, tb), k - 1);
 			} else {
 				const T p = (ida & F64_FLAG) ? tri_flagged<ST>(a, base + ST * (k - 1), ta, tb)
 				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
-				if (p < best) best = p;
+				take(p, k - 1);
 			}
 			ida = idb; ga = gb; ta = tb;
 		}
@@ -127,10 +142,10 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 			const size_t at = base + ST * (size_t) k;
 			const float ta = od[a.cid[at] & ID30_MASK];
 			const float tb = od[a.cid[at + ST] & ID30_MASK];
-			const float p = tri_flagged<ST>(a, at, ta, tb);
-			if (p < best) best = p;
+			take(tri_flagged<ST>(a, at, ta, tb), k);
 		}
 	}
+	if constexpr (CL) *bk_out = bk;
 	return best;
 }
 

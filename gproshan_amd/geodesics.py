@@ -10,7 +10,9 @@ Names, argument meaning and error behaviour follow the reference:
 
 Differences, by design: errors are raised as ``GPError`` instead of asserts or
 out-of-bounds accesses (duplicate sources, an empty source list); the GPU
-arithmetic precision and band schedule are keyword options.
+arithmetic precision and band schedule are keyword options; cluster labels
+come from PTP_GPU only (the reference's CPU rule is a racy in-place update),
+NIL where unreached.
 """
 from __future__ import annotations
 
@@ -117,27 +119,30 @@ class che:
         return top, srt, lim[:nl.value].copy()
 
 
-def _driver(fn, mesh, sources, toplesets, gpu):
+def _driver(fn, mesh, sources, toplesets, gpu, clusters=False):
     src = _u32(sources)
     _, srt, lim = toplesets
     srt, lim = _u32(srt), _u32(lim)
     dist = np.full(mesh.n_vertices(), np.inf)
     if gpu:
         secs = C.c_double()
-        check(fn(mesh.handle, ptr(src), len(src), ptr(lim), len(lim), ptr(srt), ptr(dist), None, C.byref(secs)))
-        return dist, secs.value
+        cl = np.zeros(mesh.n_vertices(), np.uint32) if clusters else None
+        check(fn(mesh.handle, ptr(src), len(src), ptr(lim), len(lim), ptr(srt), ptr(dist),
+                 ptr(cl) if clusters else None, C.byref(secs)))
+        return (dist, secs.value, cl) if clusters else (dist, secs.value)
     check(fn(mesh.handle, ptr(src), len(src), ptr(lim), len(lim), ptr(srt), ptr(dist)))
     return dist
 
 
-def parallel_toplesets_propagation_gpu(mesh, sources, toplesets):
-    """Returns (dist, seconds); sorted-indirected window sweeps on the GPU."""
-    return _driver(lib().gp_parallel_toplesets_propagation_gpu, mesh, sources, toplesets, True)
+def parallel_toplesets_propagation_gpu(mesh, sources, toplesets, clusters=False):
+    """Returns (dist, seconds[, clusters]); sorted-indirected window sweeps on the GPU."""
+    return _driver(lib().gp_parallel_toplesets_propagation_gpu, mesh, sources, toplesets, True, clusters)
 
 
-def parallel_toplesets_propagation_coalescence_gpu(mesh, sources, toplesets):
-    """Returns (dist, seconds); ring-renumbered window sweeps on the GPU."""
-    return _driver(lib().gp_parallel_toplesets_propagation_coalescence_gpu, mesh, sources, toplesets, True)
+def parallel_toplesets_propagation_coalescence_gpu(mesh, sources, toplesets, clusters=False):
+    """Returns (dist, seconds[, clusters]); ring-renumbered window sweeps on the GPU."""
+    return _driver(lib().gp_parallel_toplesets_propagation_coalescence_gpu, mesh, sources, toplesets, True,
+                   clusters)
 
 
 def parallel_toplesets_propagation_cpu(mesh, sources, toplesets):
@@ -188,7 +193,7 @@ class Plan:
     def __init__(self, mesh, sources, dhat=None, **kw):
         self.mesh = mesh
         self._src = _u32(sources)
-        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped") else v) for k, v in kw.items()})
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters") else v) for k, v in kw.items()})
         h = C.c_void_p()
         if dhat is None:
             check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
@@ -209,6 +214,11 @@ class Plan:
         d = np.zeros(self.mesh.n_vertices())
         check(lib().gp_plan_result(self._h, ptr(d)))
         return d
+
+    def clusters(self):
+        c = np.zeros(self.mesh.n_vertices(), np.uint32)
+        check(lib().gp_plan_clusters(self._h, ptr(c)))
+        return c
 
     def trace(self):
         n = C.c_uint32()
@@ -282,8 +292,8 @@ class geodesics:
         src = _u32(sources)
         if len(src) == 0:
             raise GPError(L.ERR_INVALID, "no sources")            # src/geodesics.cpp:31
-        if cluster:
-            raise GPError(L.ERR_INVALID, "cluster output is not supported")
+        if cluster and opt != self.PTP_GPU:
+            raise GPError(L.ERR_INVALID, "cluster labels are produced by PTP_GPU only")
         if e_dist is not None and (not isinstance(e_dist, np.ndarray) or e_dist.dtype != np.float64
                                    or e_dist.size < n or not e_dist.flags.c_contiguous):
             raise GPError(L.ERR_INVALID, "e_dist must be a contiguous float64 array of n_vertices")
@@ -297,6 +307,15 @@ class geodesics:
             ts = mesh.compute_toplesets(src)                        # src/geodesics.cpp:192-207
             self._sorted[:] = ts[1]
             self._dist[:n] = parallel_toplesets_propagation_coalescence_cpu(mesh, src, ts)
+        elif opt == self.PTP_GPU and cluster:
+            # The reference's GPU drivers with their cluster output
+            # (src/geodesics.cpp:225-240).
+            ts = mesh.compute_toplesets(src)
+            drv = parallel_toplesets_propagation_coalescence_gpu if len(src) == 1 else parallel_toplesets_propagation_gpu
+            d, _, cl = drv(mesh, src, ts, clusters=True)
+            self._sorted[:] = ts[1]
+            self._dist[:n] = d
+            self.clusters = cl
         elif opt == self.PTP_GPU:
             # One source: ring-renumbered (coalesced) sweeps; several: sorted[]
             # indirection (src/geodesics.cpp:225-240).

@@ -13,7 +13,9 @@
  *   class geodesics (PTP_CPU / PTP_GPU options)           include/geodesics.h:18-72
  * Differences: errors throw gproshan::ptp_error (the reference asserts or
  * reads out of bounds); FM and the heat method are other algorithms and
- * throw when requested; clusters are not produced.
+ * throw when requested; cluster labels come from PTP_GPU only (the
+ * reference's CPU rule is a racy in-place update) and are NIL where
+ * unreached.
  */
 #ifndef GPROSHAN_GEODESICS_H
 #define GPROSHAN_GEODESICS_H
@@ -190,7 +192,7 @@ class geodesics {
 		(void) n_iter; (void) radio;                      // ignored by PTP in the reference too
 		if (n_vertices_ == 0) throw ptp_error(GP_ERR_INVALID, "mesh has no vertices");
 		if (sources.empty()) throw ptp_error(GP_ERR_INVALID, "no sources");
-		if (cluster) throw ptp_error(GP_ERR_INVALID, "cluster output is not supported");
+		if (cluster && opt != PTP_GPU) throw ptp_error(GP_ERR_INVALID, "cluster labels are produced by PTP_GPU only");
 		if (opt != PTP_CPU && opt != PTP_GPU) throw ptp_error(GP_ERR_INVALID, "only PTP_CPU and PTP_GPU are provided");
 		free_dist_ = e_dist == nullptr;
 		dist_ = free_dist_ ? new distance_t[n_vertices_] : e_dist;
@@ -203,6 +205,17 @@ class geodesics {
 				index_t *tp = top.data();
 				mesh->compute_toplesets(tp, sorted_index_, limits, sources);
 				parallel_toplesets_propagation_coalescence_cpu({ dist_ }, mesh, sources, { limits, sorted_index_ });
+			} else if (cluster) {
+				// The GPU drivers with their cluster output (src/geodesics.cpp:225-240).
+				clusters = new index_t[n_vertices_];
+				std::vector<index_t> top(n_vertices_), limits;
+				index_t *tp = top.data();
+				mesh->compute_toplesets(tp, sorted_index_, limits, sources);
+				if (sources.size() == 1)
+					parallel_toplesets_propagation_coalescence_gpu({ dist_, clusters }, mesh, sources,
+					                                               { limits, sorted_index_ });
+				else
+					parallel_toplesets_propagation_gpu({ dist_, clusters }, mesh, sources, { limits, sorted_index_ });
 			} else {
 				gp_ptp_opts o;
 				gp_ptp_opts_default(&o);
@@ -246,8 +259,10 @@ class geodesics {
 	{
 		if (free_dist_) delete[] dist_;
 		delete[] sorted_index_;
+		delete[] clusters;
 		dist_ = nullptr;
 		sorted_index_ = nullptr;
+		clusters = nullptr;
 	}
 
 	distance_t *dist_ = nullptr;

@@ -96,6 +96,11 @@ typedef struct gp_ptp_opts {
 	                       loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248):
 	                       no K/2 freeze, no 2*|limits| cap; it runs to the
 	                       relaxation's fixed point as tol -> 0               */
+	int clusters;       /* ring schedule: 1 = also carry cluster (Voronoi) labels,
+	                       the GPU drivers' rule (src/cuda/geodesics_ptp.cu:257-282):
+	                       source q is q + 1, a vertex whose value drops takes the
+	                       label of the nearer vertex of the winning triangle
+	                       (old values), NIL where unreached                  */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {
@@ -171,7 +176,8 @@ int gp_compute_toplesets_gpu(const gp_che *mesh, const uint32_t *sources, uint32
  * (include/geodesics_ptp.h:36; src/cuda/geodesics_ptp.cu:20-85,174-299):
  * the window schedule over sorted[] on the original numbering; writes
  * dist[n_vertices] (INF where unreached) and returns the device seconds in
- * *seconds (may be NULL).  clusters is reserved and must be NULL. */
+ * *seconds (may be NULL).  clusters (may be NULL) receives the cluster labels
+ * (gp_ptp_opts.clusters). */
 int gp_parallel_toplesets_propagation_gpu(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
                                           const uint32_t *limits, uint32_t n_limits,
                                           const uint32_t *sorted, double *dist,
@@ -226,6 +232,9 @@ int gp_plan_create(gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
 int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats);
 int gp_plan_result(gp_plan *plan, double *dist);
 int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap_sweeps, uint32_t *n_sweeps);
+/* Cluster labels[n_vertices] of the last run (plans made with opts.clusters),
+ * from the same buffer as gp_plan_result. */
+int gp_plan_clusters(gp_plan *plan, uint32_t *clusters);
 void gp_plan_destroy(gp_plan *plan);
 
 /* ---- learned band predictor (no reference counterpart) ------------------ */

@@ -100,6 +100,15 @@ int or_ptp(const or_che *m, const uint32_t *src, uint32_t ns,
            int variant, double tol, int n_threads,
            double *dist, or_sweep *trace, uint32_t trace_cap, uint32_t *n_sweeps);
 
+/* The FP64 schedule with cluster labels (the GPU drivers' rule,
+ * src/cuda/geodesics_ptp.cu:257-282): clusters[n_vertices] receives, from
+ * the same buffer as the distances, the 1-based index of the source whose
+ * front reached each vertex (NIL where unreached). */
+int or_ptp_clusters(const or_che *m, const uint32_t *src, uint32_t ns,
+                    const uint32_t *limits, uint32_t nlimits, const uint32_t *sorted,
+                    int variant, double tol, int n_threads,
+                    double *dist, uint32_t *clusters, uint32_t *n_sweeps);
+
 /* The same schedule in the reference's SINGLE_P build (all float). */
 int or_ptp_f32(const or_che *m, const uint32_t *src, uint32_t ns,
                const uint32_t *limits, uint32_t nlimits, const uint32_t *sorted,

@@ -64,6 +64,8 @@ def lib():
                              C.POINTER(u32)]
         L.or_ptp_f32.argtypes = L.or_ptp.argtypes
         L.or_ptp_fixed_point.argtypes = [C.POINTER(_Che), P, u32, C.c_double, P, C.POINTER(C.c_uint64)]
+        L.or_ptp_clusters.argtypes = [C.POINTER(_Che), P, u32, P, u32, P, C.c_int, C.c_double, C.c_int, P, P,
+                                      C.POINTER(u32)]
         L.or_fps.argtypes = [C.POINTER(_Che), P, u32, u32, C.c_uint64, C.c_double, C.c_int, C.POINTER(u32),
                              C.POINTER(C.c_double)]
         L.or_compute_error.argtypes = [P, P, u32, u32]
@@ -150,6 +152,21 @@ class Mesh:
         if trace:
             return dist, nsw.value, tr[:min(nsw.value, cap)].copy()
         return dist, nsw.value
+
+    def ptp_clusters(self, sources, variant=GPU_COAL, tol=PTP_TOL):
+        """(dist, clusters, sweeps) of the FP64 schedule with cluster labels."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        _, sorted_, limits = self.toplesets(src)
+        limits = np.ascontiguousarray(limits, dtype=np.uint32)
+        sorted_ = np.ascontiguousarray(sorted_, dtype=np.uint32)
+        dist = np.zeros(self.nv, np.float64)
+        cl = np.zeros(self.nv, np.uint32)
+        nsw = C.c_uint32(0)
+        rc = lib().or_ptp_clusters(C.byref(self._c), _p(src), len(src), _p(limits), len(limits), _p(sorted_),
+                                   variant, tol, 0, _p(dist), _p(cl), C.byref(nsw))
+        if rc != 0:
+            raise ValueError(f"or_ptp_clusters failed: {rc}")
+        return dist, cl, nsw.value
 
     def fps(self, samples, n, radio=0.0, f32=False):
         """farthest_point_sampling_ptp_gpu: (samples, radio)."""

@@ -54,7 +54,17 @@ int or_ptp(const or_che *m, const uint32_t *src, uint32_t ns,
            double *dist, or_sweep *trace, uint32_t trace_cap, uint32_t *n_sweeps)
 {
 	return ptp_f64(m, src, ns, limits, nlimits, sorted, variant, tol, n_threads,
-	               dist, trace, trace_cap, n_sweeps);
+	               dist, trace, trace_cap, n_sweeps, NULL);
+}
+
+int or_ptp_clusters(const or_che *m, const uint32_t *src, uint32_t ns,
+                    const uint32_t *limits, uint32_t nlimits, const uint32_t *sorted,
+                    int variant, double tol, int n_threads,
+                    double *dist, uint32_t *clusters, uint32_t *n_sweeps)
+{
+	if (!clusters) return OR_EINVAL;
+	return ptp_f64(m, src, ns, limits, nlimits, sorted, variant, tol, n_threads,
+	               dist, NULL, 0, n_sweeps, clusters);
 }
 
 int or_ptp_f32(const or_che *m, const uint32_t *src, uint32_t ns,
@@ -63,7 +73,7 @@ int or_ptp_f32(const or_che *m, const uint32_t *src, uint32_t ns,
                float *dist, or_sweep *trace, uint32_t trace_cap, uint32_t *n_sweeps)
 {
 	return ptp_f32(m, src, ns, limits, nlimits, sorted, variant, tol, n_threads,
-	               dist, trace, trace_cap, n_sweeps);
+	               dist, trace, trace_cap, n_sweeps, NULL);
 }
 
 /* FIFO label-correcting relaxation to the fixed point (see oracle.h).

@@ -95,6 +95,31 @@ static inline REAL SFX(relax)(const SFX(mview) *m, const REAL *old_d, uint32_t v
 	return best;
 }
 
+/* The same relaxation carrying the cluster (Voronoi) label: when a triangle
+ * lowers the value, the vertex takes the label of the nearer (by old
+ * distance, ties to the next vertex) of that triangle's two other vertices --
+ * the rule of the reference's GPU drivers, read from the old buffers
+ * (src/cuda/geodesics_ptp.cu:257-282, src/cuda/geodesics_ptp_coalescence.cu:185-207). */
+static inline REAL SFX(relax_cl)(const SFX(mview) *m, const REAL *old_d, const uint32_t *old_c, uint32_t v,
+                                 uint32_t *new_c)
+{
+	REAL best = old_d[v];
+	uint32_t c = old_c[v];
+	const uint32_t stop = m->evt[v];
+	for (uint32_t he = stop; he != OR_NIL; ) {
+		const REAL p = SFX(update_step)(m, old_d, he);
+		if (p < best) {
+			best = p;
+			const uint32_t xp = m->vt[or_prev(he)], xn = m->vt[or_next(he)];
+			c = old_d[xp] < old_d[xn] ? old_c[xp] : old_c[xn];
+		}
+		const uint32_t h2 = m->ot[or_prev(he)];
+		he = (h2 != stop) ? h2 : OR_NIL;
+	}
+	*new_c = c;
+	return best;
+}
+
 /* The window schedule shared by the four reference drivers
  * (src/geodesics_ptp.cpp:61-109 and 137-189; src/cuda/geodesics_ptp.cu:198-231;
  * src/cuda/geodesics_ptp_coalescence.cu:125-159), and, with OR_UNCAPPED, by the
@@ -109,7 +134,7 @@ static inline REAL SFX(relax)(const SFX(mview) *m, const REAL *old_d, uint32_t v
 static REAL *SFX(schedule)(const SFX(mview) *m, const uint32_t *limits, uint32_t nl,
                            const uint32_t *order, int variant, double tol, int nthreads,
                            REAL *pd0, REAL *pd1, REAL *err,
-                           or_sweep *trace, uint32_t cap, uint32_t *nsw)
+                           or_sweep *trace, uint32_t cap, uint32_t *nsw, uint32_t **pc)
 {
 	REAL *pd[2] = { pd0, pd1 };
 	const int uncapped = (variant & OR_UNCAPPED) != 0;
@@ -132,7 +157,8 @@ static REAL *SFX(schedule)(const SFX(mview) *m, const uint32_t *limits, uint32_t
 		#pragma omp parallel for schedule(static) num_threads(nthreads)
 		for (int64_t vi = start; vi < (int64_t) end; vi++) {
 			const uint32_t v = order ? order[vi] : (uint32_t) vi;
-			nd[v] = SFX(relax)(m, od, v);
+			if (pc) nd[v] = SFX(relax_cl)(m, od, pc[d], v, &pc[!d][v]);
+			else nd[v] = SFX(relax)(m, od, v);
 		}
 
 		uint32_t count = 0;
@@ -165,7 +191,7 @@ static REAL *SFX(schedule)(const SFX(mview) *m, const uint32_t *limits, uint32_t
 static int SFX(ptp)(const or_che *mesh, const uint32_t *src, uint32_t ns,
                     const uint32_t *limits, uint32_t nl, const uint32_t *sorted,
                     int variant, double tol, int nthreads,
-                    REAL *dist, or_sweep *trace, uint32_t cap, uint32_t *nsw)
+                    REAL *dist, or_sweep *trace, uint32_t cap, uint32_t *nsw, uint32_t *clusters)
 {
 	if (nthreads <= 0) nthreads = omp_get_max_threads();
 	const uint32_t nv = mesh->nv;
@@ -224,20 +250,39 @@ static int SFX(ptp)(const or_che *mesh, const uint32_t *src, uint32_t ns,
 		return OR_ENOMEM;
 	}
 	for (uint32_t v = 0; v < n; v++) pd0[v] = pd1[v] = (REAL) INFINITY;
+	/* Cluster labels: source q gets q + 1 (src/cuda/geodesics_ptp.cu:188-189);
+	 * every other vertex starts at NIL (the reference leaves it unset). */
+	uint32_t *pc[2] = { NULL, NULL };
+	if (clusters) {
+		pc[0] = (uint32_t *) malloc(sizeof(uint32_t) * n);
+		pc[1] = (uint32_t *) malloc(sizeof(uint32_t) * n);
+		if (!pc[0] || !pc[1]) {
+			free(pc[0]); free(pc[1]); free(pd0); free(pd1); free(err); free(gt);
+			if (inv) { free(inv); or_che_free(&cm); }
+			return OR_ENOMEM;
+		}
+		for (uint32_t v = 0; v < n; v++) pc[0][v] = pc[1][v] = OR_NIL;
+	}
 	for (uint32_t q = 0; q < ns; q++) {
 		const uint32_t s = inv ? inv[src[q]] : src[q];
 		pd0[s] = pd1[s] = 0;
+		if (clusters) pc[0][s] = pc[1][s] = q + 1;
 	}
 
 	REAL *res = SFX(schedule)(&mv, limits, nl, inv ? NULL : sorted, variant, tol, nthreads,
-	                          pd0, pd1, err, trace, cap, nsw);
+	                          pd0, pd1, err, trace, cap, nsw, clusters ? pc : NULL);
+	const uint32_t *cres = clusters ? (res == pd0 ? pc[0] : pc[1]) : NULL;
 
 	if (inv) {
 		for (uint32_t v = 0; v < nv; v++) dist[v] = inv[v] != OR_NIL ? res[inv[v]] : (REAL) INFINITY;
+		if (clusters)
+			for (uint32_t v = 0; v < nv; v++) clusters[v] = inv[v] != OR_NIL ? cres[inv[v]] : OR_NIL;
 	} else {
 		memcpy(dist, res, sizeof(REAL) * nv);
+		if (clusters) memcpy(clusters, cres, sizeof(uint32_t) * nv);
 	}
 
+	free(pc[0]); free(pc[1]);
 	free(pd0); free(pd1); free(err); free(gt);
 	if (inv) { free(inv); or_che_free(&cm); }
 	return OR_OK;

@@ -54,6 +54,15 @@ int main(int argc, char **argv)
 		std::printf("ptp_gpu vs ptp_cpu max rel diff %.3e\n", d);
 		ok &= d < 2e-3;   // last-written (GPU) vs last-read (CPU) buffer, FP32 vs FP64
 
+		// Cluster labels of two sources at opposite corners: each corner's
+		// own neighbourhood carries its label (1-based source index).
+		{
+			geodesics gc(&mesh, { 0, n * n - 1 }, geodesics::PTP_GPU, nullptr, true);
+			std::printf("clusters %u %u %u %u\n", gc.clusters[0], gc.clusters[1], gc.clusters[n * n - 1],
+			            gc.clusters[n * n - 2]);
+			ok &= gc.clusters[0] == 1 && gc.clusters[1] == 1 && gc.clusters[n * n - 1] == 2 && gc.clusters[n * n - 2] == 2;
+		}
+
 		// Farthest-point sampling from the corner: the first sample added is
 		// the opposite corner of the unit square.
 		std::vector<index_t> samples = { 0 };
