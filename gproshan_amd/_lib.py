@@ -26,7 +26,7 @@ LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
 # Every symbol include/gproshan_ptp.h declares.
 EXPORTS = [
     "gp_ptp_opts_default", "gp_last_error", "gp_device_check",
-    "gp_che_create", "gp_che_destroy", "gp_che_sizes", "gp_che_tables", "gp_che_link", "gp_che_mean_edge",
+    "gp_che_create", "gp_che_destroy", "gp_che_read", "gp_che_positions", "gp_che_sizes", "gp_che_tables", "gp_che_link", "gp_che_mean_edge",
     "gp_compute_toplesets", "gp_compute_toplesets_gpu",
     "gp_parallel_toplesets_propagation_gpu", "gp_parallel_toplesets_propagation_coalescence_gpu",
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
     L.gp_last_error.restype = C.c_char_p
     L.gp_device_check.argtypes = [C.c_char_p, C.c_size_t]
     L.gp_che_create.argtypes = [P, u32, P, u32, C.POINTER(P)]
+    L.gp_che_read.argtypes = [C.c_char_p, C.POINTER(P)]
+    L.gp_che_positions.argtypes = [P, P]
     L.gp_che_destroy.argtypes = [P]
     L.gp_che_destroy.restype = None
     L.gp_che_sizes.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), C.POINTER(i32)]

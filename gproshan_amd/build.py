@@ -24,7 +24,7 @@ LIB = os.path.join(HERE, "libgproshan_ptp.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["ptp_gpu.hip", "patches.hip", "rings.hip", "mlp.hip", "batch.hip", "fps.hip"]
-CPP_SOURCES = ["che.cpp", "ptp_cpu.cpp", "capi.cpp"]
+CPP_SOURCES = ["che.cpp", "ptp_cpu.cpp", "capi.cpp", "mesh_io.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -65,6 +65,28 @@ int gp_che_create(const double *xyz, uint32_t n_vertices, const uint32_t *faces,
 
 void gp_che_destroy(gp_che *mesh) { delete mesh; }
 
+int gp_che_positions(const gp_che *mesh, double *xyz)
+{
+	GP_GUARD(
+		if (!mesh || !xyz) return gp::fail(GP_ERR_INVALID, "null argument");
+		std::memcpy(xyz, mesh->m.gt.data(), sizeof(double) * mesh->m.gt.size());
+		return GP_OK;
+	)
+}
+
+int gp_che_read(const char *path, gp_che **out)
+{
+	GP_GUARD(
+		if (!out) return gp::fail(GP_ERR_INVALID, "out is null");
+		*out = nullptr;
+		std::vector<double> xyz;
+		std::vector<uint32_t> faces;
+		int rc = gp::read_mesh(path, xyz, faces);
+		if (rc) return rc;
+		return gp_che_create(xyz.data(), (uint32_t) (xyz.size() / 3), faces.data(), (uint32_t) (faces.size() / 3), out);
+	)
+}
+
 int gp_che_sizes(const gp_che *mesh, uint32_t *nv, uint32_t *nf, uint32_t *nhe, uint32_t *ne, int *manifold)
 {
 	if (!mesh) return gp::fail(GP_ERR_INVALID, "mesh is null");

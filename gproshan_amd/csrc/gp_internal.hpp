@@ -109,6 +109,9 @@ void morton_order(const Che &m, std::vector<uint32_t> &order);
 int band_order(const Che &m, const uint32_t *src, uint32_t ns, const uint32_t *levels, const double *dhat,
                double width, std::vector<uint32_t> &order, std::vector<uint32_t> &limits);
 
+// OFF / OBJ / PLY reader (mesh_io.cpp): positions [3*nv] and triangle corners.
+int read_mesh(const char *path, std::vector<double> &xyz, std::vector<uint32_t> &faces);
+
 // Builds VT/OT/EVT per the reference's rules and the fan tables.
 int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf, Che &m);
 

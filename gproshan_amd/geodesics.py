@@ -43,6 +43,21 @@ class che:
         h = C.c_void_p()
         check(lib().gp_che_create(ptr(self._xyz), len(self._xyz), ptr(self._faces), len(self._faces) // 3,
                                   C.byref(h)))
+        self._attach(h)
+
+    @classmethod
+    def read(cls, path):
+        """che_off / che_obj / che_ply: a mesh file by extension (.off, .obj, .ply)."""
+        h = C.c_void_p()
+        check(lib().gp_che_read(str(path).encode(), C.byref(h)))
+        obj = cls.__new__(cls)
+        obj._attach(h)
+        obj._xyz = np.zeros((obj._nv, 3))
+        check(lib().gp_che_positions(h, ptr(obj._xyz)))
+        obj._faces = obj.tables()[0].copy()
+        return obj
+
+    def _attach(self, h):
         self._h = h
         nv, nf, nhe, ne, man = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int()
         check(lib().gp_che_sizes(self._h, C.byref(nv), C.byref(nf), C.byref(nhe), C.byref(ne), C.byref(man)))

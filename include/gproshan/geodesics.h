@@ -10,6 +10,7 @@
  *                                                         include/geodesics_ptp.h:34-40
  *   normalize_ptp                                         include/geodesics_ptp.h:46
  *   farthest_point_sampling_ptp_gpu                       include/geodesics_ptp.h:42
+ *   che_off / che_obj / che_ply (file readers)            src/che_off.cpp, che_obj.cpp, che_ply.cpp
  *   class geodesics (PTP_CPU / PTP_GPU options)           include/geodesics.h:18-72
  * Differences: errors throw gproshan::ptp_error (the reference asserts or
  * reads out of bounds); FM and the heat method are other algorithms and
@@ -58,12 +59,13 @@ class che {
 	che(const vertex *vertices, const index_t &n_v, const index_t *faces, const index_t &n_f)
 	{
 		check(gp_che_create(&vertices[0].x, n_v, faces, n_f, &h_));
-		int man = 0;
-		uint32_t nv, nf, nhe, ne;
-		check(gp_che_sizes(h_, &nv, &nf, &nhe, &ne, &man));
-		nv_ = nv; nf_ = nf; nhe_ = nhe; ne_ = ne; manifold_ = man != 0;
-		vt_.resize(nhe_); ot_.resize(nhe_); evt_.resize(nv_);
-		check(gp_che_tables(h_, vt_.data(), ot_.data(), evt_.data()));
+		attach();
+	}
+	// che_off / che_obj / che_ply(file): the format follows the extension.
+	explicit che(const std::string &file)
+	{
+		check(gp_che_read(file.c_str(), &h_));
+		attach();
 	}
 	~che() { gp_che_destroy(h_); }
 	che(const che &) = delete;
@@ -103,10 +105,31 @@ class che {
 	gp_che *handle() const { return h_; }
 
   private:
+	void attach()
+	{
+		int man = 0;
+		uint32_t nv, nf, nhe, ne;
+		check(gp_che_sizes(h_, &nv, &nf, &nhe, &ne, &man));
+		nv_ = nv; nf_ = nf; nhe_ = nhe; ne_ = ne; manifold_ = man != 0;
+		vt_.resize(nhe_); ot_.resize(nhe_); evt_.resize(nv_);
+		check(gp_che_tables(h_, vt_.data(), ot_.data(), evt_.data()));
+	}
+
 	gp_che *h_ = nullptr;
 	size_t nv_ = 0, nf_ = 0, nhe_ = 0, ne_ = 0;
 	bool manifold_ = true;
 	std::vector<index_t> vt_, ot_, evt_;
+};
+
+// The reference's reader classes (include/che_off.h, che_obj.h, che_ply.h).
+struct che_off : che {
+	explicit che_off(const std::string &file) : che(file) {}
+};
+struct che_obj : che {
+	explicit che_obj(const std::string &file) : che(file) {}
+};
+struct che_ply : che {
+	explicit che_ply(const std::string &file) : che(file) {}
 };
 
 struct ptp_out_t {

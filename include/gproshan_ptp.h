@@ -139,6 +139,16 @@ int gp_che_create(const double *xyz, uint32_t n_vertices, const uint32_t *faces,
 /* che::~che (include/che.h:55; src/che.cpp:89-92,1404-1413). */
 void gp_che_destroy(gp_che *mesh);
 
+/* che_off / che_obj / che_ply (src/che_off.cpp:28-79, src/che_obj.cpp:24-79,
+ * src/che_ply.cpp:25-156): reads a mesh file by extension (.off, .obj, .ply)
+ * and builds it as gp_che_create does.  OFF quads split into (a,b,c),(d,a,c)
+ * and OBJ quads into (a,b,d),(b,c,d), as the reference readers do; PLY is
+ * ascii or binary_little_endian triangles.  Malformed files: GP_ERR_INVALID. */
+int gp_che_read(const char *path, gp_che **out);
+
+/* che::gt over all vertices (include/che.h:82): positions [3*n_vertices]. */
+int gp_che_positions(const gp_che *mesh, double *xyz);
+
 /* che::n_vertices/n_faces/n_half_edges/n_edges/is_manifold (include/che.h:80,93-96). */
 int gp_che_sizes(const gp_che *mesh, uint32_t *n_vertices, uint32_t *n_faces,
                  uint32_t *n_half_edges, uint32_t *n_edges, int *manifold);
