@@ -36,7 +36,7 @@ EXPORTS = [
     "gp_mlp_create", "gp_mlp_destroy", "gp_mlp_params", "gp_mlp_features", "gp_mlp_forward", "gp_mlp_train",
     "gp_mlp_predict", "gp_plan_create_learned",
     "gp_batch_create", "gp_batch_run", "gp_batch_result", "gp_batch_destroy",
-    "gp_farthest_point_sampling_ptp_gpu",
+    "gp_farthest_point_sampling_ptp_gpu", "gp_ptp_iter_error",
 ]
 MLP_NPARAM = 6337
 MLP_NFEAT = 10
@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
     L.gp_batch_result.argtypes = [P, u32, P, u32, C.POINTER(u32)]
     L.gp_batch_destroy.argtypes = [P]
     L.gp_batch_destroy.restype = None
+    L.gp_ptp_iter_error.argtypes = [P, P, u32, P, C.c_int, P, P, u32, C.POINTER(u32), P, u32, C.POINTER(u32)]
     L.gp_farthest_point_sampling_ptp_gpu.argtypes = [P, P, u32, u32, C.c_uint64, C.c_double, C.c_int,
                                                      C.POINTER(u32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L

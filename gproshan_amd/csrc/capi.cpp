@@ -474,4 +474,16 @@ int gp_farthest_point_sampling_ptp_gpu(gp_che *mesh, uint32_t *samples, uint32_t
 	)
 }
 
+int gp_ptp_iter_error(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *exact,
+                      int precision, uint32_t *iter, double *err, uint32_t cap, uint32_t *n_err,
+                      uint32_t *band, uint32_t cap_band, uint32_t *n_band)
+{
+	GP_GUARD(
+		if (!mesh) return gp::fail(GP_ERR_INVALID, "null argument");
+		if (precision != GP_PREC_F32 && precision != GP_PREC_F64) return gp::fail(GP_ERR_INVALID, "bad precision");
+		return gp::ptp_iter_error(&mesh->m, sources, n_sources, exact, precision == GP_PREC_F64, iter, err, cap, n_err,
+		                          band, cap_band, n_band);
+	)
+}
+
 }  // extern "C"

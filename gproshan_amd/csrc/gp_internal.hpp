@@ -139,6 +139,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
 int plan_result(gp_plan *p, double *dist);
 int plan_clusters(gp_plan *p, uint32_t *clusters);
+int ptp_iter_error(Che *mesh, const uint32_t *src, uint32_t ns, const double *exact, bool f64, uint32_t *iter,
+                   double *err, uint32_t cap, uint32_t *n_err, uint32_t *band, uint32_t cap_band, uint32_t *n_band);
 int plan_trace(gp_plan *p, uint32_t *out, uint32_t cap, uint32_t *n);
 
 // Patch / band schedules (patches.hip).  dhat == null: no band gating.

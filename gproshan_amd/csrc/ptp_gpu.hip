@@ -655,6 +655,95 @@ int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const doubl
 	return GP_OK;
 }
 
+// compute_error (src/test_geodesics_ptp.cpp:362-371) on the device: the sum of
+// |d - exact| / exact over exact > 0 (the caller scales by 100 / (n - s)).
+template <typename T>
+__global__ void k_iter_err(const T *d, const double *exact, uint32_t n, double *acc)
+{
+	double e = 0;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+		if (exact[v] > 0) e += fabs((double) d[v] - exact[v]) / exact[v];
+	for (uint32_t o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+	if ((threadIdx.x & 63u) == 0) atomicAdd(acc, e);
+}
+
+int ptp_iter_error(Che *mesh, const uint32_t *src, uint32_t ns, const double *exact, bool f64, uint32_t *iter,
+                   double *err, uint32_t cap, uint32_t *n_err, uint32_t *band, uint32_t cap_band, uint32_t *n_band)
+{
+	if (!mesh || !src || !exact || !n_err || !n_band) return fail(GP_ERR_INVALID, "iter_error: null argument");
+	*n_err = *n_band = 0;
+	const uint32_t nv = mesh->nv;
+	std::vector<uint32_t> lev(nv), sorted(nv), lim;
+	int rc = toplesets_host(*mesh, src, ns, lev.data(), sorted.data(), lim);
+	if (rc) return rc;
+	if (lim.size() < 2 || lim.back() != nv)
+		return fail(GP_ERR_INVALID, "iter_error: every vertex must be reached from the sources");
+	gp_ptp_opts o;
+	gp_ptp_opts_default(&o);
+	o.precision = f64 ? GP_PREC_F64 : GP_PREC_F32;
+	o.coalesce = 1;
+	o.uncapped = 1;
+	o.launch_mode = GP_LAUNCH_STREAM;
+	gp_plan *pp = nullptr;
+	if ((rc = plan_create_rings(mesh, src, ns, lim.data(), (uint32_t) lim.size(), sorted.data(), o, &pp))) return rc;
+	std::unique_ptr<gp_plan> p(pp);
+	// Exact distances in the ring numbering (test_geodesics_ptp_coalescence.cu:37).
+	std::vector<double> exc(nv);
+	for (uint32_t q = 0; q < nv; q++) exc[q] = exact[sorted[q]];
+	double *d_exact = nullptr, *d_acc = nullptr;
+	struct Free {
+		std::vector<void *> v;
+		~Free() { for (void *q : v) (void) hipFree(q); }
+	} fr;
+	if (hipMalloc(&d_exact, 8ull * nv) != hipSuccess || hipMalloc(&d_acc, 8) != hipSuccess) {
+		fr.v = { d_exact, d_acc };
+		return fail(GP_ERR_NOMEM, "iter_error: allocation failed");
+	}
+	fr.v = { d_exact, d_acc };
+	GP_HIP(hipMemcpy(d_exact, exc.data(), 8ull * nv, hipMemcpyHostToDevice));
+	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));
+	const uint32_t done0 = p->nl < 3 ? 1u : 0u;
+	if (f64) {
+		k_init<double><<<ig, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->n, p->d_state, done0);
+		k_sources<double><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->d_src, p->ns);
+	} else {
+		k_init<float><<<ig, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->n, p->d_state, done0);
+		k_sources<float><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((float *) p->d_dist0, (float *) p->d_dist1, p->d_src, p->ns);
+	}
+	GP_HIP(hipGetLastError());
+	const KArgs a = make_args(p.get());
+	uint32_t ne = 0, nb = 0;
+	for (uint32_t k = 0; !done0 && k <= p->max_iter; k++) {
+		enqueue_sweeps(p.get(), a, k, 1);
+		GP_HIP(hipGetLastError());
+		GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipStreamSynchronize(p->stream));
+		if (p->h_state->done) break;
+		// Sweep k + 1 just ran with this state (the uncapped loop has no K/2).
+		const Slot s = p->h_state->slot[k & 3u];
+		if (nb < cap_band && band) {
+			uint32_t *r = band + 4 * (size_t) nb;
+			r[0] = k + 1; r[1] = s.i; r[2] = s.j; r[3] = lim[s.j] - lim[s.i];
+		}
+		nb++;
+		if (s.j == p->nl - 1) {
+			const void *nd = s.d ? p->d_dist0 : p->d_dist1;    // the buffer the sweep wrote
+			GP_HIP(hipMemsetAsync(d_acc, 0, 8, p->stream));
+			if (f64) k_iter_err<double><<<ig, 256, 0, p->stream>>>((const double *) nd, d_exact, nv, d_acc);
+			else k_iter_err<float><<<ig, 256, 0, p->stream>>>((const float *) nd, d_exact, nv, d_acc);
+			GP_HIP(hipGetLastError());
+			double acc = 0;
+			GP_HIP(hipMemcpyAsync(&acc, d_acc, 8, hipMemcpyDeviceToHost, p->stream));
+			GP_HIP(hipStreamSynchronize(p->stream));
+			if (ne < cap && iter && err) { iter[ne] = k + 1; err[ne] = acc * 100.0 / (double) (nv - ns); }
+			ne++;
+		}
+	}
+	*n_err = ne;
+	*n_band = nb;
+	return GP_OK;
+}
+
 }  // namespace gp
 
 extern "C" void gp_plan_destroy(gp_plan *p) { delete p; }

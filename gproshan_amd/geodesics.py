@@ -182,6 +182,21 @@ def farthest_point_sampling_ptp_gpu(mesh, samples, n, radio=0.0, precision=L.PRE
     return buf[:cnt.value].copy(), r.value, secs.value
 
 
+def iter_error_parallel_toplesets_propagation_coalescence_gpu(mesh, sources, exact, precision=L.PREC_F64,
+                                                              cap=1 << 16):
+    """Returns (iter, err %, band rows (sweep, i, j, window)) of the uncapped
+    loop (src/cuda/test_geodesics_ptp_coalescence.cu:23-100, 214-266)."""
+    src = _u32(sources)
+    ex = np.ascontiguousarray(exact, dtype=np.float64)
+    it = np.zeros(cap, np.uint32)
+    er = np.zeros(cap)
+    band = np.zeros((cap, 4), np.uint32)
+    ne, nb = C.c_uint32(), C.c_uint32()
+    check(lib().gp_ptp_iter_error(mesh.handle, ptr(src), len(src), ptr(ex), int(precision), ptr(it), ptr(er), cap,
+                                  C.byref(ne), ptr(band), cap, C.byref(nb)))
+    return it[:min(ne.value, cap)], er[:min(ne.value, cap)], band[:min(nb.value, cap)]
+
+
 def normalize_ptp(dist):
     d = np.ascontiguousarray(dist, dtype=np.float64)
     check(lib().gp_normalize_ptp(ptr(d), len(d)))

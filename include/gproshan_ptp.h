@@ -279,6 +279,19 @@ int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t 
 int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, gp_mlp *mlp,
                            const gp_ptp_opts *opts, gp_plan **out);
 
+/* iter_error_parallel_toplesets_propagation_coalescence_gpu
+ * (src/cuda/test_geodesics_ptp_coalescence.cu:23-100, 214-266; used by
+ * src/test_geodesics_ptp.cpp:196-213): the uncapped loop on the ring-numbered
+ * mesh, one sweep at a time.  band[4 * cap_band] receives (sweep, i, j,
+ * window size) for every sweep (the reference's "band" file); iter[cap] /
+ * err[cap] receive (sweep, compute_error in %) after every sweep run once j is
+ * the last ring (its "_error.iter" file).  exact[n_vertices] in the mesh's
+ * numbering; every vertex must be reached.  *n_err / *n_band: the full counts
+ * (entries beyond the capacities are not written). */
+int gp_ptp_iter_error(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *exact,
+                      int precision, uint32_t *iter, double *err, uint32_t cap, uint32_t *n_err,
+                      uint32_t *band, uint32_t cap_band, uint32_t *n_band);
+
 /* ---- farthest-point sampling ------------------------------------------- */
 
 /* farthest_point_sampling_ptp_gpu (src/cuda/geodesics_ptp.cu:87-172,

@@ -134,6 +134,16 @@ int or_ptp_fixed_point(const or_che *m, const uint32_t *src, uint32_t ns,
 int or_fps(const or_che *m, uint32_t *samples, uint32_t n_init, uint32_t cap, uint64_t n, double radio,
            int f32, uint32_t *n_out, double *radio_out);
 
+/* iter_error_parallel_toplesets_propagation_coalescence_gpu
+ * (src/cuda/test_geodesics_ptp_coalescence.cu:23-100, 214-266): the uncapped
+ * loop on the ring-numbered mesh (every vertex must be reached); band[4*k] =
+ * (sweep, i, j, end - start) for every sweep; (iter[k], err[k]) = (sweep,
+ * compute_error in %) after every sweep run once j is the last ring.  The
+ * counts are returned even when they exceed the capacities. */
+int or_iter_error(const or_che *m, const uint32_t *src, uint32_t ns, const double *exact, int f32,
+                  uint32_t *iter, double *err, uint32_t cap, uint32_t *n_err,
+                  uint32_t *band, uint32_t cap_band, uint32_t *n_band);
+
 /* compute_error (src/test_geodesics_ptp.cpp:362-371): mean relative error
  * in percent over n - s vertices. */
 double or_compute_error(const double *dist, const double *exact, uint32_t n, uint32_t s);

@@ -66,6 +66,8 @@ def lib():
         L.or_ptp_fixed_point.argtypes = [C.POINTER(_Che), P, u32, C.c_double, P, C.POINTER(C.c_uint64)]
         L.or_ptp_clusters.argtypes = [C.POINTER(_Che), P, u32, P, u32, P, C.c_int, C.c_double, C.c_int, P, P,
                                       C.POINTER(u32)]
+        L.or_iter_error.argtypes = [C.POINTER(_Che), P, u32, P, C.c_int, P, P, u32, C.POINTER(u32), P, u32,
+                                    C.POINTER(u32)]
         L.or_fps.argtypes = [C.POINTER(_Che), P, u32, u32, C.c_uint64, C.c_double, C.c_int, C.POINTER(u32),
                              C.POINTER(C.c_double)]
         L.or_compute_error.argtypes = [P, P, u32, u32]
@@ -167,6 +169,21 @@ class Mesh:
         if rc != 0:
             raise ValueError(f"or_ptp_clusters failed: {rc}")
         return dist, cl, nsw.value
+
+    def iter_error(self, sources, exact, f32=False, cap=1 << 16):
+        """((iter, err) pairs, band rows (sweep, i, j, window)) of the uncapped
+        loop on the ring-numbered mesh."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        ex = np.ascontiguousarray(exact, dtype=np.float64)
+        it = np.zeros(cap, np.uint32)
+        er = np.zeros(cap, np.float64)
+        band = np.zeros((cap, 4), np.uint32)
+        ne, nb = C.c_uint32(0), C.c_uint32(0)
+        rc = lib().or_iter_error(C.byref(self._c), _p(src), len(src), _p(ex), int(f32), _p(it), _p(er), cap,
+                                 C.byref(ne), _p(band), cap, C.byref(nb))
+        if rc != 0:
+            raise ValueError(f"or_iter_error failed: {rc}")
+        return it[:min(ne.value, cap)], er[:min(ne.value, cap)], band[:min(nb.value, cap)]
 
     def fps(self, samples, n, radio=0.0, f32=False):
         """farthest_point_sampling_ptp_gpu: (samples, radio)."""

@@ -169,6 +169,55 @@ out:
 	return rc;
 }
 
+int or_iter_error(const or_che *m, const uint32_t *src, uint32_t ns, const double *exact, int f32,
+                  uint32_t *iter, double *err, uint32_t cap, uint32_t *n_err,
+                  uint32_t *band, uint32_t cap_band, uint32_t *n_band)
+{
+	const uint32_t nv = m->nv;
+	*n_err = *n_band = 0;
+	uint32_t *levels = (uint32_t *) malloc(sizeof(uint32_t) * nv);
+	uint32_t *sorted = (uint32_t *) malloc(sizeof(uint32_t) * nv);
+	uint32_t *limits = (uint32_t *) malloc(sizeof(uint32_t) * ((size_t) nv + 2));
+	uint32_t *inv = (uint32_t *) malloc(sizeof(uint32_t) * nv);
+	uint32_t *srcc = (uint32_t *) malloc(sizeof(uint32_t) * (ns ? ns : 1));
+	double *exc = (double *) malloc(sizeof(double) * nv);
+	double *V = (double *) malloc(sizeof(double) * 3 * (size_t) nv);
+	uint32_t *F = (uint32_t *) malloc(sizeof(uint32_t) * (m->nhe ? m->nhe : 1));
+	float *gt32 = NULL;
+	or_che cm;
+	int built = 0, rc = OR_OK;
+	uint32_t nl = 0;
+	if (!levels || !sorted || !limits || !inv || !srcc || !exc || !V || !F) { rc = OR_ENOMEM; goto out; }
+	rc = or_toplesets(m, src, ns, OR_NIL, levels, sorted, limits, nv + 2, &nl);
+	if (rc) goto out;
+	if (limits[nl - 1] != nv) { rc = OR_EINVAL; goto out; }   /* the reference renumbers every vertex */
+	/* Coalesced copy (src/cuda/test_geodesics_ptp_coalescence.cu:27-44). */
+	for (uint32_t q = 0; q < nv; q++) {
+		for (int k = 0; k < 3; k++) V[3 * (size_t) q + k] = m->gt[3 * (size_t) sorted[q] + k];
+		inv[sorted[q]] = q;
+		exc[q] = exact[sorted[q]];
+	}
+	for (uint32_t he = 0; he < m->nhe; he++) F[he] = inv[m->vt[he]];
+	rc = or_che_build(V, nv, F, m->nhe / 3, &cm);
+	if (rc) goto out;
+	built = 1;
+	for (uint32_t q = 0; q < ns; q++) srcc[q] = inv[src[q]];
+	if (f32) {
+		gt32 = (float *) malloc(sizeof(float) * 3 * (size_t) nv);
+		if (!gt32) { rc = OR_ENOMEM; goto out; }
+		for (size_t q = 0; q < 3 * (size_t) nv; q++) gt32[q] = (float) cm.gt[q];
+		mview_f32 mv = { nv, gt32, cm.vt, cm.ot, cm.evt };
+		rc = iter_error_f32(&mv, srcc, ns, limits, nl, exc, 1e-3, iter, err, cap, n_err, band, cap_band, n_band);
+	} else {
+		mview_f64 mv = { nv, cm.gt, cm.vt, cm.ot, cm.evt };
+		rc = iter_error_f64(&mv, srcc, ns, limits, nl, exc, 1e-3, iter, err, cap, n_err, band, cap_band, n_band);
+	}
+out:
+	if (built) or_che_free(&cm);
+	free(levels); free(sorted); free(limits); free(inv); free(srcc); free(exc); free(V); free(F); free(gt32);
+	return rc;
+}
+
 double or_compute_error(const double *dist, const double *exact, uint32_t n, uint32_t s)
 {
 	double e = 0;

@@ -287,3 +287,48 @@ static int SFX(ptp)(const or_che *mesh, const uint32_t *src, uint32_t ns,
 	if (inv) { free(inv); or_che_free(&cm); }
 	return OR_OK;
 }
+
+/* iter_error_run_ptp_coalescence_gpu (src/cuda/test_geodesics_ptp_coalescence.cu:214-266)
+ * on an already coalesced mesh (ring-numbered, exact distances in that
+ * numbering): the uncapped loop; every sweep appends (sweep, i, j, window
+ * size) to band; after every sweep run once j has reached the last ring,
+ * (sweep, compute_error of the buffer it wrote) goes to iter/err.  Pinned with
+ * the schedule (the loop is the uncapped one of SFX(schedule)). */
+static int SFX(iter_error)(const SFX(mview) *m, const uint32_t *src_c, uint32_t ns, const uint32_t *limits,
+                           uint32_t nl, const double *exact_c, double tol,
+                           uint32_t *iter, double *errv, uint32_t cap, uint32_t *n_err,
+                           uint32_t *band, uint32_t cap_band, uint32_t *n_band)
+{
+	const uint32_t n = m->nv;
+	REAL *pd[2] = { (REAL *) malloc(sizeof(REAL) * n), (REAL *) malloc(sizeof(REAL) * n) };
+	if (!pd[0] || !pd[1]) { free(pd[0]); free(pd[1]); return OR_ENOMEM; }
+	for (uint32_t v = 0; v < n; v++) pd[0][v] = pd[1][v] = (REAL) INFINITY;
+	for (uint32_t q = 0; q < ns; q++) pd[0][src_c[q]] = pd[1][src_c[q]] = 0;
+	uint32_t d = 0, i = 1, j = 2, it = 0, ne = 0, nb = 0;
+	while (nl >= 3 && i < j) {
+		it++;
+		const uint32_t start = limits[i], end = limits[j], ncond = limits[i + 1] - start;
+		const REAL *od = pd[d];
+		REAL *nd = pd[!d];
+		for (uint32_t v = start; v < end; v++) nd[v] = SFX(relax)(m, od, v);
+		if (nb < cap_band) { band[4 * nb] = it; band[4 * nb + 1] = i; band[4 * nb + 2] = j; band[4 * nb + 3] = end - start; }
+		nb++;
+		if (j == nl - 1) {
+			double e = 0;
+			for (uint32_t v = 0; v < n; v++)
+				if (exact_c[v] > 0) e += fabs((double) nd[v] - exact_c[v]) / exact_c[v];
+			if (ne < cap) { iter[ne] = it; errv[ne] = e * 100 / (double) (n - ns); }
+			ne++;
+		}
+		uint32_t count = 0;
+		for (uint32_t v = start; v < start + ncond; v++) count += (double) (FABS(nd[v] - od[v]) / od[v]) < tol;
+		if (count == ncond) i++;
+		if (j < nl - 1) j++;
+		d = !d;
+	}
+	*n_err = ne;
+	*n_band = nb;
+	free(pd[0]); free(pd[1]);
+	return OR_OK;
+}
+
