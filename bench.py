@@ -40,11 +40,11 @@ B_RELAX_F64 = 92
 PEAK_HBM_GBS = 8000.0
 ERR_BAR = 1e-4
 # HBM bytes per dispatch of the headline kernel on the default workload, from
-# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, 1,176
+# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, 1,184
 # dispatches of one band_learned run) scaled by what the same counters read for
 # a 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
 # profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources).
-MEASURED_TRAFFIC = {("band_learned", 1024, 1): (43687.9 * 2.0 + 1516.6 * 1.0) * 1024}
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): (35396.7 * 2.0 + 1506.4 * 1.0) * 1024}
 
 
 def log(*a):

@@ -180,7 +180,7 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W)
 {
-	constexpr double SIN2_MIN = 0.03, COS_MIN = -0.9;
+	constexpr double SIN2_MIN = 0.01, COS_MIN = -0.97;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
 	if (W != 64 && W != 1) return fail(GP_ERR_INVALID, "layout: chunk width must be 64 or 1");
 	L.nchunks = (nr + W - 1) / W;
