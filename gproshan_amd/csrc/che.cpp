@@ -178,7 +178,8 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	return GP_OK;
 }
 
-int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W)
+int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W,
+                 uint32_t uniform_max)
 {
 	constexpr double SIN2_MIN = 0.01, COS_MIN = -0.97;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
@@ -198,6 +199,11 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		}
 		L.coff[c + 1] = L.coff[c] + S;
 		L.max_slots = std::max(L.max_slots, S);
+	}
+	L.uniform = 0;
+	if (uniform_max && W == 64 && L.max_slots <= uniform_max) {
+		for (uint32_t c = 0; c <= L.nchunks; c++) L.coff[c] = c * L.max_slots;
+		L.uniform = L.max_slots;
 	}
 	const size_t lanes = (size_t) L.coff[L.nchunks] * W;
 	L.cid.assign(std::max<size_t>(lanes, 1), 0);

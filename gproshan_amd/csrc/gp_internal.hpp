@@ -84,6 +84,7 @@ constexpr uint16_t LOCAL_OUT = 0xFFFF;
 constexpr uint32_t ID30_MASK = 0x3FFFFFFFu;
 struct Layout {
 	uint32_t nchunks = 0, max_slots = 0;
+	uint32_t uniform = 0;                  // every chunk has this many slots (0: see coff)
 	unsigned long long n_flagged = 0;
 	std::vector<uint32_t> coff;
 	std::vector<uint32_t> cid;
@@ -97,8 +98,12 @@ struct Layout {
 // W = 64: chunks of 64 positions, slot k of lane l at (coff[c] + k) * 64 + l
 // (one wave per chunk, coalesced); W = 1: vertex-major, slot k of position p
 // at coff[p] + k (one thread walks a vertex's slots in a few cache lines).
+// uniform_max > 0 (W = 64 only): when no chunk needs more than uniform_max
+// slots, every chunk gets the longest chunk's slot count, so chunk c starts at
+// row c * max_slots and a kernel needs no coff read to find it (the extra
+// slots are empty).
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile = 0,
-                 uint32_t W = 64);
+                 uint32_t W = 64, uint32_t uniform_max = 0);
 
 // All vertices in Morton order of their positions, one scale for the three
 // axes (a flat axis takes few bits), so consecutive vertices are close.

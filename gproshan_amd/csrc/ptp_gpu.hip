@@ -103,8 +103,15 @@ int upload_mesh(Che &m)
 // No workgroup reads a slot or counter while another workgroup of the same
 // launch writes it, so kernel boundaries are the only synchronisation.  The
 // captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
+//
+// A slot also carries its sweep's window in positions (ws, wc, we: start, end
+// of the convergence-tested ring, end) and, written at the end of the sweep,
+// the two windows the next sweep can have (ring i kept or retired: cs/cc[0|1],
+// end ce), so that the next sweep knows its window from the slot alone,
+// without a dependent read of limits.
 struct Slot {
-	uint32_t i, j, d, iter, done, fresh, pad0, pad1;
+	uint32_t i, j, d, iter, done, fresh, ws, wc;
+	uint32_t we, cs0, cc0, cs1, cc1, ce, pad0, pad1;
 };
 struct PState {
 	Slot slot[4];
@@ -116,6 +123,7 @@ struct PState {
 struct KArgs {
 	const uint32_t *limits;
 	const uint32_t *coff;      // per chunk: first slot row
+	uint32_t sb;               // > 0: every chunk has sb slots (chunk c at row c * sb; coff unused)
 	const uint32_t *cid;
 	const float4 *cgeo;
 	const double4 *cgeo64;
@@ -132,7 +140,7 @@ struct KArgs {
 	uint32_t *cl0, *cl1;       // cluster labels (double-buffered like the distances), or null
 };
 
-// Relaxes the window [start, end) (positions); returns this thread's count of
+// Relaxes the window [start, end) (positions); returns the wave's count of
 // non-converged vertices among positions [start, cend).
 template <bool F64, bool CL>
 __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
@@ -158,7 +166,8 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 	uint32_t bad = 0;
 	for (uint32_t c = lo + lw; c < hi; c += nw_x) {
 		const uint32_t pos = (c << 6) + lane;
-		const uint32_t cb = a.coff[c], S = a.coff[c + 1] - cb;
+		const uint32_t cb = a.sb ? c * a.sb : a.coff[c], S = a.sb ? a.sb : a.coff[c + 1] - cb;
+		bool lbad = false;
 		if (pos >= start && pos < end) {
 			const uint32_t v = a.order ? a.order[pos] : pos;
 			const T old = od[v];
@@ -180,16 +189,11 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 				}
 				ncl[v] = c;
 			}
-			if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+			if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
 		}
+		bad += (uint32_t) __popcll(__ballot(lbad));
 	}
 	return bad;
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
-{
-	for (uint32_t o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-	return x;
 }
 
 // Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
@@ -212,14 +216,18 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 		// The previous sweep's outcome: the reference loop tail.
 		const uint32_t nbad = st->bad[kp];
 		const uint32_t pi = a.k2 && prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
-		cur.i = pi + (nbad == 0 ? 1u : 0u);
+		const bool adv = nbad == 0;
+		cur.i = pi + (adv ? 1u : 0u);
 		cur.j = prev.j + (prev.j < a.nl - 1 ? 1u : 0u);
 		cur.d = prev.d ^ 1u;
 		cur.iter = prev.iter + 1;
 		cur.done = !(cur.i < cur.j) || !(cur.iter < a.max_iter);
+		cur.ws = adv ? prev.cs1 : prev.cs0;
+		cur.wc = adv ? prev.cc1 : prev.cc0;
+		cur.we = prev.ce;
 		if (lead) {
-			const uint32_t l0 = a.limits[pi], l1 = a.limits[pi + 1], lj = a.limits[prev.j];
-			const uint32_t ncond = l1 - l0;
+			const uint32_t l0 = prev.ws, lj = prev.we;
+			const uint32_t ncond = prev.wc - prev.ws;
 			if (a.trace) {
 				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
 				r[0] = prev.iter + 1; r[1] = pi; r[2] = prev.j; r[3] = l0; r[4] = lj; r[5] = ncond;
@@ -231,18 +239,36 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 			st->sweeps = cur.iter;
 		}
 	}
+	const uint32_t i = a.k2 && cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
+	if (prev.fresh && !cur.done) {
+		cur.ws = a.limits[i]; cur.wc = a.limits[i + 1]; cur.we = a.limits[cur.j];
+	}
+	// The next sweep's two possible windows (read here, stored at the end).
+	uint32_t nx[5] = { 0, 0, 0, 0, 0 };
+	if (lead && !cur.done) {
+		const uint32_t jn = cur.j + (cur.j < a.nl - 1 ? 1u : 0u);
+		for (uint32_t adv = 0; adv < 2; adv++) {
+			const uint32_t in = i + adv;
+			const uint32_t iw = a.k2 && in < (jn >> 1) ? (jn >> 1) : in;
+			if (in < jn) { nx[2 * adv] = a.limits[iw]; nx[2 * adv + 1] = a.limits[iw + 1]; }
+		}
+		nx[4] = a.limits[jn];
+	}
 	if (lead) {
 		st->slot[k4] = cur;
 		st->bad[(k4 + 1u) & 3u] = 0;
 		if (cur.done) { st->final_d = cur.d; st->done = 1; }
 	}
 	if (cur.done) return;
-	const uint32_t i = a.k2 && cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
-	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
+	const uint32_t start = cur.ws, cend = cur.wc, end = cur.we;
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = wave_sum(sweep_body<F64, CL>(a, start, end, cend, od, nd));
+	const uint32_t bad = sweep_body<F64, CL>(a, start, end, cend, od, nd);
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
+	if (lead) {
+		Slot *sl = &st->slot[k4];
+		sl->cs0 = nx[0]; sl->cc0 = nx[1]; sl->cs1 = nx[2]; sl->cc1 = nx[3]; sl->ce = nx[4];
+	}
 }
 
 template <typename T>
@@ -297,7 +323,7 @@ struct gp_plan {
 	gp::DeviceMesh *dm = nullptr;
 	gp_ptp_opts opts{};
 	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
-	uint32_t nchunks = 0, max_slots = 0;
+	uint32_t nchunks = 0, max_slots = 0, sb = 0;
 	unsigned long long n_flagged = 0;
 	gp::PatchPlan *pp = nullptr;             // patch / band schedules
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
@@ -386,10 +412,11 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	const bool f64 = opts.precision == GP_PREC_F64;
 	if (opts.schedule != GP_SCHED_RINGS) return fail(GP_ERR_INVALID, "plan_create_rings: ring schedule only");
 	Layout L;
-	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L);
+	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, 0, 64, MAXS);
 	if (rc) return rc;
 	p->nchunks = L.nchunks;
 	p->max_slots = L.max_slots;
+	p->sb = L.uniform;
 	p->n_flagged = L.n_flagged;
 	// The drivers cap the loop at 2*|limits| sweeps (src/geodesics_ptp.cpp:66-67);
 	// the uncapped loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248)
@@ -450,7 +477,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 static KArgs make_args(const gp_plan *p)
 {
 	KArgs a{};
-	a.limits = p->d_limits; a.coff = p->d_coff; a.cid = p->d_cid; a.cgeo = p->d_cgeo; a.cgeo64 = p->d_cgeo64;
+	a.limits = p->d_limits; a.coff = p->d_coff; a.sb = p->sb; a.cid = p->d_cid; a.cgeo = p->d_cgeo; a.cgeo64 = p->d_cgeo64;
 	a.cx64 = p->d_cx64; a.fcum = p->d_fcum; a.order = p->d_order; a.dist0 = p->d_dist0; a.dist1 = p->d_dist1;
 	a.st = p->d_state; a.trace = p->d_trace;
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;

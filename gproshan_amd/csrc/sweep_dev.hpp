@@ -96,18 +96,21 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 		T fa = (T) INFINITY, fb = (T) INFINITY;
 #pragma unroll
 		for (uint32_t k = 0; k + 1 < MAXS; k++) {
-			if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) {
-				if constexpr (F64) {
-					take(tri_f64(g[k], g[k + 1], t[k], t[k + 1]), k);
-				} else {
-					if (id[k] & F64_FLAG) {
-						if (k == k0) { fa = t[k]; fb = t[k + 1]; }
-						else fmask |= 1u << k;
-					} else {
-						take(update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1], a.h,
-						                          a.inv_h), k);
-					}
-				}
+			if constexpr (F64) {
+				if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) take(tri_f64(g[k], g[k + 1], t[k], t[k + 1]), k);
+			} else if (k + 1 < S) {
+				// Straight-line per triangle: every lane evaluates the FP32
+				// update and keeps it only for a real, unflagged triangle;
+				// flagged ones are routed to the FP64 update by selects.
+				const bool real = !(id[k + 1] & SLOT_EMPTY);
+				const bool fl = (id[k] & F64_FLAG) != 0u;
+				const float p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1], a.h,
+				                                     a.inv_h);
+				take((real && !fl) ? p : (T) INFINITY, k);
+				const bool first = real && fl && k == k0;
+				fa = first ? t[k] : fa;
+				fb = first ? t[k + 1] : fb;
+				fmask |= (real && fl && k != k0) ? (1u << k) : 0u;
 			}
 		}
 		if constexpr (!F64) {

@@ -109,15 +109,20 @@ GP_HD float stable_rcp(float x)
 }
 GP_HD double stable_rcp(double x) { return 1.0 / x; }
 
+// Branch-free: both the planar solution and the edge paths are evaluated and
+// the result is selected, so the unrolled fan loops of the kernels run as
+// straight-line code whose triangles the compiler can interleave.  The
+// selection is the one of the branching form: the edge paths when an input is
+// infinite, the discriminant is negative or the front does not arrive from
+// inside the wedge (the planar terms computed from an infinite input are
+// discarded by the same test).
 template <typename T>
 GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 {
 	const T inf = (T) INFINITY;
-	if (!(t0 != inf && t1 != inf)) {
-		const T dp0 = t0 + stable_sqrt(q0) * h;
-		const T dp1 = t1 + stable_sqrt(q1) * h;
-		return (dp1 < dp0) ? dp1 : dp0;
-	}
+	const T dp0 = t0 + stable_sqrt(q0) * h;
+	const T dp1 = t1 + stable_sqrt(q1) * h;
+	const T edge = (dp1 < dp0) ? dp1 : dp0;
 	const T m = t0 < t1 ? t0 : t1;
 	const T u0 = (t0 - m) * inv_h;
 	const T u1 = (t1 - m) * inv_h;
@@ -127,12 +132,8 @@ GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 	const T pp = (D + stable_sqrt(disc)) * stable_rcp(S);
 	const T c0 = q1 * (u0 - pp) - g * (u1 - pp);
 	const T c1 = q0 * (u1 - pp) - g * (u0 - pp);
-	if (disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0) {
-		const T dp0 = t0 + stable_sqrt(q0) * h;
-		const T dp1 = t1 + stable_sqrt(q1) * h;
-		return (dp1 < dp0) ? dp1 : dp0;
-	}
-	return m + pp * h;
+	const bool planar = (t0 != inf && t1 != inf) && !(disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0);
+	return planar ? m + pp * h : edge;
 }
 
 }  // namespace gp
