@@ -279,7 +279,7 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 	b->tabs.resize(nmeshes);
 	for (uint32_t k = 0; k < nmeshes; k++) {
 		const Che &m0 = *meshes[k];
-		if (m0.nv >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "batch: mesh %u has too many vertices", k);
+		if (m0.nv >= SLOT_WRAP) return fail(GP_ERR_CAPACITY, "batch: mesh %u has too many vertices", k);
 		for (uint32_t v = 0; v < m0.nv; v++)
 			if (m0.fan_off[v + 1] - m0.fan_off[v] > MAX_LINK)
 				return fail(GP_ERR_CAPACITY, "batch: mesh %u vertex %u: link over %u", k, v, MAX_LINK);
@@ -293,7 +293,7 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 		m.mean_edge = m0.mean_edge;      // the same h as the mesh itself
 		if ((rc = upload_mesh(m))) return rc;
 		Layout L;
-		if ((rc = build_layout(m, nullptr, m.nv, false, L, 0, 1))) return rc;
+		if ((rc = build_layout(m, nullptr, m.nv, false, L, 0, 1, 0, true))) return rc;
 		if ((rc = up(t.coff, L.coff.data(), L.coff.size())) || (rc = up(t.cid, L.cid.data(), L.cid.size())) ||
 		    (rc = up(t.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
 		    (rc = up(t.cgeo64, reinterpret_cast<const double4 *>(L.cgeo64.data()), L.cgeo64.size())) ||

@@ -179,11 +179,17 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 }
 
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W,
-                 uint32_t uniform_max)
+                 uint32_t uniform_max, bool wrap)
 {
 	constexpr double SIN2_MIN = 0.01, COS_MIN = -0.97;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
 	if (W != 64 && W != 1) return fail(GP_ERR_INVALID, "layout: chunk width must be 64 or 1");
+	// Slots of a vertex: its fan entries, less the repeated first entry of a
+	// closed fan when wrapping.
+	auto slots = [&](uint32_t v) {
+		const uint32_t n = m.fan_off[v + 1] - m.fan_off[v];
+		return (wrap && n && m.fan_closed[v]) ? n - 1 : n;
+	};
 	L.nchunks = (nr + W - 1) / W;
 	L.coff.assign((size_t) L.nchunks + 1, 0);
 	L.fcum.assign((size_t) nr + 1, 0);
@@ -194,8 +200,8 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		for (uint32_t pos = W * c; pos < std::min(nr, W * c + W); pos++) {
 			const uint32_t v = order ? order[pos] : pos;
 			if (v >= m.nv) return fail(GP_ERR_INVALID, "layout: vertex %u out of range", v);
-			if (v >= SLOT_EMPTY) return fail(GP_ERR_CAPACITY, "layout: vertex ids must stay below 2^30");
-			S = std::max(S, m.fan_off[v + 1] - m.fan_off[v]);
+			if (v >= SLOT_WRAP) return fail(GP_ERR_CAPACITY, "layout: vertex ids must stay below 2^29");
+			S = std::max(S, slots(v));
 		}
 		L.coff[c + 1] = L.coff[c] + S;
 		L.max_slots = std::max(L.max_slots, S);
@@ -223,12 +229,12 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		const uint32_t S = L.coff[c + 1] - L.coff[c];
 		const size_t base = (size_t) L.coff[c] * W + lane;
 		const uint32_t v = order ? order[pos] : pos;
-		const uint32_t s0 = m.fan_off[v], n = m.fan_off[v + 1] - s0;
+		const uint32_t s0 = m.fan_off[v], n = m.fan_off[v + 1] - s0, ns = slots(v);
 		const double *pv = &m.gt[3 * (size_t) v];
 		uint32_t nflag = 0;
 		for (uint32_t k = 0; k < S; k++) {
 			const size_t at = base + W * (size_t) k;
-			if (k >= n) { L.cid[at] = v | SLOT_EMPTY; continue; }
+			if (k >= ns) { L.cid[at] = v | SLOT_EMPTY; continue; }
 			const uint32_t a = m.fan[s0 + k];
 			const double *pa = &m.gt[3 * (size_t) a];
 			// Same subtraction as the reference's vertex::operator-.
@@ -258,6 +264,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w };
 				L.cgeo64[at] = gd;
 			}
+			if (k == 0 && ns < n) id |= SLOT_WRAP;
 			L.cid[at] = id;
 		}
 		flagged_at[pos] = nflag;
@@ -276,7 +283,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 				const size_t at = (size_t) (L.coff[c] + k) * W + lane;
 				const uint32_t id = L.cid[at];
 				if (id & SLOT_EMPTY) continue;
-				const uint32_t u = id & ID30_MASK;
+				const uint32_t u = id & SLOT_ID_MASK;
 				if (u / tile == pos / tile) L.lidx[at] = (uint16_t) (u % tile);
 			}
 		}

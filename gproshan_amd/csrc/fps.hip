@@ -242,7 +242,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	if (rc) return rc;
 	FpsRun r;
 	Layout L;
-	if ((rc = build_layout(m, nullptr, nv, F64, L, 0, 1))) return rc;
+	if ((rc = build_layout(m, nullptr, nv, F64, L, 0, 1, 0, true))) return rc;
 	auto as_d4 = [](const std::vector<Geo64> &v) {
 		std::vector<double4> o(v.size());
 		std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());

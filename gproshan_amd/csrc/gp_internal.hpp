@@ -56,7 +56,9 @@ constexpr uint32_t F64_FLAG = 0x80000000u;
 
 // FP32 geometry of fan entry k of a vertex, in units of the mean edge length h:
 // x = |X_k|^2, y = X_k.X_{k+1}, z = |X_k x X_{k+1}|^2, w = |X_{k+1} - X_k|^2
-// (y, z, w describe triangle (k, k+1) and are 0 on the last entry).
+// (y, z, w describe triangle (k, k+1) and are 0 on the last entry of an open
+// fan; entry k + 1 is entry 0 after the last entry of a closed fan stored
+// without its repeat).
 struct Geo32 { float x, y, z, w; };
 struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
 
@@ -75,13 +77,16 @@ struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
 // vertex.  Chunk c has S_c = coff[c+1] - coff[c] slots, the longest fan in it;
 // slot k of lane l lives at (coff[c] + k) * 64 + l, so every slot load of a
 // wave is one contiguous 256-byte (or 1 KiB) access.  Slot k holds fan entry
-// k: cid = its vertex id, bit 31 = FP64 update for the triangle (k, k+1),
-// bit 30 = empty slot (id = the vertex itself, never a triangle).
+// k: cid = its vertex id, bit 31 = FP64 update for the triangle of slot k,
+// bit 30 = empty slot (id = the vertex itself, never a triangle), bit 29 on
+// slot 0 = the fan is closed and stored without its repeated first entry: the
+// triangle of the last real slot joins it to slot 0 (layouts built with wrap).
 // FP32 solves keep cgeo/cgeo64 (Geo32/Geo64 of entry k); FP64 solves keep
 // cx64 = the FP64 edge vector a - v (x, y, z, 0).
 constexpr uint32_t SLOT_EMPTY = 0x40000000u;
+constexpr uint32_t SLOT_WRAP = 0x20000000u;    // also the vertex count limit of a layout
 constexpr uint16_t LOCAL_OUT = 0xFFFF;
-constexpr uint32_t ID30_MASK = 0x3FFFFFFFu;
+constexpr uint32_t SLOT_ID_MASK = 0x1FFFFFFFu;
 struct Layout {
 	uint32_t nchunks = 0, max_slots = 0;
 	uint32_t uniform = 0;                  // every chunk has this many slots (0: see coff)
@@ -101,9 +106,11 @@ struct Layout {
 // uniform_max > 0 (W = 64 only): when no chunk needs more than uniform_max
 // slots, every chunk gets the longest chunk's slot count, so chunk c starts at
 // row c * max_slots and a kernel needs no coff read to find it (the extra
-// slots are empty).
+// slots are empty).  wrap: closed fans drop their repeated first entry (one
+// slot less per interior vertex) and mark slot 0 with SLOT_WRAP; the kernels
+// of relax_lane read both forms, the patch kernels only the plain one.
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile = 0,
-                 uint32_t W = 64, uint32_t uniform_max = 0);
+                 uint32_t W = 64, uint32_t uniform_max = 0, bool wrap = false);
 
 // All vertices in Morton order of their positions, one scale for the three
 // axes (a flat axis takes few bits), so consecutive vertices are close.

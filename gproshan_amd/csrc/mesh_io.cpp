@@ -43,7 +43,7 @@ static int read_off(std::istream &is, std::vector<double> &xyz, std::vector<uint
 	if (!(is >> head) || head.find("OFF") == std::string::npos)
 		return fail(GP_ERR_INVALID, "off: missing OFF header");
 	if (!(is >> nv >> nf >> ne)) return fail(GP_ERR_INVALID, "off: missing counts");
-	if (nv >= SLOT_EMPTY || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "off: counts too large");
+	if (nv >= SLOT_WRAP || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "off: counts too large");
 	const bool colour = head[0] == 'C', normal = head[0] == 'N';
 	xyz.resize(3 * nv);
 	for (uint64_t v = 0; v < nv; v++) {
@@ -167,7 +167,7 @@ static int read_ply(std::istream &is, std::vector<double> &xyz, std::vector<uint
 	}
 	if (!end) return fail(GP_ERR_INVALID, "ply: no end_header");
 	if (vtypes.size() < 3 || !ply_float(vtypes[0])) return fail(GP_ERR_INVALID, "ply: x y z must be the first vertex properties");
-	if (nv >= SLOT_EMPTY || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "ply: counts too large");
+	if (nv >= SLOT_WRAP || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "ply: counts too large");
 	if (nf && (!ply_size(cnt_t) || !ply_size(idx_t))) return fail(GP_ERR_INVALID, "ply: face list types");
 	xyz.resize(3 * nv);
 	faces.clear();

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 				const uint32_t li = a.lidx[base + 64 * k];
 				pk[k] = li | ((id & SLOT_EMPTY) ? PK_EMPTY : 0u) | ((id & F64_FLAG) ? PK_F64 : 0u);
 				g[k] = a.cgeo[base + 64 * k];
-				if (li == LOCAL_OUT && !(id & SLOT_EMPTY)) tout[k] = a.dist[id & ID30_MASK];
+				if (li == LOCAL_OUT && !(id & SLOT_EMPTY)) tout[k] = a.dist[id & SLOT_ID_MASK];
 				if ((id & F64_FLAG) && nf < FCACHE) {
 					fks = (fks & ~(0xFu << (4 * nf))) | (k << (4 * nf));
 					nf++;
@@ -237,9 +237,9 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 					fmore &= fmore - 1u;
 					const size_t at = base + 64 * (size_t) k;
 					const uint32_t la = a.lidx[at], lb = a.lidx[at + 64];
-					const float tka = la != LOCAL_OUT ? sd[b][la] : a.dist[a.cid[at] & ID30_MASK];
-					const float tkb = lb != LOCAL_OUT ? sd[b][lb] : a.dist[a.cid[at + 64] & ID30_MASK];
-					const float q = tri_flagged(a, at, tka, tkb);
+					const float tka = la != LOCAL_OUT ? sd[b][la] : a.dist[a.cid[at] & SLOT_ID_MASK];
+					const float tkb = lb != LOCAL_OUT ? sd[b][lb] : a.dist[a.cid[at + 64] & SLOT_ID_MASK];
+					const float q = tri_flagged(a, at, at + 64, tka, tkb);
 					if (q < best) best = q;
 				}
 			} else if (!fast) {
@@ -247,14 +247,14 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 				uint32_t ida = a.cid[base];
 				uint32_t lia = a.lidx[base];
 				float4 ga = a.cgeo[base];
-				float ta = lia != LOCAL_OUT ? sd[b][lia] : a.dist[ida & ID30_MASK];
+				float ta = lia != LOCAL_OUT ? sd[b][lia] : a.dist[ida & SLOT_ID_MASK];
 				for (uint32_t k = 1; k < S; k++) {
 					const uint32_t idb = a.cid[base + 64 * k];
 					if (idb & SLOT_EMPTY) break;
 					const uint32_t lib = a.lidx[base + 64 * k];
 					const float4 gb = a.cgeo[base + 64 * k];
-					const float tb = lib != LOCAL_OUT ? sd[b][lib] : a.dist[idb & ID30_MASK];
-					const float q = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), ta, tb)
+					const float tb = lib != LOCAL_OUT ? sd[b][lib] : a.dist[idb & SLOT_ID_MASK];
+					const float q = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), base + 64 * k, ta, tb)
 					                                 : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, h, inv_h);
 					if (q < best) best = q;
 					ida = idb; ga = gb; ta = tb;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 			for (uint32_t k = 0; k < S; k++) {
 				const uint32_t idk = a.cid[base + 64 * k];
 				if ((idk & SLOT_EMPTY) || a.lidx[base + 64 * k] != LOCAL_OUT) continue;
-				const uint32_t q = (idk & ID30_MASK) / P;
+				const uint32_t q = (idk & SLOT_ID_MASK) / P;
 				uint32_t hsh = q & (WAKE_TAB - 1u), probe = 0;
 				for (; probe < WAKE_TAB; probe++) {
 					const uint32_t old = atomicCAS(&s_wake[hsh], 0u, q + 1u);

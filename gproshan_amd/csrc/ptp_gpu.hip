@@ -142,7 +142,7 @@ struct KArgs {
 
 // Relaxes the window [start, end) (positions); returns the wave's count of
 // non-converged vertices among positions [start, cend).
-template <bool F64, bool CL>
+template <bool F64, bool CL, uint32_t MS>
 __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
                                                const typename std::conditional<F64, double, float>::type *od,
                                                typename std::conditional<F64, double, float>::type *nd)
@@ -173,7 +173,7 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 			const T old = od[v];
 			const size_t base = (size_t) cb * 64 + lane;
 			uint32_t bk = NO_TRI;
-			const T best = relax_lane<F64, 64, true, CL>(a, base, S, od, old, &bk);
+			const T best = relax_lane<F64, 64, true, CL, MS>(a, base, S, od, old, &bk);
 			nd[v] = best;
 			if constexpr (CL) {
 				// The label of the nearer of the winning triangle's two other
@@ -183,8 +183,9 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 				uint32_t *ncl = od == (const T *) a.dist0 ? a.cl1 : a.cl0;
 				uint32_t c = ocl[v];
 				if (bk != NO_TRI) {
-					const uint32_t ia = a.cid[base + 64 * (size_t) bk] & ID30_MASK;
-					const uint32_t ib = a.cid[base + 64 * (size_t) (bk + 1)] & ID30_MASK;
+					const uint32_t ia = a.cid[base + 64 * (size_t) bk] & SLOT_ID_MASK;
+					const uint32_t nb = bk + 1 < S ? a.cid[base + 64 * (size_t) (bk + 1)] : SLOT_EMPTY;
+					const uint32_t ib = ((nb & SLOT_EMPTY) ? a.cid[base] : nb) & SLOT_ID_MASK;
 					c = od[ib] < od[ia] ? ocl[ib] : ocl[ia];
 				}
 				ncl[v] = c;
@@ -197,7 +198,7 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 }
 
 // Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
-template <bool F64, bool CL>
+template <bool F64, bool CL, uint32_t MS>
 __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 {
 	using T = typename std::conditional<F64, double, float>::type;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	const uint32_t start = cur.ws, cend = cur.wc, end = cur.we;
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = sweep_body<F64, CL>(a, start, end, cend, od, nd);
+	const uint32_t bad = sweep_body<F64, CL, MS>(a, start, end, cend, od, nd);
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 	if (lead) {
 		Slot *sl = &st->slot[k4];
@@ -324,6 +325,7 @@ struct gp_plan {
 	gp_ptp_opts opts{};
 	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
 	uint32_t nchunks = 0, max_slots = 0, sb = 0;
+	bool ms8 = false;                        // every fan fits 8 slots: the 8-slot kernels
 	unsigned long long n_flagged = 0;
 	gp::PatchPlan *pp = nullptr;             // patch / band schedules
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
@@ -412,7 +414,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	const bool f64 = opts.precision == GP_PREC_F64;
 	if (opts.schedule != GP_SCHED_RINGS) return fail(GP_ERR_INVALID, "plan_create_rings: ring schedule only");
 	Layout L;
-	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, 0, 64, MAXS);
+	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, 0, 64, MAXS, true);
 	if (rc) return rc;
 	p->nchunks = L.nchunks;
 	p->max_slots = L.max_slots;
@@ -428,8 +430,12 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	if (f64) GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<true, false>, 256, 0));
-	else GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_sweep<false, false>, 256, 0));
+	// Fans of up to 8 slots (interior valence up to 8 once closed fans drop
+	// their repeat) run a kernel with 8 register-resident slots.
+	p->ms8 = L.max_slots <= 8;
+	const void *kfn = f64 ? (p->ms8 ? (const void *) k_sweep<true, false, 8> : (const void *) k_sweep<true, false, MAXS>)
+	                      : (p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>);
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, 256, 0));
 	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
 
@@ -493,11 +499,15 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 		const uint32_t k4 = (k0 + q) & 3u;
 		const bool cl = a.cl0 != nullptr;
 		if (p->opts.precision == GP_PREC_F64) {
-			if (cl) k_sweep<true, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else k_sweep<true, false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			if (cl && p->ms8) k_sweep<true, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else if (cl) k_sweep<true, true, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else if (p->ms8) k_sweep<true, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else k_sweep<true, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		} else {
-			if (cl) k_sweep<false, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else k_sweep<false, false><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			if (cl && p->ms8) k_sweep<false, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else if (cl) k_sweep<false, true, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else if (p->ms8) k_sweep<false, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			else k_sweep<false, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		}
 	}
 }

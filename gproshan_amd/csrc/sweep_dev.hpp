@@ -12,14 +12,14 @@ namespace gp {
 
 constexpr uint32_t MAXS = 10;  // register-resident slots; longer fans take the streaming loop
 
-// FP64 update of triangle (k, k+1) of the lane at slot base (flagged
-// triangles).  ST is the distance between a lane's consecutive slots: 64 in
-// the chunked layout, 1 in the vertex-major one.
-template <uint32_t ST = 64, typename A>
-__device__ __forceinline__ float tri_flagged(const A &a, size_t at, float ta, float tb)
+// FP64 update of a flagged triangle: slot at (its first vertex) and slot
+// at_n (its second vertex: the next slot, or the lane's slot 0 for the
+// triangle that closes a fan stored without its repeated first entry).
+template <typename A>
+__device__ __forceinline__ float tri_flagged(const A &a, size_t at, size_t at_n, float ta, float tb)
 {
 	const double4 gd = a.cgeo64[at];
-	const double q1 = a.cgeo64[at + ST].x;
+	const double q1 = a.cgeo64[at_n].x;
 	return (float) update_stable<double>(gd.x, q1, gd.y, gd.z, gd.w, (double) ta, (double) tb, (double) a.h,
 	                                     (double) a.inv_h);
 }
@@ -31,17 +31,22 @@ __device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, 
 	return update_ref_f64_x(X0, X1, ta, tb);
 }
 
-// New value of the vertex at lane slot base (chunk with S slots).  FP32:
-// flagged triangles are skipped in the slot loop and done afterwards in a
-// loop that runs as many times as the most-flagged lane of the wave needs,
-// so the FP64 code runs ~1-2 times per wave instead of once per slot.
+// New value of the vertex at lane slot base (chunk with S slots; fans of up to
+// MS slots are held in registers, longer ones streamed; ST is the
+// distance between a lane's consecutive slots: 64 in the chunked layout, 1 in
+// the vertex-major one).  Triangle k joins slot k and the next slot, or slot 0
+// after the last real slot when slot 0 carries SLOT_WRAP (a closed fan stored
+// without its repeated first entry).  FP32: flagged triangles are skipped in
+// the slot loop; the first one of each lane is evaluated in FP64 from
+// geometry fetched with the neighbour distances, later ones (rare) in a loop
+// that runs as many times as the most-flagged lane of the wave needs.
 //
-// CL: also report in *bk the triangle (slot k of triangle (k, k+1)) that set
-// the new value -- the first in fan order among those reaching the minimum,
-// NO_TRI if the value did not drop -- for the cluster label.
+// CL: also report in *bk the triangle that set the new value -- the first in
+// fan order among those reaching the minimum, NO_TRI if the value did not
+// drop -- for the cluster label.
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
 
-template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, typename A>
+template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, uint32_t MS = MAXS, typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
 relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
            typename std::conditional<F64, double, float>::type best, uint32_t *bk_out = nullptr)
@@ -64,88 +69,109 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 	const G *geo;
 	if constexpr (F64) geo = a.cx64; else geo = a.cgeo;
 	uint32_t fmask = 0;
-	if (S <= MAXS) {
-		uint32_t id[MAXS];
-		G g[MAXS];
-		T t[MAXS];
+	if (S <= MS) {
+		uint32_t id[MS];
+		G g[MS];
+		T t[MS];
 #pragma unroll
-		for (uint32_t k = 0; k < MAXS; k++) {
+		for (uint32_t k = 0; k < MS; k++) {
 			id[k] = SLOT_EMPTY;
 			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
 		}
+		const bool wrap = (id[0] & SLOT_WRAP) != 0u;
+		// Slot k + 1 is real (else the triangle of slot k closes onto slot 0).
+		auto next_real = [&](uint32_t k) {
+			return k + 1 < MS && k + 1 < S && !(id[(k + 1) % MS] & SLOT_EMPTY);
+		};
 		// FP32 (PF): the FP64 geometry of the first flagged triangle is
 		// requested with the neighbour distances (both depend only on the slot
 		// ids), so it costs no extra memory round trip; later flagged
 		// triangles of the lane (rare) are deferred to the loop below.  PF =
 		// false (register-tight callers) defers them all.
-		uint32_t k0 = MAXS;
+		uint32_t k0 = MS, kn0 = 0;
 		double4 gd0 = { 0, 0, 0, 0 };
 		double q10 = 0;
 		if constexpr (!F64 && PF) {
 #pragma unroll
-			for (uint32_t k = MAXS; k-- > 0;)
-				if (k + 1 < S && (id[k] & F64_FLAG)) k0 = k;
-			if (k0 != MAXS) {
+			for (uint32_t k = MS; k-- > 0;)
+				if (k < S && (id[k] & F64_FLAG)) { k0 = k; kn0 = next_real(k) ? k + 1 : 0; }
+			if (k0 != MS) {
 				gd0 = a.cgeo64[base + ST * (size_t) k0];
-				q10 = a.cgeo64[base + ST * (size_t) (k0 + 1)].x;
+				q10 = a.cgeo64[base + ST * (size_t) kn0].x;
 			}
 		}
 #pragma unroll
-		for (uint32_t k = 0; k < MAXS; k++)
-			t[k] = (k < S) ? od[id[k] & ID30_MASK] : (T) INFINITY;
+		for (uint32_t k = 0; k < MS; k++)
+			t[k] = (k < S) ? od[id[k] & SLOT_ID_MASK] : (T) INFINITY;
 		T fa = (T) INFINITY, fb = (T) INFINITY;
 #pragma unroll
-		for (uint32_t k = 0; k + 1 < MAXS; k++) {
-			if constexpr (F64) {
-				if (k + 1 < S && !(id[k + 1] & SLOT_EMPTY)) take(tri_f64(g[k], g[k + 1], t[k], t[k + 1]), k);
-			} else if (k + 1 < S) {
-				// Straight-line per triangle: every lane evaluates the FP32
-				// update and keeps it only for a real, unflagged triangle;
-				// flagged ones are routed to the FP64 update by selects.
-				const bool real = !(id[k + 1] & SLOT_EMPTY);
-				const bool fl = (id[k] & F64_FLAG) != 0u;
-				const float p = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, t[k], t[k + 1], a.h,
-				                                     a.inv_h);
-				take((real && !fl) ? p : (T) INFINITY, k);
-				const bool first = real && fl && k == k0;
-				fa = first ? t[k] : fa;
-				fb = first ? t[k + 1] : fb;
-				fmask |= (real && fl && k != k0) ? (1u << k) : 0u;
+		for (uint32_t k = 0; k < MS; k++) {
+			if (k < S) {
+				const bool nr = next_real(k);
+				const bool tri = !(id[k] & SLOT_EMPTY) && (nr || wrap);
+				const T tb = nr ? t[(k + 1) % MS] : t[0];
+				if constexpr (F64) {
+					const G gb = nr ? g[(k + 1) % MS] : g[0];
+					if (tri) take(tri_f64(g[k], gb, t[k], tb), k);
+				} else {
+					// Straight-line per triangle: every lane evaluates the
+					// FP32 update and keeps it only for a real, unflagged
+					// triangle; flagged ones are routed to the FP64 update by
+					// selects.
+					const bool fl = (id[k] & F64_FLAG) != 0u;
+					const float q1 = nr ? g[(k + 1) % MS].x : g[0].x;
+					const float p = update_stable<float>(g[k].x, q1, g[k].y, g[k].z, g[k].w, t[k], tb, a.h, a.inv_h);
+					take((tri && !fl) ? p : (T) INFINITY, k);
+					const bool first = tri && fl && k == k0;
+					fa = first ? t[k] : fa;
+					fb = first ? tb : fb;
+					fmask |= (tri && fl && k != k0) ? (1u << k) : 0u;
+				}
 			}
 		}
 		if constexpr (!F64) {
-			if (k0 != MAXS) {
+			if (k0 != MS) {
 				take((T) update_stable<double>(gd0.x, q10, gd0.y, gd0.z, gd0.w, (double) fa, (double) fb,
 				                               (double) a.h, (double) a.inv_h), k0);
 			}
 		}
 	} else {
-		uint32_t ida = a.cid[base];
-		G ga = geo[base];
-		T ta = od[ida & ID30_MASK];
-		for (uint32_t k = 1; k < S; k++) {
+		const uint32_t id0 = a.cid[base];
+		const G g0 = geo[base];
+		const T t0 = od[id0 & SLOT_ID_MASK];
+		uint32_t ida = id0;
+		G ga = g0;
+		T ta = t0;
+		uint32_t k = 1;
+		auto tri = [&](const G &gb, T tb, size_t at_n, uint32_t kt) {
+			if constexpr (F64) {
+				take(tri_f64(ga, gb, ta, tb), kt);
+			} else {
+				const T p = (ida & F64_FLAG) ? tri_flagged(a, base + ST * (size_t) kt, at_n, ta, tb)
+				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
+				take(p, kt);
+			}
+		};
+		for (; k < S; k++) {
 			const uint32_t idb = a.cid[base + ST * k];
 			if (idb & SLOT_EMPTY) break;
 			const G gb = geo[base + ST * k];
-			const T tb = od[idb & ID30_MASK];
-			if constexpr (F64) {
-				take(tri_f64(ga, gb, ta, tb), k - 1);
-			} else {
-				const T p = (ida & F64_FLAG) ? tri_flagged<ST>(a, base + ST * (k - 1), ta, tb)
-				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
-				take(p, k - 1);
-			}
+			const T tb = od[idb & SLOT_ID_MASK];
+			tri(gb, tb, base + ST * (size_t) k, k - 1);
 			ida = idb; ga = gb; ta = tb;
 		}
+		if (id0 & SLOT_WRAP) tri(g0, t0, base, k - 1);
 	}
 	if constexpr (!F64) {
 		while (fmask) {
 			const uint32_t k = __builtin_ctz(fmask);
 			fmask &= fmask - 1u;
 			const size_t at = base + ST * (size_t) k;
-			const float ta = od[a.cid[at] & ID30_MASK];
-			const float tb = od[a.cid[at + ST] & ID30_MASK];
-			take(tri_flagged<ST>(a, at, ta, tb), k);
+			const uint32_t idn = k + 1 < S ? a.cid[at + ST] : SLOT_EMPTY;
+			const size_t at_n = (idn & SLOT_EMPTY) ? base : at + ST;
+			const float ta = od[a.cid[at] & SLOT_ID_MASK];
+			const float tb = od[a.cid[at_n] & SLOT_ID_MASK];
+			take(tri_flagged(a, at, at_n, ta, tb), k);
 		}
 	}
 	if constexpr (CL) *bk_out = bk;
