@@ -156,6 +156,31 @@ def anisotropic_grid(n: int = 2048, stretch: float = 100.0, jitter: float = 0.4,
     return xyz, grid_faces(n, n, flip)
 
 
+def polar_disk(n_rings: int = 40, n_sect: int = 24, seed: int = 0):
+    """Disk around a centre vertex of valence ``n_sect``: ``n_rings`` rings of
+    ``n_sect`` vertices (odd rings rotated half a sector, radii jittered by up
+    to 10 % of the ring spacing, heights by +-0.025), quads split along
+    alternating diagonals.  The centre's fan is longer than the register-held
+    slots of the sweep kernels when ``n_sect`` > 10.  Draw order: per ring,
+    per sector, radius jitter then height."""
+    rng = np.random.default_rng(seed)
+    pts = [[0.0, 0.0, 0.0]]
+    for r in range(1, n_rings + 1):
+        for k in range(n_sect):
+            a = 2 * np.pi * (k + 0.5 * (r % 2)) / n_sect
+            rad = r + 0.1 * rng.uniform(-1, 1)
+            pts.append([rad * np.cos(a), rad * np.sin(a), 0.05 * (rng.uniform() - 0.5)])
+
+    def vid(r, k):
+        return 1 + (r - 1) * n_sect + (k % n_sect)
+    faces = [[0, vid(1, k), vid(1, k + 1)] for k in range(n_sect)]
+    for r in range(1, n_rings):
+        for k in range(n_sect):
+            a, b, c, d = vid(r, k), vid(r, k + 1), vid(r + 1, k), vid(r + 1, k + 1)
+            faces += [[a, c, d], [a, d, b]] if (r + k) % 2 else [[a, c, b], [b, c, d]]
+    return np.array(pts), np.array(faces, np.uint32)
+
+
 def draw_sources(rng: np.random.Generator, nv: int, k: int) -> np.ndarray:
     """``k`` distinct source vertices drawn uniformly without replacement, in
     draw order (source q gets cluster id q + 1)."""

@@ -41,6 +41,8 @@ MESHES = [
     ("aniso64", lambda: G.anisotropic_grid(64, 20.0, 0.4, 7)),
     ("islands", _two_islands),
     ("ico2", lambda: G.icosphere(2, 0.01, 8)),
+    ("polar24", lambda: G.polar_disk(30, 24, 9)),
+    ("polar10", lambda: G.polar_disk(30, 10, 10)),
 ]
 
 

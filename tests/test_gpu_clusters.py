@@ -17,6 +17,7 @@ CASES = [
     ("ico5_3src", lambda: G.icosphere(5, 0.01, 0), [0, 4000, 9000]),
     ("hf256_16src", lambda: G.height_field(256, 7),
      G.draw_sources(np.random.default_rng(7), 256 * 256, 16).tolist()),
+    ("polar24_3src", lambda: G.polar_disk(40, 24, 4), [0, 500, 900]),
 ]
 
 

@@ -17,6 +17,8 @@ CASES = [
     ("hf64", lambda: G.height_field(64, 9), [100], 16, 0.0),
     ("hf96_3init", lambda: G.height_field(96, 3), [5, 4000, 9000], 20, 0.0),
     ("hf64_radius", lambda: G.height_field(64, 11), [0], 200, 0.2),
+    ("polar24", lambda: G.polar_disk(20, 24, 2), [0], 16, 0.0),
+    ("polar10", lambda: G.polar_disk(20, 10, 3), [7], 16, 0.0),
 ]
 
 

