@@ -44,7 +44,7 @@ ERR_BAR = 1e-4
 # dispatches of one band_learned run) scaled by what the same counters read for
 # a 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
 # profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources).
-MEASURED_TRAFFIC = {("band_learned", 1024, 1): (35396.7 * 2.0 + 1506.4 * 1.0) * 1024}
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): (31678.4 * 2.0 + 1506.5 * 1.0) * 1024}
 
 
 def log(*a):
@@ -234,7 +234,7 @@ def main():
                      "traffic": round(traffic) if traffic else None,
                      "algorithmic_bytes_per_launch": round(bytes_alg / launches),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false>"},
+                     "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false, false, 8u>"},
         "speedup_vs_plain_at_bar": (round(rows["plain_uncapped_f32"]["kernel_ms"] / h["kernel_ms"], 3)
                                     if "plain_uncapped_f32" in rows else None),
         "speedup_vs_plain_ref": (round(rows["plain_ref_f32"]["kernel_ms"] / h["kernel_ms"], 3)
