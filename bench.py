@@ -147,19 +147,42 @@ def main():
             continue
         t = time.time()
         plan = P.Plan(mesh, src, **kw)
+        setup_s = time.time() - t
         rows[name] = time_plan(plan, args.steps, args.warmup)
+        rows[name]["setup_s"] = setup_s
         plan.close()
         st = rows[name]["st"]
         log(f"[bench] {name}: {rows[name]['kernel_ms']:.3f} ms/field, sweeps {st['sweeps']}, "
             f"relaxations {st['relaxations']}, visits {st['patch_visits']} (plan {time.time() - t:.1f}s)")
+    # The same reference-schedule field with its rings and ring-numbered tables
+    # built on the device (gp_ptp_opts.device_setup) instead of on the host.
+    dev_setup = None
+    if not only or "plain_ref_f32" in only:
+        t = time.time()
+        plan = P.Plan(mesh, src, precision=L.PREC_F32, device_setup=True)
+        setup_s = time.time() - t
+        r = time_plan(plan, args.steps, args.warmup)
+        plan.close()
+        dev_setup = {"row": "plain_ref_f32", "host_setup_s": round(setup_s, 4), "kernel_ms": round(r["kernel_ms"], 4),
+                     "sweeps": r["st"]["sweeps"],
+                     "same_as_host_built": bool(np.array_equal(r["dist"], rows["plain_ref_f32"]["dist"]))
+                     if "plain_ref_f32" in rows else None}
+        log(f"[bench] plain_ref_f32 with device setup: {dev_setup}")
     if net is not None and (not only or "band_learned" in only):
+        t_pred = time.perf_counter()
         dhat, pred_ms = net.predict(mesh, src)
+        pred_wall_ms = (time.perf_counter() - t_pred) * 1e3
+        t = time.time()
         plan = P.Plan(mesh, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, **band_kw)
+        setup_s = time.time() - t
         rows["band_learned"] = time_plan(plan, args.steps, args.warmup)
+        rows["band_learned"]["setup_s"] = setup_s
         rows["band_learned"]["predict_ms"] = pred_ms
+        rows["band_learned"]["predict_wall_ms"] = pred_wall_ms
         plan.close()
         st = rows["band_learned"]["st"]
-        log(f"[bench] band_learned: {rows['band_learned']['kernel_ms']:.3f} ms/field (+{pred_ms:.2f} ms features+MLP), "
+        log(f"[bench] band_learned: {rows['band_learned']['kernel_ms']:.3f} ms/field (+{pred_ms:.2f} ms features+MLP kernels, "
+            f"{pred_wall_ms:.1f} ms host call with toplesets), "
             f"sweeps {st['sweeps']}, visits {st['patch_visits']}")
 
     errs = {}
@@ -242,10 +265,14 @@ def main():
         "rows": {n: {"kernel_ms": round(r["kernel_ms"], 4), "kernel_ms_min": round(r["kernel_ms_min"], 4),
                      "sweeps": r["st"]["sweeps"], "relaxations": r["st"]["relaxations"],
                      "f64_updates": r["st"]["f64_updates"], "patch_visits": r["st"]["patch_visits"],
+                     "host_setup_s": round(r["setup_s"], 3),
                      "deferred": r["st"]["deferred"], **errs.get(n, {}),
-                     **({"predict_ms": round(r["predict_ms"], 4)} if "predict_ms" in r else {})}
+                     **({"predict_ms": round(r["predict_ms"], 4),
+                         "predict_wall_ms": round(r["predict_wall_ms"], 2)} if "predict_ms" in r else {})}
                  for n, r in rows.items()},
     }
+    if dev_setup:
+        out["device_setup"] = dev_setup
     if cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)

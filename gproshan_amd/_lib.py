@@ -52,7 +52,8 @@ class PtpOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("result_buffer", C.c_int), ("schedule", C.c_int),
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
                 ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int),
-                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int), ("clusters", C.c_int)]
+                ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int), ("clusters", C.c_int),
+                ("device_setup", C.c_int)]
 
 
 class PtpStats(C.Structure):

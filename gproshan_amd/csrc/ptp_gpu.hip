@@ -36,10 +36,31 @@
 #include <type_traits>
 
 #include "device.hpp"
+#include "levels.hpp"
 #include "sweep_dev.hpp"
 #include "update_step.hpp"
 
 namespace gp {
+
+// Vertex-major slot tables of a mesh in its own numbering (closed fans without
+// their repeated first entry), the source that device-built plans gather their
+// ring-ordered tables from.  Built once per mesh and precision.
+struct MasterView {
+	const uint32_t *coff, *cid;
+	const float4 *cgeo;
+	const double4 *cgeo64, *cx64;
+};
+struct MasterLayout {
+	uint32_t *coff = nullptr, *cid = nullptr;
+	float4 *cgeo = nullptr;
+	double4 *cgeo64 = nullptr, *cx64 = nullptr;
+	uint32_t max_slots = 0;
+	MasterView view() const { return MasterView{ coff, cid, cgeo, cgeo64, cx64 }; }
+	~MasterLayout()
+	{
+		for (void *p : { (void *) coff, (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
+	}
+};
 
 void device_mesh_free(DeviceMesh *d)
 {
@@ -48,6 +69,8 @@ void device_mesh_free(DeviceMesh *d)
 	(void) hipFree(d->fan);
 	(void) hipFree(d->pos);
 	(void) hipFree(d->closed);
+	delete d->master[0];
+	delete d->master[1];
 	delete d;
 }
 
@@ -312,6 +335,69 @@ __global__ void k_to_f64(const T *in, double *out, uint32_t n)
 	for (uint32_t v = t; v < n; v += gridDim.x * blockDim.x) out[v] = (double) in[v];
 }
 
+// Device-built plans: the result in the caller's numbering.
+__global__ void k_from_positions(const double *in, const uint32_t *inv, uint32_t nv, double *out)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x)
+		out[v] = inv[v] != GP_NIL ? in[inv[v]] : (double) INFINITY;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t x)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) p[v] = x;
+}
+
+// inv[sorted[p]] = p for the nr reached vertices; src_pos = 0 .. ns - 1 (ring 0
+// holds exactly the sources).
+__global__ void k_positions(const uint32_t *sorted, uint32_t nr, uint32_t *inv, uint32_t *src_pos, uint32_t ns)
+{
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nr; q += gridDim.x * blockDim.x) {
+		inv[sorted[q]] = q;
+		if (q < ns) src_pos[q] = q;
+	}
+}
+
+// Ring-ordered chunked tables of a device-built plan, gathered from the mesh's
+// vertex-major tables: position p (vertex sorted[p]) takes lane p % 64 of chunk
+// p / 64, S slots per chunk; neighbour ids become positions (flags kept),
+// missing slots are empty (id = p).  The same tables the host builder makes
+// for the ring-numbered mesh, up to the fan's starting entry.
+template <bool F64>
+__global__ void k_gather_layout(const uint32_t *sorted, const uint32_t *inv, uint32_t nr, uint32_t npos, uint32_t S,
+                                const MasterView m, uint32_t *cid, float4 *cgeo, double4 *cgeo64, double4 *cx64)
+{
+	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
+		const size_t base = (size_t) (p >> 6) * S * 64 + (p & 63u);
+		uint32_t mb = 0, n = 0;
+		if (p < nr) {
+			const uint32_t v = sorted[p];
+			mb = m.coff[v];
+			n = m.coff[v + 1] - mb;
+		}
+		for (uint32_t k = 0; k < S; k++) {
+			const size_t at = base + 64 * (size_t) k;
+			if (k < n) {
+				const uint32_t id = m.cid[mb + k];
+				cid[at] = inv[id & SLOT_ID_MASK] | (id & ~SLOT_ID_MASK);
+				if constexpr (F64) {
+					cx64[at] = m.cx64[mb + k];
+				} else {
+					cgeo[at] = m.cgeo[mb + k];
+					cgeo64[at] = m.cgeo64[mb + k];
+				}
+			} else {
+				cid[at] = p | SLOT_EMPTY;
+				if constexpr (F64) {
+					cx64[at] = make_double4(0, 0, 0, 0);
+				} else {
+					cgeo[at] = make_float4(0.f, 0.f, 0.f, 0.f);
+					cgeo64[at] = make_double4(0, 0, 0, 0);
+				}
+			}
+		}
+	}
+}
+
 }  // namespace gp
 
 // ---------------------------------------------------------------------------
@@ -335,6 +421,7 @@ struct gp_plan {
 	double4 *d_cgeo64 = nullptr, *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
 	uint32_t *d_cl0 = nullptr, *d_cl1 = nullptr;   // cluster labels (opts.clusters)
+	uint32_t *d_inv = nullptr;                // device-built plans: original vertex -> position (NIL: unreached)
 	double *d_out = nullptr;
 	gp::PState *d_state = nullptr, *h_state = nullptr;   // h_state: 2 pinned snapshots
 	hipStream_t stream = nullptr;
@@ -346,7 +433,8 @@ struct gp_plan {
 	{
 		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
 		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
-		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_cl0, (void *) d_cl1 })
+		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_cl0, (void *) d_cl1,
+		                 (void *) d_inv })
 			(void) hipFree(p);
 		if (pp) gp::patch_destroy(pp);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
@@ -368,6 +456,51 @@ static int upload(V *&dst, const std::vector<typename std::remove_const<V>::type
 	const size_t bytes = sizeof(V) * std::max<size_t>(src.size(), 1);
 	if (hipMalloc(&dst, bytes) != hipSuccess) return fail(GP_ERR_NOMEM, "device allocation of %zu bytes failed", bytes);
 	if (!src.empty()) GP_HIP(hipMemcpy(dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+	return GP_OK;
+}
+
+// The sweep kernel's launch width: one wave per chunk, at most as many
+// workgroups as fit on the device at once (idle workgroups of small windows
+// then cost no second dispatch round).  Fans of up to 8 slots (interior
+// valence up to 8 once closed fans drop their repeat) run a kernel with 8
+// register-resident slots.
+static int sweep_grid(gp_plan *p, bool f64)
+{
+	int dev = 0, ncu = 0, occ = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	p->ms8 = p->max_slots <= 8;
+	const void *kfn = f64 ? (p->ms8 ? (const void *) k_sweep<true, false, 8> : (const void *) k_sweep<true, false, MAXS>)
+	                      : (p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>);
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, 256, 0));
+	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
+	p->grid = p->opts.grid ? p->opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (p->nchunks + 3) / 4));
+	return GP_OK;
+}
+
+static int plan_stream(gp_plan *p)
+{
+	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+	GP_HIP(hipEventCreate(&p->e0));
+	GP_HIP(hipEventCreate(&p->e1));
+	GP_HIP(hipEventCreate(&p->ev[0]));
+	GP_HIP(hipEventCreate(&p->ev[1]));
+	return GP_OK;
+}
+
+// Distances, labels, result, schedule state and trace of a ring plan.
+static int plan_run_buffers(gp_plan *p, bool f64)
+{
+	const size_t es = f64 ? sizeof(double) : sizeof(float);
+	if (p->opts.clusters &&
+	    (hipMalloc(&p->d_cl0, 4ull * p->n) != hipSuccess || hipMalloc(&p->d_cl1, 4ull * p->n) != hipSuccess))
+		return fail(GP_ERR_NOMEM, "plan: cluster allocation failed");
+	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
+	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
+	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
+	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + 1)) != hipSuccess ||
+	    hipHostMalloc(&p->h_state, 2 * sizeof(PState)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	return GP_OK;
 }
 
@@ -425,26 +558,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	// has no cap, so a generous one stands in as a guard.
 	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (opts.uncapped ? 64 * nl + 64 : (nl << 1));
 	p->block = 256;
-	// One wave per chunk, at most as many workgroups as fit on the device at
-	// once (idle workgroups of small windows then cost no second dispatch round).
-	int dev = 0, ncu = 0, occ = 0;
-	GP_HIP(hipGetDevice(&dev));
-	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	// Fans of up to 8 slots (interior valence up to 8 once closed fans drop
-	// their repeat) run a kernel with 8 register-resident slots.
-	p->ms8 = L.max_slots <= 8;
-	const void *kfn = f64 ? (p->ms8 ? (const void *) k_sweep<true, false, 8> : (const void *) k_sweep<true, false, MAXS>)
-	                      : (p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>);
-	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, 256, 0));
-	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
-	p->grid = opts.grid ? opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (L.nchunks + 3) / 4));
-
-	const size_t es = f64 ? sizeof(double) : sizeof(float);
-	GP_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-	GP_HIP(hipEventCreate(&p->e0));
-	GP_HIP(hipEventCreate(&p->e1));
-	GP_HIP(hipEventCreate(&p->ev[0]));
-	GP_HIP(hipEventCreate(&p->ev[1]));
+	if ((rc = sweep_grid(p.get(), f64))) return rc;
+	if ((rc = plan_stream(p.get()))) return rc;
 	std::vector<uint32_t> lim(limits, limits + nl);
 	if ((rc = upload(p->d_limits, lim)) || (rc = upload(p->d_src, srcm)) || (rc = upload(p->d_coff, L.coff)) ||
 	    (rc = upload(p->d_cid, L.cid)))
@@ -467,15 +582,121 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		std::vector<uint32_t> ord(sorted, sorted + nr);
 		if ((rc = upload(p->d_order, ord))) return rc;
 	}
-	if (opts.clusters &&
-	    (hipMalloc(&p->d_cl0, 4ull * p->n) != hipSuccess || hipMalloc(&p->d_cl1, 4ull * p->n) != hipSuccess))
-		return fail(GP_ERR_NOMEM, "plan: cluster allocation failed");
-	if (hipMalloc(&p->d_dist0, es * p->n) != hipSuccess || hipMalloc(&p->d_dist1, es * p->n) != hipSuccess ||
-	    hipMalloc(&p->d_out, sizeof(double) * p->n) != hipSuccess ||
-	    hipMalloc(&p->d_state, sizeof(PState)) != hipSuccess ||
-	    hipMalloc(&p->d_trace, sizeof(uint32_t) * 8 * (size_t) (p->max_iter + 1)) != hipSuccess ||
-	    hipHostMalloc(&p->h_state, 2 * sizeof(PState)) != hipSuccess)
+	if ((rc = plan_run_buffers(p.get(), f64))) return rc;
+	*out = p.release();
+	return GP_OK;
+}
+
+// Plan of the ring schedule on a ring-numbered layout built on the device:
+// rings by the frontier BFS and counting sort (no host BFS, no host
+// renumbering), tables gathered from the mesh's vertex-major tables.  The
+// rings are the reference's as sets, the order inside a ring is not (nothing
+// depends on it); no FP64-update count.  GP_ERR_CAPACITY when a fan needs
+// more register-held slots than the sweep kernels hold (the caller then takes
+// the host-built path).
+int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out)
+{
+	*out = nullptr;
+	if (!mesh || !src) return fail(GP_ERR_INVALID, "plan: null argument");
+	if (ns == 0) return fail(GP_ERR_INVALID, "plan: no sources");
+	if (opts.schedule != GP_SCHED_RINGS || !opts.coalesce || opts.clusters)
+		return fail(GP_ERR_INVALID, "plan_create_device: ring schedule, coalesced, without labels only");
+	{
+		std::vector<uint8_t> seen(mesh->nv, 0);
+		for (uint32_t q = 0; q < ns; q++) {
+			if (src[q] >= mesh->nv) return fail(GP_ERR_INVALID, "plan: source %u out of range", src[q]);
+			if (seen[src[q]]) return fail(GP_ERR_DUPLICATE_SOURCE, "plan: source %u listed twice", src[q]);
+			seen[src[q]] = 1;
+		}
+	}
+	int rc = upload_mesh(*mesh);
+	if (rc) return rc;
+	const bool f64 = opts.precision == GP_PREC_F64;
+	DeviceMesh *dm = mesh->dev;
+	MasterLayout *ml = dm->master[f64 ? 1 : 0];
+	if (!ml) {
+		Layout L;
+		if ((rc = build_layout(*mesh, nullptr, mesh->nv, f64, L, 0, 1, 0, true))) return rc;
+		auto m = std::make_unique<MasterLayout>();
+		m->max_slots = L.max_slots;
+		auto as_d4 = [](const std::vector<Geo64> &v) {
+			std::vector<double4> o(v.size());
+			std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
+			return o;
+		};
+		if ((rc = upload(m->coff, L.coff)) || (rc = upload(m->cid, L.cid))) return rc;
+		if (f64) {
+			if ((rc = upload(m->cx64, as_d4(L.cx64)))) return rc;
+		} else {
+			std::vector<float4> cg(L.cgeo.size());
+			std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+			if ((rc = upload(m->cgeo, cg)) || (rc = upload(m->cgeo64, as_d4(L.cgeo64)))) return rc;
+		}
+		ml = dm->master[f64 ? 1 : 0] = m.release();
+	}
+	if (ml->max_slots > MAXS) return fail(GP_ERR_CAPACITY, "plan: a fan needs %u slots", ml->max_slots);
+	auto p = std::make_unique<gp_plan>();
+	p->mesh = mesh;
+	p->opts = opts;
+	p->nv_orig = mesh->nv;
+	p->ns = ns;
+	p->dm = dm;
+	if ((rc = plan_stream(p.get()))) return rc;
+	// Rings of the sources on the device.
+	Levels lv;
+	if ((rc = lv.create(dm, p->stream))) return rc;
+	std::vector<uint32_t> srcv(src, src + ns);
+	uint32_t *d_srcv = nullptr;
+	if ((rc = upload(d_srcv, srcv))) return rc;
+	std::unique_ptr<uint32_t, hipError_t (*)(void *)> src_guard(d_srcv, hipFree);
+	lv.clear();
+	lv.seed(d_srcv, 0, ns);
+	if ((rc = lv.expand())) return rc;
+	lv.sort();
+	uint32_t nl = 0;
+	GP_HIP(hipMemcpyAsync(&nl, lv.nl, 4, hipMemcpyDeviceToHost, p->stream));
+	GP_HIP(hipStreamSynchronize(p->stream));
+	if (nl < 2 || nl > mesh->nv + 1) return fail(GP_ERR_KERNEL, "plan: bad ring count %u", nl);
+	uint32_t nr = 0;
+	GP_HIP(hipMemcpy(&nr, lv.limits + (nl - 1), 4, hipMemcpyDeviceToHost));
+	p->nl = nl;
+	p->nr = nr;
+	p->n = nr;
+	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (opts.uncapped ? 64 * nl + 64 : (nl << 1));
+	p->block = 256;
+	const uint32_t S = ml->max_slots;
+	p->nchunks = (nr + 63) / 64;
+	p->max_slots = S;
+	p->sb = S;
+	p->n_flagged = 0;
+	const size_t lanes = (size_t) p->nchunks * S * 64;
+	if (hipMalloc(&p->d_limits, 4ull * nl) != hipSuccess || hipMalloc(&p->d_src, 4ull * ns) != hipSuccess ||
+	    hipMalloc(&p->d_inv, 4ull * mesh->nv) != hipSuccess || hipMalloc(&p->d_coff, 4) != hipSuccess ||
+	    hipMalloc(&p->d_cid, 4 * std::max<size_t>(lanes, 1)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
+	if (f64) {
+		if (hipMalloc(&p->d_cx64, sizeof(double4) * std::max<size_t>(lanes, 1)) != hipSuccess)
+			return fail(GP_ERR_NOMEM, "plan: device allocation failed");
+	} else if (hipMalloc(&p->d_cgeo, sizeof(float4) * std::max<size_t>(lanes, 1)) != hipSuccess ||
+	           hipMalloc(&p->d_cgeo64, sizeof(double4) * std::max<size_t>(lanes, 1)) != hipSuccess) {
+		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
+	}
+	GP_HIP(hipMemcpyAsync(p->d_limits, lv.limits, 4ull * nl, hipMemcpyDeviceToDevice, p->stream));
+	const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(2048, (mesh->nv + 255) / 256));
+	k_fill_u32<<<g, 256, 0, p->stream>>>(p->d_inv, mesh->nv, GP_NIL);
+	k_positions<<<g, 256, 0, p->stream>>>(lv.sorted, nr, p->d_inv, p->d_src, ns);
+	const uint32_t npos = p->nchunks * 64;
+	const uint32_t gg = std::max<uint32_t>(1, std::min<uint32_t>(4096, (npos + 255) / 256));
+	if (f64)
+		k_gather_layout<true><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr, npos, S, ml->view(), p->d_cid,
+		                                                  nullptr, nullptr, p->d_cx64);
+	else
+		k_gather_layout<false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr, npos, S, ml->view(), p->d_cid,
+		                                                   p->d_cgeo, p->d_cgeo64, nullptr);
+	GP_HIP(hipGetLastError());
+	GP_HIP(hipStreamSynchronize(p->stream));
+	if ((rc = sweep_grid(p.get(), f64))) return rc;
+	if ((rc = plan_run_buffers(p.get(), f64))) return rc;
 	*out = p.release();
 	return GP_OK;
 }
@@ -632,7 +853,17 @@ int plan_result(gp_plan *p, double *dist)
 	else
 		k_to_f64<float><<<g, 256, 0, p->stream>>>((const float *) buf, p->d_out, p->n);
 	GP_HIP(hipGetLastError());
-	if (p->coal) {
+	if (p->d_inv) {
+		double *d_orig = nullptr;
+		if (hipMalloc(&d_orig, sizeof(double) * p->nv_orig) != hipSuccess)
+			return fail(GP_ERR_NOMEM, "plan_result: allocation failed");
+		const uint32_t go = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->nv_orig + 255) / 256));
+		k_from_positions<<<go, 256, 0, p->stream>>>(p->d_out, p->d_inv, p->nv_orig, d_orig);
+		const hipError_t e1 = hipMemcpyAsync(dist, d_orig, sizeof(double) * p->nv_orig, hipMemcpyDeviceToHost, p->stream);
+		const hipError_t e2 = hipStreamSynchronize(p->stream);
+		(void) hipFree(d_orig);
+		if (e1 != hipSuccess || e2 != hipSuccess) return fail(GP_ERR_HIP, "plan_result: copy failed");
+	} else if (p->coal) {
 		std::vector<double> tmp(p->n);
 		GP_HIP(hipMemcpyAsync(tmp.data(), p->d_out, sizeof(double) * p->n, hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipStreamSynchronize(p->stream));

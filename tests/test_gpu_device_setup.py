@@ -1,0 +1,81 @@
+"""Plans built on the device (gp_ptp_opts.device_setup) against plans built on
+the host.
+
+The device build replaces the host toplesets and renumbering by a frontier BFS
+and a counting sort: the rings are the same sets, the order inside a ring is
+not the reference's discovery order.  A sweep relaxes whole rings from the
+previous buffer, so the schedule trace must be identical and the FP64 field
+bit-identical to the host-built plan (and so to the FP64 oracle); FP32 fields
+agree to the rounding of the mean edge length (the host build scales the
+geometry by the mean edge of the renumbered mesh, summed in another order).
+A fan longer than the register-held slots falls back to the host build.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from gproshan_amd import _lib as L
+from gproshan_amd import geodesics as P
+from gproshan_amd import meshgen as G
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("ico5", lambda: G.icosphere(5, 0.01, 0), [0]),
+    ("hf128_4src", lambda: G.height_field(128, 1), [777, 5000, 9000, 16000]),
+    ("hf256_16src", lambda: G.height_field(256, 7),
+     G.draw_sources(np.random.default_rng(7), 256 * 256, 16).tolist()),
+    ("aniso128", lambda: G.anisotropic_grid(128, 20.0, 0.4, 3), [4000]),
+    ("polar10_rim", lambda: G.polar_disk(30, 10, 1), [1 + 29 * 10 + 3]),
+    ("polar24", lambda: G.polar_disk(40, 24, 0), [0]),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request):
+    name, gen, src = request.param
+    xyz, f = gen()
+    print(f"\n[case] {name} V={len(xyz)} sources={src}", flush=True)
+    return name, np.array(src, np.uint32), P.che(xyz, f)
+
+
+def _solve(C, src, **kw):
+    plan = P.Plan(C, src, **kw)
+    st = plan.run()
+    d = plan.result()
+    tr = plan.trace()
+    st2 = plan.run()
+    d2 = plan.result()
+    plan.close()
+    assert np.array_equal(d.view(np.uint64), d2.view(np.uint64)) and st2["sweeps"] == st["sweeps"]
+    return d, tr, st
+
+
+@pytest.mark.parametrize("sched", [dict(), dict(uncapped=True, tol=1e-6)], ids=["ref", "uncapped"])
+def test_device_setup_f64_identical(case, oracle, sched):
+    name, src, C = case
+    host, tr_h, st_h = _solve(C, src, precision=L.PREC_F64, **sched)
+    dev, tr_d, st_d = _solve(C, src, precision=L.PREC_F64, device_setup=True, **sched)
+    print(f"[device f64] {name} {sched} sweeps {st_d['sweeps']}/{st_h['sweeps']}", flush=True)
+    assert np.array_equal(host.view(np.uint64), dev.view(np.uint64))
+    assert np.array_equal(tr_h, tr_d)
+
+
+def test_device_setup_f32_close(case):
+    name, src, C = case
+    host, tr_h, _ = _solve(C, src, precision=L.PREC_F32)
+    dev, tr_d, _ = _solve(C, src, precision=L.PREC_F32, device_setup=True)
+    mx, _ = rel_err(dev, host)
+    same = len(tr_h) == len(tr_d) and np.array_equal(tr_h[:, 1:7], tr_d[:, 1:7])
+    print(f"[device f32] {name} max {mx:.2e} same schedule {same}", flush=True)
+    assert mx <= (1e-6 if same else 1e-3)
+    assert np.array_equal(np.isinf(host), np.isinf(dev))
+
+
+def test_device_setup_errors():
+    xyz, f = G.height_field(16, 0)
+    C = P.che(xyz, f)
+    with pytest.raises(P.GPError):
+        P.Plan(C, [3, 3], device_setup=True)
+    with pytest.raises(P.GPError):
+        P.Plan(C, [len(xyz)], device_setup=True)
