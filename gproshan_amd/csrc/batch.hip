@@ -6,14 +6,17 @@
 // runs it entirely by itself: the ring build (the same BFS as the toplesets
 // kernel), then the reference window schedule (K/2 band limit, PTP_TOL test,
 // 2*|limits| cap, last-written buffer) with __syncthreads as the only barrier,
-// relaxing vertices through sorted[] (as the reference's non-coalesced
-// driver does, src/cuda/geodesics_ptp.cu:174-299).  A field's window is in
-// BFS order, not memory order, so each mesh is stored vertex-major (a vertex's
-// slots are contiguous) and renumbered in Morton order (a window's vertices
-// share cache lines); the renumbering keeps every fan's order, hence the
-// rings and the arithmetic, and the output is scattered back.  Every
-// workgroup leaves the loop when the queue is exhausted, so the launch always
-// drains.
+// on the rings of the reference's non-coalesced driver
+// (src/cuda/geodesics_ptp.cu:174-299).  Each mesh is stored vertex-major (a
+// vertex's slots are contiguous) in Morton order; after its BFS the workgroup
+// gathers the field's own ring-ordered tables into its scratch (64 positions
+// to a chunk, one lane per vertex, neighbour ids mapped to positions, as the
+// coalesced driver renumbers the mesh, src/geodesics_ptp.cpp:15-38), so every
+// slot load of a sweep is one contiguous access; the FP64 geometry of flagged
+// triangles is read from the mesh's tables through the slot's index there.
+// The gather keeps every fan's order, hence the rings and the arithmetic, and
+// the output is scattered back.  Every workgroup leaves the loop when the
+// queue is exhausted, so the launch always drains.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +42,7 @@ struct BMesh {
 	const double4 *cx64;
 	const uint32_t *perm;   // Morton position -> original vertex id
 	float h, inv_h;
-	uint32_t nv, pad;
+	uint32_t nv, ms;        // ms: the longest fan (slots of a field's chunk)
 };
 
 struct BJob {
@@ -57,12 +60,33 @@ struct BArgs {
 	float *dist;
 	float *out;
 	double tol;
+	// Per-workgroup field tables (fslots slots each): ids in positions, FP32
+	// geometry, the slot's index in the mesh's tables; inv: vertex -> position.
+	uint32_t *fcid, *fmidx, *finv;
+	float4 *fgeo;
+	size_t fslots;
+};
+
+// FP64 geometry of a field slot, read from the mesh's table.
+struct MeshGeo64 {
+	const double4 *geo;
+	const uint32_t *midx;
+	__device__ double4 operator[](size_t at) const { return geo[midx[at]]; }
+};
+
+// The field's tables as relax_lane reads them.
+struct FieldTables {
+	const uint32_t *cid;
+	const float4 *cgeo;
+	MeshGeo64 cgeo64;
+	const double4 *cx64;
+	float h, inv_h;
 };
 
 __global__ void k_batch_reset(uint32_t *queue) { *queue = 0; }
 
 // NT threads per field (a multiple of 64, at most 1024).
-template <uint32_t NT>
+template <uint32_t NT, uint32_t MS>
 __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 {
 	__shared__ uint32_t s_job, s_bad;
@@ -73,6 +97,9 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 	uint32_t *limits = a.limits + wg * (mv + 2);
 	unsigned long long *key = a.key + wg * mv;
 	float *d0 = a.dist + 2 * wg * mv, *d1 = d0 + mv;
+	uint32_t *fcid = a.fcid + wg * a.fslots, *fmidx = a.fmidx + wg * a.fslots, *finv = a.finv + wg * mv;
+	float4 *fgeo = a.fgeo + wg * a.fslots;
+	const uint32_t lane = tid & 63u, wave = tid >> 6, nwave = NT >> 6;
 	for (;;) {
 		if (tid == 0) s_job = atomicAdd(a.queue, 1u);
 		__syncthreads();
@@ -83,14 +110,44 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 		const BMesh M = a.meshes[a.jobs[job].mesh];
 		const uint32_t nv = M.nv;
 		for (uint32_t v = tid; v < nv; v += nt) {
-			level[v] = GP_NIL; key[v] = ~0ull; sorted[v] = GP_NIL;
+			level[v] = GP_NIL; key[v] = ~0ull; sorted[v] = GP_NIL; finv[v] = GP_NIL;
 			d0[v] = INFINITY; d1[v] = INFINITY;
 		}
 		__syncthreads();
-		if (tid == 0) { d0[src] = 0.f; d1[src] = 0.f; }
 		const BfsJob bj{ &a.jobs[job].source, 1u, level, key, sorted, limits, a.nlim + wg };
 		bfs_block(M.fan_off, M.fan, M.closed, bj);
 		const uint32_t nl = __hip_atomic_load(a.nlim + wg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		// The field's ring-ordered tables: position p holds vertex sorted[p]
+		// (the source is position 0).
+		const uint32_t nr = nl ? limits[nl - 1] : 0u;
+		for (uint32_t q = tid; q < nr; q += nt) finv[sorted[q]] = q;
+		__syncthreads();
+		const uint32_t S = M.ms, npos = (nr + 63u) & ~63u;
+		for (uint32_t q = tid; q < npos; q += nt) {
+			const size_t fb = (size_t) (q >> 6) * S * 64 + (q & 63u);
+			uint32_t mb = 0, n = 0;
+			if (q < nr) {
+				const uint32_t v = sorted[q];
+				mb = M.coff[v];
+				n = M.coff[v + 1] - mb;
+			}
+			for (uint32_t k = 0; k < S; k++) {
+				const size_t at = fb + 64 * (size_t) k;
+				if (k < n) {
+					const uint32_t id = M.cid[mb + k];
+					fcid[at] = finv[id & SLOT_ID_MASK] | (id & ~SLOT_ID_MASK);
+					fgeo[at] = M.cgeo[mb + k];
+					fmidx[at] = mb + k;
+				} else {
+					fcid[at] = q | SLOT_EMPTY;
+					fgeo[at] = make_float4(0.f, 0.f, 0.f, 0.f);
+					fmidx[at] = 0;
+				}
+			}
+		}
+		if (tid == 0 && nr) { d0[0] = 0.f; d1[0] = 0.f; }
+		__syncthreads();
+		const FieldTables F{ fcid, fgeo, MeshGeo64{ M.cgeo64, fmidx }, nullptr, M.h, M.inv_h };
 		uint32_t i = 1, j = 2, d = 0, iter = 0;
 		unsigned long long relax = 0;
 		const uint32_t max_iter = nl << 1;
@@ -103,16 +160,18 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 			if (tid == 0) s_bad = 0;
 			__syncthreads();
 			uint32_t bad = 0;
-			for (uint32_t pos = start + tid; pos < end; pos += nt) {
-				const uint32_t v = sorted[pos];
-				const uint32_t cb = M.coff[v];
-				const float old = od[v];
-				const float best = relax_lane<false, 1, false>(M, (size_t) cb, M.coff[v + 1] - cb, od, old);
-				nd[v] = best;
-				if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+			for (uint32_t c = (start >> 6) + wave; c < ((end + 63u) >> 6); c += nwave) {
+				const uint32_t pos = (c << 6) + lane;
+				bool lbad = false;
+				if (pos >= start && pos < end) {
+					const float old = od[pos];
+					const float best = relax_lane<false, 64, false, false, MS>(F, (size_t) c * S * 64 + lane, S, od, old);
+					nd[pos] = best;
+					if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
+				}
+				bad += (uint32_t) __popcll(__ballot(lbad));
 			}
-			for (uint32_t o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
-			if ((tid & 63u) == 0 && bad) atomicAdd(&s_bad, bad);
+			if (lane == 0 && bad) atomicAdd(&s_bad, bad);
 			__syncthreads();
 			const uint32_t nbad = s_bad;
 			relax += end - start;
@@ -125,7 +184,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 		}
 		const float *res = d ? d1 : d0;            // the buffer the last sweep wrote
 		float *out = a.out + a.jobs[job].out;
-		for (uint32_t v = tid; v < nv; v += nt) out[M.perm[v]] = res[v];
+		for (uint32_t v = tid; v < nv; v += nt) out[M.perm[v]] = finv[v] != GP_NIL ? res[finv[v]] : INFINITY;
 		if (tid == 0) { a.jobs[job].sweeps = iter; a.jobs[job].relax = relax; }
 		__syncthreads();
 	}
@@ -155,7 +214,9 @@ struct gp_batch {
 	uint32_t *d_level = nullptr, *d_sorted = nullptr, *d_limits = nullptr, *d_nlim = nullptr;
 	unsigned long long *d_key = nullptr;
 	float *d_dist = nullptr;
-	uint32_t njobs = 0, max_nv = 0, grid = 0, block = 0;
+	uint32_t *d_fcid = nullptr, *d_fmidx = nullptr, *d_finv = nullptr;
+	float4 *d_fgeo = nullptr;
+	uint32_t njobs = 0, max_nv = 0, max_slots = 0, grid = 0, block = 0;
 	unsigned long long total_out = 0;
 	double tol = GP_PTP_TOL;
 	hipStream_t stream = nullptr;
@@ -165,11 +226,13 @@ struct gp_batch {
 	void free_scratch()
 	{
 		for (void *p : { (void *) d_level, (void *) d_sorted, (void *) d_limits, (void *) d_nlim, (void *) d_key,
-		                 (void *) d_dist })
+		                 (void *) d_dist, (void *) d_fcid, (void *) d_fmidx, (void *) d_finv, (void *) d_fgeo })
 			(void) hipFree(p);
 		d_level = d_sorted = d_limits = d_nlim = nullptr;
 		d_key = nullptr;
 		d_dist = nullptr;
+		d_fcid = d_fmidx = d_finv = nullptr;
+		d_fgeo = nullptr;
 		if (graph) (void) hipGraphExecDestroy(graph);
 		graph = nullptr;
 	}
@@ -196,15 +259,16 @@ static int up(V *&dst, const V *src, size_t n)
 	return GP_OK;
 }
 
+// Fields whose fans all fit 8 slots run the 8-slot register path.
 template <uint32_t NT>
-static int launch_shape(uint32_t njobs, uint32_t &grid, const void *&fn)
+static int launch_shape(uint32_t njobs, bool ms8, uint32_t &grid, const void *&fn)
 {
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batch<NT>, NT, 0));
+	fn = ms8 ? (const void *) k_batch<NT, 8> : (const void *) k_batch<NT, MAXS>;
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, NT, 0));
 	grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t) ncu * (uint32_t) std::max(1, occ)));
-	fn = (const void *) k_batch<NT>;
 	return GP_OK;
 }
 
@@ -217,21 +281,31 @@ static int batch_configure(gp_batch *b, uint32_t block)
 	const void *fn = nullptr;
 	int rc;
 	switch (block) {
-	case 256: rc = launch_shape<256>(b->njobs, grid, fn); break;
-	case 512: rc = launch_shape<512>(b->njobs, grid, fn); break;
-	case 1024: rc = launch_shape<1024>(b->njobs, grid, fn); break;
+	case 256: rc = launch_shape<256>(b->njobs, b->max_slots <= 8, grid, fn); break;
+	case 512: rc = launch_shape<512>(b->njobs, b->max_slots <= 8, grid, fn); break;
+	case 1024: rc = launch_shape<1024>(b->njobs, b->max_slots <= 8, grid, fn); break;
 	default: return fail(GP_ERR_INVALID, "batch: block must be 256, 512 or 1024 (got %u)", block);
 	}
 	if (rc) return rc;
-	const size_t mv = b->max_nv, g = grid;
+	const size_t mv = b->max_nv;
+	// Field tables: the largest field's chunks at the longest fan's width.
+	const size_t fslots = ((mv + 63) / 64) * 64 * (size_t) b->max_slots;
+	const size_t per_wg = 4 * mv * 3 + 4 * (mv + 2) + 8 * mv + 8 * mv + fslots * (4 + 4 + 16);
+	// At most 32 GiB of scratch: a wider grid than that would take no more
+	// fields at once than the workgroups that fit.
+	const size_t cap = std::max<size_t>(1, (32ull << 30) / std::max<size_t>(per_wg, 1));
+	grid = (uint32_t) std::min<size_t>(grid, cap);
+	const size_t g = grid;
 	if (hipMalloc(&b->d_level, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_sorted, 4 * mv * g) != hipSuccess ||
 	    hipMalloc(&b->d_limits, 4 * (mv + 2) * g) != hipSuccess || hipMalloc(&b->d_nlim, 4 * g) != hipSuccess ||
-	    hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess) {
+	    hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess ||
+	    hipMalloc(&b->d_finv, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_fcid, 4 * fslots * g) != hipSuccess ||
+	    hipMalloc(&b->d_fmidx, 4 * fslots * g) != hipSuccess || hipMalloc(&b->d_fgeo, 16 * fslots * g) != hipSuccess) {
 		b->free_scratch();
 		return fail(GP_ERR_NOMEM, "batch: scratch allocation failed (%u workgroups x %u vertices)", grid, b->max_nv);
 	}
 	BArgs a{ b->d_meshes, b->d_jobs, b->njobs, b->max_nv, b->d_queue, b->d_level, b->d_sorted, b->d_limits, b->d_nlim,
-	         b->d_key, b->d_dist, b->d_out, b->tol };
+	         b->d_key, b->d_dist, b->d_out, b->tol, b->d_fcid, b->d_fmidx, b->d_finv, b->d_fgeo, fslots };
 	void *kargs[] = { &a };
 	hipGraph_t gr = nullptr;
 	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -300,8 +374,9 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 		    (rc = up(t.perm, order.data(), order.size())))
 			return rc;
 		bm[k] = BMesh{ m.dev->fan_off, m.dev->fan, m.dev->closed, t.coff, t.cid, t.cgeo, t.cgeo64, nullptr, t.perm,
-		               m.dev->h, m.dev->inv_h, m.nv, 0 };
+		               m.dev->h, m.dev->inv_h, m.nv, L.max_slots };
 		b->max_nv = std::max(b->max_nv, m.nv);
+		b->max_slots = std::max(b->max_slots, L.max_slots);
 	}
 	// Queue order: longest fields first (work of a field grows ~ V^1.5), so
 	// the last jobs taken are the short ones and the grid drains evenly.
