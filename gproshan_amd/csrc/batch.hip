@@ -67,18 +67,11 @@ struct BArgs {
 	size_t fslots;
 };
 
-// FP64 geometry of a field slot, read from the mesh's table.
-struct MeshGeo64 {
-	const double4 *geo;
-	const uint32_t *midx;
-	__device__ double4 operator[](size_t at) const { return geo[midx[at]]; }
-};
-
 // The field's tables as relax_lane reads them.
 struct FieldTables {
 	const uint32_t *cid;
 	const float4 *cgeo;
-	MeshGeo64 cgeo64;
+	GeoRemap64 cgeo64;
 	const double4 *cx64;
 	float h, inv_h;
 };
@@ -147,7 +140,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 		}
 		if (tid == 0 && nr) { d0[0] = 0.f; d1[0] = 0.f; }
 		__syncthreads();
-		const FieldTables F{ fcid, fgeo, MeshGeo64{ M.cgeo64, fmidx }, nullptr, M.h, M.inv_h };
+		const FieldTables F{ fcid, fgeo, GeoRemap64{ M.cgeo64, fmidx }, nullptr, M.h, M.inv_h };
 		uint32_t i = 1, j = 2, d = 0, iter = 0;
 		unsigned long long relax = 0;
 		const uint32_t max_iter = nl << 1;

@@ -17,9 +17,11 @@
 // and limits.  The order inside a ring differs from the reference's discovery
 // order; a sweep relaxes a whole window of rings from the previous buffer, so
 // the order inside a ring changes neither any value nor any ring test, and
-// the samples and radius are the reference's.  The mesh keeps its numbering;
-// its slots are stored vertex-major (a window vertex reads its fan from
-// consecutive addresses).
+// the samples and radius are the reference's.  Each round gathers the mesh's
+// vertex-major slot tables into ring-ordered chunks (positions: ring 0 first,
+// neighbour ids mapped to positions) so that the sweeps read coalesced slot
+// rows as the single-field sweep kernel does; the maximum is taken over the
+// vertices in their own numbering through the positions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,25 +47,33 @@ struct FState {
 	uint32_t bad[4];
 	uint32_t done, sweeps, final_d, nl;
 	unsigned long long best;   // ordered bits of the round's largest distance
-	uint32_t arg, stop, pad2, pad3;
+	uint32_t arg, stop, nr, pad3;         // nr: vertices reached this round
+};
+
+// Tables of a round: the rings of the current samples, gathered into
+// ring-ordered chunks (positions), read by relax_lane.
+struct FTables {
+	const uint32_t *cid;
+	const float4 *cgeo;
+	GeoRemap64 cgeo64;
+	const double4 *cx64;
+	float h, inv_h;
 };
 
 struct FArgs {
-	const uint32_t *coff, *cid;
-	const float4 *cgeo;
-	const double4 *cgeo64;
-	const double4 *cx64;
-	float h, inv_h;
-	const uint32_t *sorted, *limits, *nl;   // rings of the current samples
-	void *dist0, *dist1;
+	FTables t;
+	uint32_t S;                             // slots per chunk
+	const uint32_t *limits, *nl;            // rings of the current samples (device)
+	void *dist0, *dist1;                    // by position
 	FState *st;
 	double tol;
 };
 
 // Sweep k of a round: the reference loop tail on the previous state, then the
-// window relaxation (as the single-field sweep kernel, with |limits| read from
-// the device because the rings are rebuilt every round).
-template <bool F64>
+// window relaxation as the single-field sweep kernel does it (one wave per
+// chunk of 64 positions, XCD-aware split), with |limits| read from the device
+// because the rings are rebuilt every round.
+template <bool F64, uint32_t MS>
 __global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
 {
 	using T = typename std::conditional<F64, double, float>::type;
@@ -98,24 +108,42 @@ __global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
 	const uint32_t start = a.limits[i], cend = a.limits[i + 1], end = a.limits[cur.j];
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	// XCD-aware split of the window: workgroups b % 8 == x take the x-th
-	// contiguous eighth of the positions.
+	const uint32_t c0 = start >> 6, c1 = (end + 63u) >> 6;
+	const uint32_t lane = threadIdx.x & 63u;
+	// XCD-aware split: workgroups b % 8 == x take the x-th contiguous eighth
+	// of the window's chunks.
 	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
 	const uint32_t x = blockIdx.x & 7u;
 	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
-	const uint32_t len = end - start, part = (len + ngrp - 1) / ngrp;
-	const uint32_t lo = start + x * part, hi = min(end, lo + part);
+	const uint32_t wpb = blockDim.x >> 6;
+	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
+	const uint32_t part = (c1 - c0 + ngrp - 1) / ngrp;
+	const uint32_t lo = c0 + x * part, hi = min(c1, lo + part);
 	uint32_t bad = 0;
-	for (uint32_t pos = lo + (blockIdx.x >> 3) * blockDim.x + threadIdx.x; pos < hi; pos += nbx * blockDim.x) {
-		const uint32_t v = a.sorted[pos];
-		const uint32_t cb = a.coff[v];
-		const T old = od[v];
-		const T best = relax_lane<F64, 1, false>(a, (size_t) cb, a.coff[v + 1] - cb, od, old);
-		nd[v] = best;
-		if (pos < cend) bad += !(fabs((double) best - (double) old) / (double) old < a.tol);
+	for (uint32_t c = lo + lw; c < hi; c += nbx * wpb) {
+		const uint32_t pos = (c << 6) + lane;
+		bool lbad = false;
+		if (pos >= start && pos < end) {
+			const T old = od[pos];
+			const T best = relax_lane<F64, 64, false, false, MS>(a.t, (size_t) c * a.S * 64 + lane, a.S, od, old);
+			nd[pos] = best;
+			if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
+		}
+		bad += (uint32_t) __popcll(__ballot(lbad));
 	}
-	for (uint32_t o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
-	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
+	if (lane == 0 && bad) atomicAdd(&st->bad[k4], bad);
+}
+
+// Positions of a round: inv[v] = p for the reached vertices (sorted[p] = v),
+// NIL for the others.
+__global__ void k_fps_inv_clear(uint32_t *inv, uint32_t n)
+{
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) inv[v] = GP_NIL;
+}
+__global__ void k_fps_inv(const uint32_t *sorted, const FState *st, uint32_t *inv)
+{
+	const uint32_t nr = st->nr;
+	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nr; p += gridDim.x * blockDim.x) inv[sorted[p]] = p;
 }
 
 // Round set-up: both distance buffers at infinity.
@@ -127,18 +155,20 @@ __global__ void k_fps_prep(T *d0, T *d1, uint32_t n)
 	}
 }
 
+// The samples are ring 0: positions 0 .. ns - 1.
 template <typename T>
 __global__ void k_fps_sources(T *d0, T *d1, const BfsJob *job)
 {
-	for (uint32_t q = threadIdx.x; q < job->ns; q += blockDim.x) { d0[job->src[q]] = (T) 0; d1[job->src[q]] = (T) 0; }
+	for (uint32_t q = threadIdx.x; q < job->ns; q += blockDim.x) { d0[q] = (T) 0; d1[q] = (T) 0; }
 }
 
-__global__ void k_fps_state_init(FState *st, const uint32_t *nl)
+__global__ void k_fps_state_init(FState *st, const uint32_t *nl, const uint32_t *limits)
 {
 	uint32_t *w = (uint32_t *) st;
 	for (uint32_t q = 0; q < sizeof(FState) / 4; q++) w[q] = 0;
 	const uint32_t n = *nl;
 	st->nl = n;
+	st->nr = limits[n - 1];
 	st->slot[3].i = 1; st->slot[3].j = 2; st->slot[3].fresh = 1;
 	st->slot[3].done = n < 3 ? 1u : 0u;     // only the sources: no sweep
 	st->done = st->slot[3].done;
@@ -147,15 +177,23 @@ __global__ void k_fps_state_init(FState *st, const uint32_t *nl)
 __device__ __forceinline__ unsigned long long ordered_bits(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ unsigned long long ordered_bits(double x) { return (unsigned long long) __double_as_longlong(x); }
 
-// Largest |distance| of the buffer the last sweep wrote (distances are >= 0 or
-// +inf, so their IEEE bit patterns order like the values).
+// Largest |distance| of the buffer the last sweep wrote, over the vertices in
+// their own numbering (unreached ones are +inf, as in the reference's field;
+// distances are >= 0 or +inf, so their IEEE bit patterns order like the
+// values).
 template <typename T>
-__global__ void k_fps_max(const void *d0, const void *d1, uint32_t n, FState *st)
+__device__ __forceinline__ T vertex_dist(const T *d, const uint32_t *inv, uint32_t v)
+{
+	return inv[v] != GP_NIL ? d[inv[v]] : (T) INFINITY;
+}
+
+template <typename T>
+__global__ void k_fps_max(const void *d0, const void *d1, const uint32_t *inv, uint32_t n, FState *st)
 {
 	const T *d = (const T *) (st->final_d ? d1 : d0);
 	unsigned long long m = 0;
 	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const unsigned long long b = ordered_bits(fabs(d[v]));
+		const unsigned long long b = ordered_bits(fabs(vertex_dist(d, inv, v)));
 		m = b > m ? b : m;
 	}
 	for (uint32_t o = 32; o > 0; o >>= 1) {
@@ -167,13 +205,13 @@ __global__ void k_fps_max(const void *d0, const void *d1, uint32_t n, FState *st
 
 // First vertex holding that maximum (the smallest index, as I?amax returns).
 template <typename T>
-__global__ void k_fps_arg(const void *d0, const void *d1, uint32_t n, FState *st)
+__global__ void k_fps_arg(const void *d0, const void *d1, const uint32_t *inv, uint32_t n, FState *st)
 {
 	const T *d = (const T *) (st->final_d ? d1 : d0);
 	const unsigned long long best = st->best;
 	uint32_t a = GP_NIL;
 	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
-		if (ordered_bits(fabs(d[v])) == best) { a = v; break; }
+		if (ordered_bits(fabs(vertex_dist(d, inv, v))) == best) { a = v; break; }
 	for (uint32_t o = 32; o > 0; o >>= 1) a = min(a, (uint32_t) __shfl_xor(a, o));
 	if ((threadIdx.x & 63u) == 0 && a != GP_NIL) atomicMin(&st->arg, a);
 }
@@ -194,9 +232,9 @@ __global__ void k_fps_append(BfsJob *job, uint32_t *samples, FState *st)
 // ---------------------------------------------------------------------------
 
 struct FpsRun {
-	uint32_t *coff = nullptr, *cid = nullptr, *samples = nullptr;
+	uint32_t *cid = nullptr, *midx = nullptr, *inv = nullptr, *samples = nullptr;
 	float4 *cgeo = nullptr;
-	double4 *cgeo64 = nullptr, *cx64 = nullptr;
+	double4 *cx64 = nullptr;
 	void *d0 = nullptr, *d1 = nullptr;
 	FState *st = nullptr, *h_st = nullptr;
 	BfsJob *job = nullptr;
@@ -205,7 +243,7 @@ struct FpsRun {
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	~FpsRun()
 	{
-		for (void *p : { (void *) coff, (void *) cid, (void *) samples, (void *) cgeo, (void *) cgeo64, (void *) cx64,
+		for (void *p : { (void *) cid, (void *) midx, (void *) inv, (void *) samples, (void *) cgeo, (void *) cx64,
 		                 d0, d1, (void *) st, (void *) job })
 			(void) hipFree(p);
 		if (h_st) (void) hipHostFree(h_st);
@@ -238,27 +276,20 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	const uint64_t rounds = n > n_init ? n - n_init : 0;
 	if (n_init + rounds > cap) return fail(GP_ERR_CAPACITY, "fps: samples needs room for %llu entries",
 	                                       (unsigned long long) (n_init + rounds));
-	int rc = upload_mesh(m);
+	VertexTables mt{};
+	uint32_t S = 0;
+	int rc = master_tables(m, F64, mt, S);
 	if (rc) return rc;
 	FpsRun r;
-	Layout L;
-	if ((rc = build_layout(m, nullptr, nv, F64, L, 0, 1, 0, true))) return rc;
-	auto as_d4 = [](const std::vector<Geo64> &v) {
-		std::vector<double4> o(v.size());
-		std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
-		return o;
-	};
-	if ((rc = fup(r.coff, L.coff.data(), L.coff.size())) || (rc = fup(r.cid, L.cid.data(), L.cid.size())))
-		return rc;
-	if (F64) {
-		const auto x = as_d4(L.cx64);
-		if ((rc = fup(r.cx64, x.data(), x.size()))) return rc;
-	} else {
-		const auto g64 = as_d4(L.cgeo64);
-		if ((rc = fup(r.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
-		    (rc = fup(r.cgeo64, g64.data(), g64.size())))
-			return rc;
-	}
+	// Ring-ordered chunked tables of a round, S slots per chunk (the longest
+	// fan; fans past the register-held slots take the streaming loop).
+	const uint32_t nchunks = (nv + 63) / 64, npos = nchunks * 64;
+	const size_t lanes = (size_t) nchunks * S * 64;
+	if (hipMalloc(&r.cid, 4 * lanes) != hipSuccess || hipMalloc(&r.inv, 4ull * nv) != hipSuccess ||
+	    (F64 ? hipMalloc(&r.cx64, sizeof(double4) * lanes)
+	         : (hipMalloc(&r.cgeo, sizeof(float4) * lanes) == hipSuccess ? hipMalloc(&r.midx, 4 * lanes)
+	                                                                      : hipErrorOutOfMemory)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "fps: table allocation failed");
 	if ((rc = fup(r.samples, samples, cap))) return rc;
 	if (hipMalloc(&r.d0, sizeof(T) * nv) != hipSuccess || hipMalloc(&r.d1, sizeof(T) * nv) != hipSuccess ||
 	    hipMalloc(&r.st, sizeof(FState)) != hipSuccess || hipMalloc(&r.job, sizeof(BfsJob)) != hipSuccess ||
@@ -272,21 +303,25 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	GP_HIP(hipEventCreate(&r.e0));
 	GP_HIP(hipEventCreate(&r.e1));
 	FArgs a{};
-	a.coff = r.coff; a.cid = r.cid; a.cgeo = r.cgeo; a.cgeo64 = r.cgeo64; a.cx64 = r.cx64;
-	a.h = m.dev->h; a.inv_h = m.dev->inv_h;
-	a.sorted = lv.sorted; a.limits = lv.limits; a.nl = lv.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
+	a.t.cid = r.cid; a.t.cgeo = r.cgeo; a.t.cgeo64 = GeoRemap64{ mt.cgeo64, r.midx }; a.t.cx64 = r.cx64;
+	a.t.h = m.dev->h; a.t.inv_h = m.dev->inv_h;
+	a.S = S; a.limits = lv.limits; a.nl = lv.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
 	a.tol = GP_PTP_TOL;
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_fps_sweep<F64>, 256, 0));
+	const bool ms8 = S <= 8;
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms8 ? k_fps_sweep<F64, 8> : k_fps_sweep<F64, MAXS>, 256, 0));
 	const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>((uint32_t) ncu * (uint32_t) std::max(1, occ),
-	                                                               (nv + 255) / 256));
+	                                                               (nchunks + 3) / 4));
 	constexpr uint32_t BIG = 64, SMALL = 8;
 	auto capture = [&](uint32_t count, hipGraphExec_t *out) -> int {
 		hipGraph_t g = nullptr;
 		GP_HIP(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
-		for (uint32_t q = 0; q < count; q++) k_fps_sweep<F64><<<grid, 256, 0, r.stream>>>(a, q & 3u);
+		for (uint32_t q = 0; q < count; q++) {
+			if (ms8) k_fps_sweep<F64, 8><<<grid, 256, 0, r.stream>>>(a, q & 3u);
+			else k_fps_sweep<F64, MAXS><<<grid, 256, 0, r.stream>>>(a, q & 3u);
+		}
 		GP_HIP(hipStreamEndCapture(r.stream, &g));
 		const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
 		(void) hipGraphDestroy(g);
@@ -295,6 +330,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	};
 	if ((rc = capture(BIG, &r.g_big)) || (rc = capture(SMALL, &r.g_small))) return rc;
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(2048, (nv + 255) / 256));
+	const uint32_t gg = std::max<uint32_t>(1, std::min<uint32_t>(4096, (npos + 255) / 256));
 	const auto t0 = std::chrono::steady_clock::now();
 	GP_HIP(hipEventRecord(r.e0, r.stream));
 	double max_dist = INFINITY;
@@ -308,7 +344,15 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		lv.seed(r.samples, count == n_init ? 0u : count - 1, count);
 		if ((rc = lv.expand())) return rc;
 		lv.sort();
-		k_fps_state_init<<<1, 1, 0, r.stream>>>(r.st, lv.nl);
+		k_fps_state_init<<<1, 1, 0, r.stream>>>(r.st, lv.nl, lv.limits);
+		k_fps_inv_clear<<<ig, 256, 0, r.stream>>>(r.inv, nv);
+		k_fps_inv<<<ig, 256, 0, r.stream>>>(lv.sorted, r.st, r.inv);
+		if (F64)
+			k_gather_tables<true, false><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, mt, r.cid,
+			                                                        nullptr, nullptr, nullptr, r.cx64);
+		else
+			k_gather_tables<false, true><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, mt, r.cid,
+			                                                        r.cgeo, nullptr, r.midx, nullptr);
 		GP_HIP(hipGetLastError());
 		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
 		GP_HIP(hipStreamSynchronize(r.stream));
@@ -327,8 +371,8 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 			j_seen = r.h_st->slot[3].j;
 		}
 		k_fps_arg_init<<<1, 1, 0, r.stream>>>(r.st);
-		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
-		k_fps_arg<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
+		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, r.inv, nv, r.st);
+		k_fps_arg<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, r.inv, nv, r.st);
 		k_fps_append<T><<<1, 1, 0, r.stream>>>(r.job, r.samples, r.st);
 		GP_HIP(hipGetLastError());
 		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));

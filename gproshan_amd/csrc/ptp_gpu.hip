@@ -45,17 +45,12 @@ namespace gp {
 // Vertex-major slot tables of a mesh in its own numbering (closed fans without
 // their repeated first entry), the source that device-built plans gather their
 // ring-ordered tables from.  Built once per mesh and precision.
-struct MasterView {
-	const uint32_t *coff, *cid;
-	const float4 *cgeo;
-	const double4 *cgeo64, *cx64;
-};
 struct MasterLayout {
 	uint32_t *coff = nullptr, *cid = nullptr;
 	float4 *cgeo = nullptr;
 	double4 *cgeo64 = nullptr, *cx64 = nullptr;
 	uint32_t max_slots = 0;
-	MasterView view() const { return MasterView{ coff, cid, cgeo, cgeo64, cx64 }; }
+	VertexTables view() const { return VertexTables{ coff, cid, cgeo, cgeo64, cx64 }; }
 	~MasterLayout()
 	{
 		for (void *p : { (void *) coff, (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
@@ -357,47 +352,6 @@ __global__ void k_positions(const uint32_t *sorted, uint32_t nr, uint32_t *inv, 
 	}
 }
 
-// Ring-ordered chunked tables of a device-built plan, gathered from the mesh's
-// vertex-major tables: position p (vertex sorted[p]) takes lane p % 64 of chunk
-// p / 64, S slots per chunk; neighbour ids become positions (flags kept),
-// missing slots are empty (id = p).  The same tables the host builder makes
-// for the ring-numbered mesh, up to the fan's starting entry.
-template <bool F64>
-__global__ void k_gather_layout(const uint32_t *sorted, const uint32_t *inv, uint32_t nr, uint32_t npos, uint32_t S,
-                                const MasterView m, uint32_t *cid, float4 *cgeo, double4 *cgeo64, double4 *cx64)
-{
-	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
-		const size_t base = (size_t) (p >> 6) * S * 64 + (p & 63u);
-		uint32_t mb = 0, n = 0;
-		if (p < nr) {
-			const uint32_t v = sorted[p];
-			mb = m.coff[v];
-			n = m.coff[v + 1] - mb;
-		}
-		for (uint32_t k = 0; k < S; k++) {
-			const size_t at = base + 64 * (size_t) k;
-			if (k < n) {
-				const uint32_t id = m.cid[mb + k];
-				cid[at] = inv[id & SLOT_ID_MASK] | (id & ~SLOT_ID_MASK);
-				if constexpr (F64) {
-					cx64[at] = m.cx64[mb + k];
-				} else {
-					cgeo[at] = m.cgeo[mb + k];
-					cgeo64[at] = m.cgeo64[mb + k];
-				}
-			} else {
-				cid[at] = p | SLOT_EMPTY;
-				if constexpr (F64) {
-					cx64[at] = make_double4(0, 0, 0, 0);
-				} else {
-					cgeo[at] = make_float4(0.f, 0.f, 0.f, 0.f);
-					cgeo64[at] = make_double4(0, 0, 0, 0);
-				}
-			}
-		}
-	}
-}
-
 }  // namespace gp
 
 // ---------------------------------------------------------------------------
@@ -604,9 +558,43 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	return GP_OK;
 }
 
+// Vertex-major slot tables of the mesh (built on the host once per mesh and
+// precision, kept with the device mesh) and the longest fan's slot count.
+int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
+{
+	int rc = upload_mesh(mesh);
+	if (rc) return rc;
+	DeviceMesh *dm = mesh.dev;
+	MasterLayout *ml = dm->master[f64 ? 1 : 0];
+	if (!ml) {
+		Layout L;
+		if ((rc = build_layout(mesh, nullptr, mesh.nv, f64, L, 0, 1, 0, true))) return rc;
+		auto m = std::make_unique<MasterLayout>();
+		m->max_slots = L.max_slots;
+		auto as_d4 = [](const std::vector<Geo64> &v) {
+			std::vector<double4> o(v.size());
+			std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
+			return o;
+		};
+		if ((rc = upload(m->coff, L.coff)) || (rc = upload(m->cid, L.cid))) return rc;
+		if (f64) {
+			if ((rc = upload(m->cx64, as_d4(L.cx64)))) return rc;
+		} else {
+			std::vector<float4> cg(L.cgeo.size());
+			std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+			if ((rc = upload(m->cgeo, cg)) || (rc = upload(m->cgeo64, as_d4(L.cgeo64)))) return rc;
+		}
+		ml = dm->master[f64 ? 1 : 0] = m.release();
+	}
+	out = ml->view();
+	max_slots = ml->max_slots;
+	return GP_OK;
+}
+
 // Plan of the ring schedule on a ring-numbered layout built on the device:
 // rings by the frontier BFS and counting sort (no host BFS, no host
-// renumbering), tables gathered from the mesh's vertex-major tables.  The
+// renumbering), tables gathered from the mesh's vertex-major tables
+// (k_gather_tables; every chunk at the longest fan's width).  The
 // rings are the reference's as sets, the order inside a ring is not (nothing
 // depends on it); no FP64-update count.  GP_ERR_CAPACITY when a fan needs
 // more register-held slots than the sweep kernels hold (the caller then takes
@@ -630,28 +618,10 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	if (rc) return rc;
 	const bool f64 = opts.precision == GP_PREC_F64;
 	DeviceMesh *dm = mesh->dev;
-	MasterLayout *ml = dm->master[f64 ? 1 : 0];
-	if (!ml) {
-		Layout L;
-		if ((rc = build_layout(*mesh, nullptr, mesh->nv, f64, L, 0, 1, 0, true))) return rc;
-		auto m = std::make_unique<MasterLayout>();
-		m->max_slots = L.max_slots;
-		auto as_d4 = [](const std::vector<Geo64> &v) {
-			std::vector<double4> o(v.size());
-			std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
-			return o;
-		};
-		if ((rc = upload(m->coff, L.coff)) || (rc = upload(m->cid, L.cid))) return rc;
-		if (f64) {
-			if ((rc = upload(m->cx64, as_d4(L.cx64)))) return rc;
-		} else {
-			std::vector<float4> cg(L.cgeo.size());
-			std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
-			if ((rc = upload(m->cgeo, cg)) || (rc = upload(m->cgeo64, as_d4(L.cgeo64)))) return rc;
-		}
-		ml = dm->master[f64 ? 1 : 0] = m.release();
-	}
-	if (ml->max_slots > MAXS) return fail(GP_ERR_CAPACITY, "plan: a fan needs %u slots", ml->max_slots);
+	VertexTables mt{};
+	uint32_t S = 0;
+	if ((rc = master_tables(*mesh, f64, mt, S))) return rc;
+	if (S > MAXS) return fail(GP_ERR_CAPACITY, "plan: a fan needs %u slots", S);
 	auto p = std::make_unique<gp_plan>();
 	p->mesh = mesh;
 	p->opts = opts;
@@ -681,7 +651,6 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	p->n = nr;
 	p->max_iter = opts.max_sweeps ? opts.max_sweeps : (opts.uncapped ? 64 * nl + 64 : (nl << 1));
 	p->block = 256;
-	const uint32_t S = ml->max_slots;
 	p->nchunks = (nr + 63) / 64;
 	p->max_slots = S;
 	p->sb = S;
@@ -704,12 +673,13 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	k_positions<<<g, 256, 0, p->stream>>>(lv.sorted, nr, p->d_inv, p->d_src, ns);
 	const uint32_t npos = p->nchunks * 64;
 	const uint32_t gg = std::max<uint32_t>(1, std::min<uint32_t>(4096, (npos + 255) / 256));
+	const uint32_t *nr_at = lv.limits + (nl - 1);
 	if (f64)
-		k_gather_layout<true><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr, npos, S, ml->view(), p->d_cid,
-		                                                  nullptr, nullptr, p->d_cx64);
+		k_gather_tables<true, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, mt,
+		                                                         p->d_cid, nullptr, nullptr, nullptr, p->d_cx64);
 	else
-		k_gather_layout<false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr, npos, S, ml->view(), p->d_cid,
-		                                                   p->d_cgeo, p->d_cgeo64, nullptr);
+		k_gather_tables<false, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, mt,
+		                                                          p->d_cid, p->d_cgeo, p->d_cgeo64, nullptr, nullptr);
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipStreamSynchronize(p->stream));
 	if ((rc = sweep_grid(p.get(), f64))) return rc;

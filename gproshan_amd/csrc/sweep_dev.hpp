@@ -178,4 +178,59 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 	return best;
 }
 
+// Vertex-major slot tables of a mesh in its own numbering (closed fans
+// without their repeated first entry): slot k of vertex v at coff[v] + k.
+struct VertexTables {
+	const uint32_t *coff, *cid;
+	const float4 *cgeo;
+	const double4 *cgeo64, *cx64;
+};
+
+// The mesh's vertex-major tables for the precision (built once per mesh) and
+// its longest fan's slot count.
+int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots);
+
+// FP64 geometry of a chunked slot read from the vertex-major tables through
+// the slot's index there (flagged triangles are rare: no chunked copy).
+struct GeoRemap64 {
+	const double4 *geo;
+	const uint32_t *midx;
+	__device__ double4 operator[](size_t at) const { return geo[midx[at]]; }
+};
+
+// Ring-ordered chunked tables gathered from vertex-major ones: position p
+// (vertex sorted[p], p < *nr_at) takes lane p % 64 of chunk p / 64, S slots
+// per chunk; neighbour ids become positions (flags kept), missing slots are
+// empty (id = p).  FP32 tables keep the FP64 geometry as a copy (cgeo64) or,
+// MIDX, as the slot's index in the vertex-major tables (midx).
+template <bool F64, bool MIDX>
+__global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, const uint32_t *nr_at, uint32_t npos,
+                                uint32_t S, VertexTables m, uint32_t *cid, float4 *cgeo, double4 *cgeo64,
+                                uint32_t *midx, double4 *cx64)
+{
+	const uint32_t nr = *nr_at;
+	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
+		const size_t base = (size_t) (p >> 6) * S * 64 + (p & 63u);
+		uint32_t mb = 0, n = 0;
+		if (p < nr) {
+			const uint32_t v = sorted[p];
+			mb = m.coff[v];
+			n = m.coff[v + 1] - mb;
+		}
+		for (uint32_t k = 0; k < S; k++) {
+			const size_t at = base + 64 * (size_t) k;
+			const bool real = k < n;
+			const uint32_t id = real ? m.cid[mb + k] : 0u;
+			cid[at] = real ? (inv[id & SLOT_ID_MASK] | (id & ~SLOT_ID_MASK)) : (p | SLOT_EMPTY);
+			if constexpr (F64) {
+				cx64[at] = real ? m.cx64[mb + k] : make_double4(0, 0, 0, 0);
+			} else {
+				cgeo[at] = real ? m.cgeo[mb + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+				if constexpr (MIDX) midx[at] = real ? mb + k : 0u;
+				else cgeo64[at] = real ? m.cgeo64[mb + k] : make_double4(0, 0, 0, 0);
+			}
+		}
+	}
+}
+
 }  // namespace gp
