@@ -50,8 +50,13 @@ __global__ void k_lv_seed(const uint32_t *src, uint32_t first, uint32_t ns, uint
 }
 
 // One BFS level over all workgroups.  A neighbour whose level drops to L + 1
-// (atomicMin: exactly one thread sees it drop) is appended once; appends are
-// counted per wave.
+// (atomicMin: exactly one thread sees it drop) is appended once.  A lane
+// issues the atomics of LVU fan entries back to back (slots past its fan
+// apply the no-op minimum NIL to its own vertex), so one level costs one
+// atomic round trip per LVU entries, not one per entry; the appends of a wave
+// take one counter atomic per pass (ballot per entry slot).
+constexpr uint32_t LVU = 8;
+
 __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const uint32_t *fan, uint32_t *level,
                                                  uint32_t *queue, LevelState *ls, uint32_t k4)
 {
@@ -76,10 +81,11 @@ __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const 
 	}
 	if (cur.done) return;
 	const uint32_t lane = threadIdx.x & 63u;
+	const unsigned long long below = (1ull << lane) - 1ull;
 	const uint32_t nl1 = cur.L + 1;
 	const uint32_t stride = gridDim.x * blockDim.x;
-	// Every lane of a wave runs the same number of iterations (the wave-wide
-	// append below needs them all).
+	// Wave-uniform loops: every lane of a wave runs the same iterations (the
+	// ballots and the shuffle below need them all).
 	for (uint32_t b = cur.fa + (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)); b < cur.fb; b += stride) {
 		const uint32_t idx = b + lane;
 		const bool act = idx < cur.fb;
@@ -88,20 +94,35 @@ __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const 
 		const uint32_t len = e1 - e0;
 		uint32_t mx = len;
 		for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
-		for (uint32_t e = 0; e < mx; e++) {
-			bool app = false;
-			uint32_t w = 0;
-			if (e < len) {
-				w = fan[e0 + e];
-				app = atomicMin(&level[w], nl1) > nl1;
+		for (uint32_t eb = 0; eb < mx; eb += LVU) {
+			uint32_t w[LVU];
+			uint32_t app = 0;
+			if (eb < len) {
+#pragma unroll
+				for (uint32_t k = 0; k < LVU; k++) w[k] = fan[min(e0 + eb + k, e1 - 1u)];
+#pragma unroll
+				for (uint32_t k = 0; k < LVU; k++) {
+					const bool ok = eb + k < len;
+					const uint32_t old = atomicMin(&level[ok ? w[k] : u], ok ? nl1 : GP_NIL);
+					app |= (ok && old > nl1) ? (1u << k) : 0u;
+				}
 			}
-			const unsigned long long m = __ballot(app);
-			if (m) {
+			unsigned long long m[LVU];
+			uint32_t tot = 0;
+#pragma unroll
+			for (uint32_t k = 0; k < LVU; k++) {
+				m[k] = __ballot((app >> k) & 1u);
+				tot += (uint32_t) __popcll(m[k]);
+			}
+			if (tot) {
 				uint32_t base = 0;
-				if (lane == (uint32_t) (__ffsll((long long) m) - 1))
-					base = atomicAdd(&ls->cnt[k4], (uint32_t) __popcll(m));
-				base = __shfl(base, __ffsll((long long) m) - 1);
-				if (app) queue[cur.fb + base + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = w;
+				if (lane == 0) base = atomicAdd(&ls->cnt[k4], tot);
+				base = cur.fb + __shfl(base, 0);
+#pragma unroll
+				for (uint32_t k = 0; k < LVU; k++) {
+					if ((app >> k) & 1u) queue[base + (uint32_t) __popcll(m[k] & below)] = w[k];
+					base += (uint32_t) __popcll(m[k]);
+				}
 			}
 		}
 	}
@@ -109,15 +130,26 @@ __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const 
 
 // Counting sort of the reached vertices by level: histogram (and the largest
 // level), exclusive scan into limits (one workgroup), scatter into sorted[].
+// Neighbouring vertex ids mostly share a level, so a wave takes one atomic per
+// distinct level among its 64 vertices.
 __global__ void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LevelState *ls)
 {
+	const uint32_t lane = threadIdx.x & 63u;
 	uint32_t mx = 0;
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const uint32_t l = level[v];
-		if (l != GP_NIL) { atomicAdd(&hist[l], 1u); mx = max(mx, l); }
+	for (uint32_t b = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += gridDim.x * blockDim.x) {
+		const uint32_t v = b + lane;
+		const uint32_t l = v < n ? level[v] : GP_NIL;
+		if (l != GP_NIL) mx = max(mx, l);
+		for (unsigned long long pend = __ballot(l != GP_NIL); pend;) {
+			const uint32_t first = (uint32_t) __ffsll((long long) pend) - 1u;
+			const uint32_t l0 = __shfl(l, first);
+			const unsigned long long same = __ballot(l == l0);
+			if (lane == first) atomicAdd(&hist[l0], (uint32_t) __popcll(same));
+			pend &= ~same;
+		}
 	}
 	for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
-	if ((threadIdx.x & 63u) == 0) atomicMax(&ls->max_level, mx);
+	if (lane == 0) atomicMax(&ls->max_level, mx);
 }
 
 __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limits, uint32_t *nl, const LevelState *ls)
@@ -150,9 +182,21 @@ __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limi
 
 __global__ void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor, uint32_t *sorted)
 {
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const uint32_t l = level[v];
-		if (l != GP_NIL) sorted[atomicAdd(&cursor[l], 1u)] = v;
+	const uint32_t lane = threadIdx.x & 63u;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	for (uint32_t b = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += gridDim.x * blockDim.x) {
+		const uint32_t v = b + lane;
+		const uint32_t l = v < n ? level[v] : GP_NIL;
+		for (unsigned long long pend = __ballot(l != GP_NIL); pend;) {
+			const uint32_t first = (uint32_t) __ffsll((long long) pend) - 1u;
+			const uint32_t l0 = __shfl(l, first);
+			const unsigned long long same = __ballot(l == l0);
+			uint32_t base = 0;
+			if (lane == first) base = atomicAdd(&cursor[l0], (uint32_t) __popcll(same));
+			base = __shfl(base, first);
+			if (l == l0) sorted[base + (uint32_t) __popcll(same & below)] = v;
+			pend &= ~same;
+		}
 	}
 }
 
