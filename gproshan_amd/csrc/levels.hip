@@ -130,26 +130,61 @@ __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const 
 
 // Counting sort of the reached vertices by level: histogram (and the largest
 // level), exclusive scan into limits (one workgroup), scatter into sorted[].
-// Neighbouring vertex ids mostly share a level, so a wave takes one atomic per
-// distinct level among its 64 vertices.
-__global__ void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LevelState *ls)
+// Global atomics on a few hundred level counters serialise, so each workgroup
+// counts a block of SORT_BLOCK vertices in LDS bins (levels lo .. lo + SORT_BINS
+// - 1 of the block; levels past them, rare, count globally) and adds or
+// reserves one global count per level present in the block.
+constexpr uint32_t SORT_BLOCK = 8192, SORT_BINS = 4096, SORT_T = 1024;
+
+// Smallest and largest reached level of the block [b0, b1) (every thread).
+__device__ void block_level_range(const uint32_t *level, uint32_t b0, uint32_t b1, uint32_t *red, uint32_t &lo,
+                                  uint32_t &hi)
 {
-	const uint32_t lane = threadIdx.x & 63u;
-	uint32_t mx = 0;
-	for (uint32_t b = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += gridDim.x * blockDim.x) {
-		const uint32_t v = b + lane;
-		const uint32_t l = v < n ? level[v] : GP_NIL;
-		if (l != GP_NIL) mx = max(mx, l);
-		for (unsigned long long pend = __ballot(l != GP_NIL); pend;) {
-			const uint32_t first = (uint32_t) __ffsll((long long) pend) - 1u;
-			const uint32_t l0 = __shfl(l, first);
-			const unsigned long long same = __ballot(l == l0);
-			if (lane == first) atomicAdd(&hist[l0], (uint32_t) __popcll(same));
-			pend &= ~same;
-		}
+	lo = GP_NIL; hi = 0;
+	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
+		const uint32_t l = level[v];
+		if (l != GP_NIL) { lo = min(lo, l); hi = max(hi, l); }
 	}
-	for (uint32_t o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t) __shfl_xor(mx, o));
-	if (lane == 0) atomicMax(&ls->max_level, mx);
+	for (uint32_t o = 32; o > 0; o >>= 1) {
+		lo = min(lo, (uint32_t) __shfl_xor(lo, o));
+		hi = max(hi, (uint32_t) __shfl_xor(hi, o));
+	}
+	const uint32_t w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63u) == 0) { red[w] = lo; red[16 + w] = hi; }
+	__syncthreads();
+	lo = GP_NIL; hi = 0;
+	for (uint32_t q = 0; q < SORT_T / 64; q++) { lo = min(lo, red[q]); hi = max(hi, red[16 + q]); }
+}
+
+// LDS counts of the block's levels lo .. lo + SORT_BINS - 1; the others are
+// counted in glob (when given).
+__device__ void block_level_counts(const uint32_t *level, uint32_t b0, uint32_t b1, uint32_t lo, uint32_t *bins,
+                                   uint32_t *glob)
+{
+	for (uint32_t q = threadIdx.x; q < SORT_BINS; q += SORT_T) bins[q] = 0;
+	__syncthreads();
+	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
+		const uint32_t l = level[v];
+		if (l == GP_NIL) continue;
+		if (l - lo < SORT_BINS) atomicAdd(&bins[l - lo], 1u);
+		else if (glob) atomicAdd(&glob[l], 1u);
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(SORT_T) void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LevelState *ls)
+{
+	__shared__ uint32_t bins[SORT_BINS];
+	__shared__ uint32_t red[32];
+	const uint32_t b0 = blockIdx.x * SORT_BLOCK, b1 = min(n, b0 + SORT_BLOCK);
+	uint32_t lo, hi;
+	block_level_range(level, b0, b1, red, lo, hi);
+	if (lo == GP_NIL) return;   // nothing reached in the block (uniform)
+	block_level_counts(level, b0, b1, lo, bins, hist);
+	const uint32_t range = min(SORT_BINS, hi - lo + 1);
+	for (uint32_t q = threadIdx.x; q < range; q += SORT_T)
+		if (bins[q]) atomicAdd(&hist[lo + q], bins[q]);
+	if (threadIdx.x == 0) atomicMax(&ls->max_level, hi);
 }
 
 __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limits, uint32_t *nl, const LevelState *ls)
@@ -180,23 +215,25 @@ __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limi
 	if (threadIdx.x == 0) { limits[0] = 0; *nl = nb + 1; }
 }
 
-__global__ void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor, uint32_t *sorted)
+__global__ __launch_bounds__(SORT_T) void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor,
+                                                       uint32_t *sorted)
 {
-	const uint32_t lane = threadIdx.x & 63u;
-	const unsigned long long below = (1ull << lane) - 1ull;
-	for (uint32_t b = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); b < n; b += gridDim.x * blockDim.x) {
-		const uint32_t v = b + lane;
-		const uint32_t l = v < n ? level[v] : GP_NIL;
-		for (unsigned long long pend = __ballot(l != GP_NIL); pend;) {
-			const uint32_t first = (uint32_t) __ffsll((long long) pend) - 1u;
-			const uint32_t l0 = __shfl(l, first);
-			const unsigned long long same = __ballot(l == l0);
-			uint32_t base = 0;
-			if (lane == first) base = atomicAdd(&cursor[l0], (uint32_t) __popcll(same));
-			base = __shfl(base, first);
-			if (l == l0) sorted[base + (uint32_t) __popcll(same & below)] = v;
-			pend &= ~same;
-		}
+	__shared__ uint32_t bins[SORT_BINS];
+	__shared__ uint32_t red[32];
+	const uint32_t b0 = blockIdx.x * SORT_BLOCK, b1 = min(n, b0 + SORT_BLOCK);
+	uint32_t lo, hi;
+	block_level_range(level, b0, b1, red, lo, hi);
+	if (lo == GP_NIL) return;
+	block_level_counts(level, b0, b1, lo, bins, nullptr);
+	// Reserve the block's run of each level; the bins become its cursors.
+	const uint32_t range = min(SORT_BINS, hi - lo + 1);
+	for (uint32_t q = threadIdx.x; q < range; q += SORT_T)
+		if (bins[q]) bins[q] = atomicAdd(&cursor[lo + q], bins[q]);
+	__syncthreads();
+	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
+		const uint32_t l = level[v];
+		if (l == GP_NIL) continue;
+		sorted[l - lo < SORT_BINS ? atomicAdd(&bins[l - lo], 1u) : atomicAdd(&cursor[l], 1u)] = v;
 	}
 }
 
@@ -257,11 +294,11 @@ int Levels::expand()
 
 void Levels::sort()
 {
-	const uint32_t g = fill_grid(nv + 1);
+	const uint32_t g = fill_grid(nv + 1), gb = (nv + SORT_BLOCK - 1) / SORT_BLOCK;
 	k_lv_zero<<<g, 256, 0, stream>>>(hist, nv + 1);
-	k_lv_hist<<<g, 256, 0, stream>>>(level, nv, hist, ls);
+	k_lv_hist<<<gb, SORT_T, 0, stream>>>(level, nv, hist, ls);
 	k_lv_scan<<<1, 1024, 0, stream>>>(hist, limits, nl, ls);
-	k_lv_scatter<<<g, 256, 0, stream>>>(level, nv, hist, sorted);
+	k_lv_scatter<<<gb, SORT_T, 0, stream>>>(level, nv, hist, sorted);
 }
 
 }  // namespace gp
