@@ -90,7 +90,8 @@ def main():
     ap.add_argument("--patch", type=int, default=512, help="patch schedule: patch size (256 or 512)")
     ap.add_argument("--stress", action="store_true", help="precision stress case instead (2048^2 1:100 grid, 4 sources)")
     ap.add_argument("--batch", action="store_true", help="batched case instead (256 meshes, one launch)")
-    ap.add_argument("--only", default="", help="comma-separated row names to run (default: all)")
+    ap.add_argument("--only", default="", help="comma-separated row names to run (default: all; with --next: "
+                                                   "fps, clusters, iter_error, readers)")
     ap.add_argument("--next", action="store_true",
                     help="rows around the hot path instead (FPS, clusters, iteration error, mesh readers)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -297,33 +298,38 @@ def next_rows(args, L, P, G):
     nv = mesh.n_vertices()
     log(f"[next] mesh V={nv} ({time.time() - t:.1f}s)")
     rows = {}
+    only = set(filter(None, args.only.split(",")))   # fps, clusters, iter_error, readers
+
+    def want(k):
+        return not only or k in only
     # Farthest-point sampling.
     n_fps, n_cpu = 32, 5
     fps = {}
-    for name, prec in (("fps_f32", L.PREC_F32), ("fps_f64", L.PREC_F64)):
+    for name, prec in (("fps_f32", L.PREC_F32), ("fps_f64", L.PREC_F64)) if want("fps") else ():
         P.farthest_point_sampling_ptp_gpu(mesh, src, 3, 0.0, prec)
         smp, r, secs = P.farthest_point_sampling_ptp_gpu(mesh, src, n_fps, 0.0, prec)
         fps[name] = smp
         rows[name] = {"ms_per_sample": round(secs * 1e3 / (len(smp) - len(src)), 3), "samples": len(smp),
                       "radius": r, "seconds": round(secs, 4)}
         log(f"[next] {name}: {len(smp)} samples in {secs * 1e3:.1f} ms, radius {r:.6f}")
-    t0 = time.perf_counter()
-    cpu_smp, _ = om.fps(src, n_cpu)
-    cpu_s = time.perf_counter() - t0
-    rows["fps_f64"]["oracle_prefix_identical"] = bool(np.array_equal(fps["fps_f64"][:n_cpu], cpu_smp))
-    rows["fps_cpu_oracle"] = {"ms_per_sample": round(cpu_s * 1e3 / (len(cpu_smp) - len(src)), 1),
-                              "samples": len(cpu_smp), "cores": os.cpu_count()}
-    log(f"[next] fps oracle: {len(cpu_smp)} samples in {cpu_s:.1f}s; GPU FP64 prefix identical: "
-        f"{rows['fps_f64']['oracle_prefix_identical']}")
+    if want("fps") and not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu_smp, _ = om.fps(src, n_cpu)
+        cpu_s = time.perf_counter() - t0
+        rows["fps_f64"]["oracle_prefix_identical"] = bool(np.array_equal(fps["fps_f64"][:n_cpu], cpu_smp))
+        rows["fps_cpu_oracle"] = {"ms_per_sample": round(cpu_s * 1e3 / (len(cpu_smp) - len(src)), 1),
+                                  "samples": len(cpu_smp), "cores": os.cpu_count()}
+        log(f"[next] fps oracle: {len(cpu_smp)} samples in {cpu_s:.1f}s; GPU FP64 prefix identical: "
+            f"{rows['fps_f64']['oracle_prefix_identical']}")
     # Cluster labels.
     src16 = G.draw_sources(np.random.default_rng(args.seed + 2000), nv, 16)
-    for name, prec in (("clusters_f32", L.PREC_F32), ("clusters_f64", L.PREC_F64)):
+    for name, prec in (("clusters_f32", L.PREC_F32), ("clusters_f64", L.PREC_F64)) if want("clusters") else ():
         plan = P.Plan(mesh, src16, precision=prec, clusters=True)
         r = time_plan(plan, args.steps, args.warmup)
         labels = plan.clusters()
         plan.close()
         rows[name] = {"kernel_ms": round(r["kernel_ms"], 4), "sweeps": r["st"]["sweeps"]}
-        if prec == L.PREC_F64:
+        if prec == L.PREC_F64 and not args.no_cpu:
             t0 = time.perf_counter()
             dref, cref, nsw = om.ptp_clusters(src16, O.GPU_COAL)
             rows[name]["oracle_seconds"] = round(time.perf_counter() - t0, 2)
@@ -332,36 +338,39 @@ def next_rows(args, L, P, G):
         log(f"[next] {name}: {r['kernel_ms']:.2f} ms/field, sweeps {r['st']['sweeps']} {rows[name]}")
     # Iteration-error loop (one sweep per launch with a host read-back, as the
     # reference's instrumented test runs it).
-    exact, _, _ = P.ptp(mesh, src, precision=L.PREC_F64, tol=1e-6, uncapped=True)
-    t0 = time.perf_counter()
-    it, err, band = P.iter_error_parallel_toplesets_propagation_coalescence_gpu(mesh, src, exact, L.PREC_F32)
-    rows["iter_error_f32"] = {"seconds": round(time.perf_counter() - t0, 3), "sweeps": int(len(band)),
-                              "error_rows": int(len(it)), "last_error_pct": float(err[-1]) if len(err) else None}
-    log(f"[next] iter_error: {rows['iter_error_f32']}")
+    if want("iter_error"):
+        exact, _, _ = P.ptp(mesh, src, precision=L.PREC_F64, tol=1e-6, uncapped=True)
+        t0 = time.perf_counter()
+        it, err, band = P.iter_error_parallel_toplesets_propagation_coalescence_gpu(mesh, src, exact, L.PREC_F32)
+        rows["iter_error_f32"] = {"seconds": round(time.perf_counter() - t0, 3), "sweeps": int(len(band)),
+                                  "error_rows": int(len(it)),
+                                  "last_error_pct": float(err[-1]) if len(err) else None}
+        log(f"[next] iter_error: {rows['iter_error_f32']}")
     # Mesh readers.
-    with tempfile.TemporaryDirectory() as d:
-        off, ply = os.path.join(d, "m.off"), os.path.join(d, "m.ply")
-        with open(off, "w") as fh:
-            fh.write(f"OFF\n{nv} {len(faces)} 0\n")
-            np.savetxt(fh, xyz, fmt="%.17g")
-            np.savetxt(fh, np.hstack([np.full((len(faces), 1), 3), faces]), fmt="%d")
-        with open(ply, "wb") as fh:
-            fh.write((f"ply\nformat binary_little_endian 1.0\nelement vertex {nv}\nproperty double x\n"
-                      f"property double y\nproperty double z\nelement face {len(faces)}\n"
-                      f"property list uchar int vertex_indices\nend_header\n").encode())
-            xyz.astype("<f8").tofile(fh)
-            rec = np.zeros(len(faces), dtype=[("n", "u1"), ("v", "<i4", 3)])
-            rec["n"] = 3
-            rec["v"] = faces
-            rec.tofile(fh)
-        for name, path in (("read_off", off), ("read_ply", ply)):
-            t0 = time.perf_counter()
-            m2 = P.che.read(path)
-            secs = time.perf_counter() - t0
-            same = bool(np.array_equal(m2.gt(), xyz) and np.array_equal(m2.tables()[0].reshape(-1, 3), faces))
-            rows[name] = {"ms": round(secs * 1e3, 1), "MB_per_s": round(os.path.getsize(path) / secs / 1e6, 1),
-                          "file_MB": round(os.path.getsize(path) / 1e6, 1), "identical": same}
-            log(f"[next] {name}: {rows[name]}")
+    if want("readers"):
+        with tempfile.TemporaryDirectory() as d:
+            off, ply = os.path.join(d, "m.off"), os.path.join(d, "m.ply")
+            with open(off, "w") as fh:
+                fh.write(f"OFF\n{nv} {len(faces)} 0\n")
+                np.savetxt(fh, xyz, fmt="%.17g")
+                np.savetxt(fh, np.hstack([np.full((len(faces), 1), 3), faces]), fmt="%d")
+            with open(ply, "wb") as fh:
+                fh.write((f"ply\nformat binary_little_endian 1.0\nelement vertex {nv}\nproperty double x\n"
+                          f"property double y\nproperty double z\nelement face {len(faces)}\n"
+                          f"property list uchar int vertex_indices\nend_header\n").encode())
+                xyz.astype("<f8").tofile(fh)
+                rec = np.zeros(len(faces), dtype=[("n", "u1"), ("v", "<i4", 3)])
+                rec["n"] = 3
+                rec["v"] = faces
+                rec.tofile(fh)
+            for name, path in (("read_off", off), ("read_ply", ply)):
+                t0 = time.perf_counter()
+                m2 = P.che.read(path)
+                secs = time.perf_counter() - t0
+                same = bool(np.array_equal(m2.gt(), xyz) and np.array_equal(m2.tables()[0].reshape(-1, 3), faces))
+                rows[name] = {"ms": round(secs * 1e3, 1), "MB_per_s": round(os.path.getsize(path) / secs / 1e6, 1),
+                              "file_MB": round(os.path.getsize(path) / 1e6, 1), "identical": same}
+                log(f"[next] {name}: {rows[name]}")
     out = {"metric": "rows around the hot path: FPS ms/sample, clusters ms/field, iteration-error loop, mesh readers",
            "unit": "ms", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
            "config": {"workload": f"height field {args.n}x{args.n} ({nv} vertices), seed {args.seed}; FPS from "
@@ -434,6 +443,14 @@ def batch(args, L, P, G):
         for s in G.draw_sources(rng, len(xyz), int(rng.integers(1, 65))):
             jobs.append((k, int(s)))
     nvf = sum(meshes[m].n_vertices() for m, _ in jobs)
+
+    def max_neighbours(f):
+        e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+        e = np.unique(e, axis=0)
+        return int(np.bincount(e.ravel()).max())
+    # The kernel holds 8 slots in registers when every fan has at most 8
+    # neighbours (closed fans store one slot per neighbour), else 10.
+    ms_slots = 8 if max(max_neighbours(f) for _, f in raw) <= 8 else 10
     log(f"[batch] {len(meshes)} meshes, {len(jobs)} fields, {nvf} vertex-fields ({time.time() - t:.1f}s)")
     t = time.time()
     b = P.Batch(meshes, jobs, block=0)
@@ -467,7 +484,7 @@ def batch(args, L, P, G):
            "fields_per_s": round(len(jobs) / (kms * 1e-3), 1),
            "vertex_fields_per_s": round(nvf / (kms * 1e-3), 1),
            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None, "kernel": f"gp::k_batch<{st['block']}u>"},
+                        "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None, "kernel": f"gp::k_batch<{st['block']}u, {ms_slots}u>"},
            "spot_check_max_rel_err": worst}
     print(json.dumps(out), flush=True)
 
