@@ -39,7 +39,7 @@ struct BMesh {
 	const uint32_t *coff, *cid;
 	const float4 *cgeo;
 	const double4 *cgeo64;
-	const double4 *cx64;
+	const Edge64 *cx64;
 	const uint32_t *perm;   // Morton position -> original vertex id
 	float h, inv_h;
 	uint32_t nv, ms;        // ms: the longest fan (slots of a field's chunk)
@@ -72,7 +72,7 @@ struct FieldTables {
 	const uint32_t *cid;
 	const float4 *cgeo;
 	GeoRemap64 cgeo64;
-	const double4 *cx64;
+	const Edge64 *cx64;
 	float h, inv_h;
 };
 

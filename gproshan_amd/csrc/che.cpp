@@ -214,7 +214,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 	const size_t lanes = (size_t) L.coff[L.nchunks] * W;
 	L.cid.assign(std::max<size_t>(lanes, 1), 0);
 	if (f64) {
-		L.cx64.assign(std::max<size_t>(lanes, 1), Geo64{0, 0, 0, 0});
+		L.cx64.assign(std::max<size_t>(lanes, 1), Edge64{0, 0, 0});
 		L.cgeo.clear(); L.cgeo64.clear();
 	} else {
 		L.cgeo.assign(std::max<size_t>(lanes, 1), Geo32{0.f, 0.f, 0.f, 0.f});
@@ -241,7 +241,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 			const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
 			uint32_t id = a;
 			if (f64) {
-				L.cx64[at] = Geo64{ X0[0], X0[1], X0[2], 0 };
+				L.cx64[at] = Edge64{ X0[0], X0[1], X0[2] };
 			} else {
 				const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
 				Geo64 gd{ q0 * ih2, 0, 0, 0 };

@@ -56,7 +56,7 @@ struct FTables {
 	const uint32_t *cid;
 	const float4 *cgeo;
 	GeoRemap64 cgeo64;
-	const double4 *cx64;
+	const Edge64 *cx64;
 	float h, inv_h;
 };
 
@@ -234,7 +234,7 @@ __global__ void k_fps_append(BfsJob *job, uint32_t *samples, FState *st)
 struct FpsRun {
 	uint32_t *cid = nullptr, *midx = nullptr, *inv = nullptr, *samples = nullptr;
 	float4 *cgeo = nullptr;
-	double4 *cx64 = nullptr;
+	Edge64 *cx64 = nullptr;
 	void *d0 = nullptr, *d1 = nullptr;
 	FState *st = nullptr, *h_st = nullptr;
 	BfsJob *job = nullptr;
@@ -286,7 +286,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	const uint32_t nchunks = (nv + 63) / 64, npos = nchunks * 64;
 	const size_t lanes = (size_t) nchunks * S * 64;
 	if (hipMalloc(&r.cid, 4 * lanes) != hipSuccess || hipMalloc(&r.inv, 4ull * nv) != hipSuccess ||
-	    (F64 ? hipMalloc(&r.cx64, sizeof(double4) * lanes)
+	    (F64 ? hipMalloc(&r.cx64, sizeof(Edge64) * lanes)
 	         : (hipMalloc(&r.cgeo, sizeof(float4) * lanes) == hipSuccess ? hipMalloc(&r.midx, 4 * lanes)
 	                                                                      : hipErrorOutOfMemory)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "fps: table allocation failed");

@@ -61,6 +61,7 @@ constexpr uint32_t F64_FLAG = 0x80000000u;
 // without its repeat).
 struct Geo32 { float x, y, z, w; };
 struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
+struct Edge64 { double x, y, z; };     // FP64 edge vector a - v of a fan entry
 
 // FP64 update for triangle (k, k+1) of a fan when bit 31 of entry k is set:
 // its angle at the updated vertex is below ~5.7 deg or above ~174 deg
@@ -82,7 +83,7 @@ struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
 // slot 0 = the fan is closed and stored without its repeated first entry: the
 // triangle of the last real slot joins it to slot 0 (layouts built with wrap).
 // FP32 solves keep cgeo/cgeo64 (Geo32/Geo64 of entry k); FP64 solves keep
-// cx64 = the FP64 edge vector a - v (x, y, z, 0).
+// cx64 = the FP64 edge vector a - v (24 bytes per slot).
 constexpr uint32_t SLOT_EMPTY = 0x40000000u;
 constexpr uint32_t SLOT_WRAP = 0x20000000u;    // also the vertex count limit of a layout
 constexpr uint16_t LOCAL_OUT = 0xFFFF;
@@ -95,7 +96,7 @@ struct Layout {
 	std::vector<uint32_t> cid;
 	std::vector<Geo32> cgeo;
 	std::vector<Geo64> cgeo64;
-	std::vector<Geo64> cx64;
+	std::vector<Edge64> cx64;
 	std::vector<unsigned long long> fcum;  // FP64-flagged triangles of positions [0, pos)
 	std::vector<uint16_t> lidx;            // tile > 0: slot neighbour's index in the tile, or LOCAL_OUT
 };

@@ -54,7 +54,7 @@ struct PArgs {
 	const uint16_t *lidx;
 	const float4 *cgeo;
 	const double4 *cgeo64;
-	const double4 *cx64;   // unused (FP32 only); keeps relax helpers' interface
+	const Edge64 *cx64;    // unused (FP32 only); keeps relax helpers' interface
 	const float *pkey;
 	float *dist;
 	uint32_t *list;        // [4][npatch]

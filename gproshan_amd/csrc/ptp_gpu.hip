@@ -48,7 +48,8 @@ namespace gp {
 struct MasterLayout {
 	uint32_t *coff = nullptr, *cid = nullptr;
 	float4 *cgeo = nullptr;
-	double4 *cgeo64 = nullptr, *cx64 = nullptr;
+	double4 *cgeo64 = nullptr;
+	Edge64 *cx64 = nullptr;
 	uint32_t max_slots = 0;
 	VertexTables view() const { return VertexTables{ coff, cid, cgeo, cgeo64, cx64 }; }
 	~MasterLayout()
@@ -145,7 +146,7 @@ struct KArgs {
 	const uint32_t *cid;
 	const float4 *cgeo;
 	const double4 *cgeo64;
-	const double4 *cx64;
+	const Edge64 *cx64;
 	const unsigned long long *fcum;  // FP64-update triangles of positions [0, pos)
 	const uint32_t *order;     // position -> vertex; null when the mesh is numbered by rings
 	void *dist0, *dist1;
@@ -372,7 +373,8 @@ struct gp_plan {
 	uint32_t *d_limits = nullptr, *d_coff = nullptr, *d_cid = nullptr;
 	unsigned long long *d_fcum = nullptr;
 	float4 *d_cgeo = nullptr;
-	double4 *d_cgeo64 = nullptr, *d_cx64 = nullptr;
+	double4 *d_cgeo64 = nullptr;
+	gp::Edge64 *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
 	uint32_t *d_cl0 = nullptr, *d_cl1 = nullptr;   // cluster labels (opts.clusters)
 	uint32_t *d_inv = nullptr;                // device-built plans: original vertex -> position (NIL: unreached)
@@ -541,7 +543,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		return o;
 	};
 	if (f64) {
-		if ((rc = upload(p->d_cx64, as_d4(L.cx64)))) return rc;
+		if ((rc = upload(p->d_cx64, L.cx64))) return rc;
 	} else {
 		std::vector<float4> cg(L.cgeo.size());
 		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
@@ -578,7 +580,7 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 		};
 		if ((rc = upload(m->coff, L.coff)) || (rc = upload(m->cid, L.cid))) return rc;
 		if (f64) {
-			if ((rc = upload(m->cx64, as_d4(L.cx64)))) return rc;
+			if ((rc = upload(m->cx64, L.cx64))) return rc;
 		} else {
 			std::vector<float4> cg(L.cgeo.size());
 			std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
@@ -661,7 +663,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	    hipMalloc(&p->d_cid, 4 * std::max<size_t>(lanes, 1)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	if (f64) {
-		if (hipMalloc(&p->d_cx64, sizeof(double4) * std::max<size_t>(lanes, 1)) != hipSuccess)
+		if (hipMalloc(&p->d_cx64, sizeof(Edge64) * std::max<size_t>(lanes, 1)) != hipSuccess)
 			return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	} else if (hipMalloc(&p->d_cgeo, sizeof(float4) * std::max<size_t>(lanes, 1)) != hipSuccess ||
 	           hipMalloc(&p->d_cgeo64, sizeof(double4) * std::max<size_t>(lanes, 1)) != hipSuccess) {

@@ -24,7 +24,7 @@ __device__ __forceinline__ float tri_flagged(const A &a, size_t at, size_t at_n,
 	                                     (double) a.inv_h);
 }
 
-__device__ __forceinline__ double tri_f64(const double4 &xa, const double4 &xb, double ta, double tb)
+__device__ __forceinline__ double tri_f64(const Edge64 &xa, const Edge64 &xb, double ta, double tb)
 {
 	const double X0[3] = { xa.x, xa.y, xa.z };
 	const double X1[3] = { xb.x, xb.y, xb.z };
@@ -52,7 +52,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
            typename std::conditional<F64, double, float>::type best, uint32_t *bk_out = nullptr)
 {
 	using T = typename std::conditional<F64, double, float>::type;
-	using G = typename std::conditional<F64, double4, float4>::type;
+	using G = typename std::conditional<F64, Edge64, float4>::type;
 	const T old = best;
 	uint32_t bk = NO_TRI;
 	// Strictly lower wins; an equal value from an earlier triangle (evaluated
@@ -183,7 +183,8 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 struct VertexTables {
 	const uint32_t *coff, *cid;
 	const float4 *cgeo;
-	const double4 *cgeo64, *cx64;
+	const double4 *cgeo64;
+	const Edge64 *cx64;
 };
 
 // The mesh's vertex-major tables for the precision (built once per mesh) and
@@ -206,7 +207,7 @@ struct GeoRemap64 {
 template <bool F64, bool MIDX>
 __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, const uint32_t *nr_at, uint32_t npos,
                                 uint32_t S, VertexTables m, uint32_t *cid, float4 *cgeo, double4 *cgeo64,
-                                uint32_t *midx, double4 *cx64)
+                                uint32_t *midx, Edge64 *cx64)
 {
 	const uint32_t nr = *nr_at;
 	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
@@ -223,7 +224,7 @@ __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, con
 			const uint32_t id = real ? m.cid[mb + k] : 0u;
 			cid[at] = real ? (inv[id & SLOT_ID_MASK] | (id & ~SLOT_ID_MASK)) : (p | SLOT_EMPTY);
 			if constexpr (F64) {
-				cx64[at] = real ? m.cx64[mb + k] : make_double4(0, 0, 0, 0);
+				cx64[at] = real ? m.cx64[mb + k] : Edge64{ 0, 0, 0 };
 			} else {
 				cgeo[at] = real ? m.cgeo[mb + k] : make_float4(0.f, 0.f, 0.f, 0.f);
 				if constexpr (MIDX) midx[at] = real ? mb + k : 0u;
