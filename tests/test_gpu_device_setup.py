@@ -20,6 +20,14 @@ from gproshan_amd import meshgen as G
 
 pytestmark = pytest.mark.gpu
 
+def _two_parts():
+    """Two disjoint height fields, sources in the first only: the second part
+    is never reached (+inf)."""
+    x1, f1 = G.height_field(32, 5)
+    x2, f2 = G.height_field(24, 6)
+    return np.vstack([x1, x2 + np.array([3.0, 0.0, 0.0])]), np.vstack([f1, f2 + len(x1)])
+
+
 CASES = [
     ("ico5", lambda: G.icosphere(5, 0.01, 0), [0]),
     ("hf128_4src", lambda: G.height_field(128, 1), [777, 5000, 9000, 16000]),
@@ -28,6 +36,7 @@ CASES = [
     ("aniso128", lambda: G.anisotropic_grid(128, 20.0, 0.4, 3), [4000]),
     ("polar10_rim", lambda: G.polar_disk(30, 10, 1), [1 + 29 * 10 + 3]),
     ("polar24", lambda: G.polar_disk(40, 24, 0), [0]),
+    ("two_parts_one_reached", lambda: _two_parts(), [40]),
 ]
 
 

@@ -12,8 +12,19 @@ from gproshan_amd import meshgen as G
 
 pytestmark = pytest.mark.gpu
 
+def _two_parts():
+    """Two disjoint height fields: the first rounds reach only the part of the
+    first sample, so the farthest vertex is the other part's first (its
+    distance is +inf)."""
+    x1, f1 = G.height_field(32, 5)
+    x2, f2 = G.height_field(24, 6)
+    x2 = x2 + np.array([3.0, 0.0, 0.0])
+    return np.vstack([x1, x2]), np.vstack([f1, f2 + len(x1)])
+
+
 CASES = [
     ("ico4", lambda: G.icosphere(4, 0.01, 2), [0], 24, 0.0),
+    ("two_parts", _two_parts, [40], 12, 0.0),
     ("hf64", lambda: G.height_field(64, 9), [100], 16, 0.0),
     ("hf96_3init", lambda: G.height_field(96, 3), [5, 4000, 9000], 20, 0.0),
     ("hf64_radius", lambda: G.height_field(64, 11), [0], 200, 0.2),
