@@ -20,7 +20,10 @@ Rows:
   * patches: the patch schedule (LDS-iterated patches, active list) at tol 1e-6.
 value = ms per field of band_learned when it meets the 1e-4 bar against the
 FP64 fixed point, else of the fastest row that does; the JSON carries every
-row with its errors against both FP64 references.
+row with its errors against both FP64 references, the end-to-end time of a
+reference-schedule solve on a fresh mesh handle (plan with host- or
+device-built rings and tables, upload, run, result download) and the
+16-source field of the same mesh.
 """
 from __future__ import annotations
 
@@ -186,6 +189,38 @@ def main():
             f"{pred_wall_ms:.1f} ms host call with toplesets), "
             f"sweeps {st['sweeps']}, visits {st['patch_visits']}")
 
+    # End to end: a fresh mesh handle (its CHE built beforehand, outside the
+    # timer, as the reference builds its che), then plan (rings, renumbering,
+    # tables, upload), one run and the result download.
+    e2e = {}
+    if not only or "e2e" in only:
+        for name, kw in (("host_setup", {}), ("device_setup", {"device_setup": True})):
+            walls, kms = [], []
+            for _ in range(3):
+                C = P.che(xyz, faces)
+                t0 = time.perf_counter()
+                plan = P.Plan(C, src, precision=L.PREC_F32, **kw)
+                stx = plan.run()
+                plan.result()
+                plan.close()
+                walls.append((time.perf_counter() - t0) * 1e3)
+                kms.append(stx["kernel_ms"])
+                del C
+            e2e[name] = {"ms_min": round(min(walls), 2), "ms_mean": round(float(np.mean(walls)), 2),
+                         "kernel_ms": round(float(np.mean(kms)), 4)}
+            log(f"[bench] end to end, plain_ref_f32, {name}: {e2e[name]}")
+    # The 16-source field of the same mesh (one multi-source field).
+    src16 = G.draw_sources(np.random.default_rng(args.seed + 2000), len(xyz), 16)
+    multi = {}
+    if not only or "src16" in only:
+        for name, kw in (("plain_ref_f32", dict(precision=L.PREC_F32)),
+                         ("plain_uncapped_f32", dict(precision=L.PREC_F32, tol=1e-6, uncapped=True))):
+            plan = P.Plan(mesh, src16, **kw)
+            multi[name] = time_plan(plan, args.steps, args.warmup)
+            plan.close()
+            log(f"[bench] 16 sources, {name}: {multi[name]['kernel_ms']:.3f} ms/field, "
+                f"sweeps {multi[name]['st']['sweeps']}")
+
     errs = {}
     cpu = None
     if not args.no_cpu:
@@ -214,6 +249,12 @@ def main():
             errs[name] = {"max_vs_ref_schedule": e_sched[0], "mean_vs_ref_schedule": e_sched[1],
                           "max_vs_fixed_point": e_fixed[0], "mean_vs_fixed_point": e_fixed[1]}
             log(f"[bench] {name}: vs reference schedule {e_sched[0]:.2e}, vs fixed point {e_fixed[0]:.2e}")
+        if multi:
+            ref16, _ = om.ptp(src16, O.GPU_COAL, n_threads=ncores)
+            for name, r in multi.items():
+                e = rel_err(r["dist"], ref16)
+                r["max_vs_ref_schedule"], r["mean_vs_ref_schedule"] = e
+                log(f"[bench] 16 sources, {name}: vs reference schedule {e[0]:.2e}")
 
     # Headline: learned bands if they meet the bar, else the fastest row that does.
     def meets(name):
@@ -274,6 +315,15 @@ def main():
     }
     if dev_setup:
         out["device_setup"] = dev_setup
+    if e2e:
+        out["end_to_end"] = e2e
+    if multi:
+        out["sources16"] = {"sources": src16.tolist(),
+                            **{n: {"kernel_ms": round(r["kernel_ms"], 4), "sweeps": r["st"]["sweeps"],
+                                   "relaxations": r["st"]["relaxations"],
+                                   **({"max_vs_ref_schedule": r["max_vs_ref_schedule"]}
+                                      if "max_vs_ref_schedule" in r else {})}
+                               for n, r in multi.items()}}
     if cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
