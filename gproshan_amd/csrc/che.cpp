@@ -181,7 +181,6 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W,
                  uint32_t uniform_max, bool wrap)
 {
-	constexpr double SIN2_MIN = 0.01, COS_MIN = -0.97;
 	const double ih = 1.0 / m.mean_edge, ih2 = ih * ih;
 	if (W != 64 && W != 1) return fail(GP_ERR_INVALID, "layout: chunk width must be 64 or 1");
 	// Slots of a vertex: its fan entries, less the repeated first entry of a
@@ -259,7 +258,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 					gd.z = det * ih2 * ih2;
 					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
 					const double qq = q0 * q1;
-					if (!(qq > 0) || det < SIN2_MIN * qq || g < COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; nflag++; }
+					if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; nflag++; }
 				}
 				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w };
 				L.cgeo64[at] = gd;

@@ -72,6 +72,7 @@ struct Edge64 { double x, y, z; };     // FP64 edge vector a - v of a fan entry
 // the corners in 0.01 <= sin^2 < 0.03 or -0.97 <= cos < -0.9, which a wider
 // test flags, are within 2.4e-7); on a 1:100 anisotropic grid 89% of the
 // corners are flagged.
+constexpr double FLAG_SIN2_MIN = 0.01, FLAG_COS_MIN = -0.97;
 
 // ELL layout of a solve.  Positions (ring order: vertex order[pos], or pos
 // when the mesh is numbered by rings) are grouped 64 to a chunk, one lane per

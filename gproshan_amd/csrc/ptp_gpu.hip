@@ -42,19 +42,24 @@
 
 namespace gp {
 
-// Vertex-major slot tables of a mesh in its own numbering (closed fans without
-// their repeated first entry), the source that device-built plans gather their
-// ring-ordered tables from.  Built once per mesh and precision.
+// Vertex-major slot tables of a mesh in its own numbering (slot k of vertex v
+// at fan_off[v] + k; a closed fan stored without its repeated first entry
+// leaves its last fan entry unused), the source that device-built plans and
+// farthest-point sampling gather their ring-ordered tables from.  Built on the
+// device once per mesh and precision.
 struct MasterLayout {
-	uint32_t *coff = nullptr, *cid = nullptr;
+	uint32_t *cid = nullptr;
 	float4 *cgeo = nullptr;
 	double4 *cgeo64 = nullptr;
 	Edge64 *cx64 = nullptr;
 	uint32_t max_slots = 0;
-	VertexTables view() const { return VertexTables{ coff, cid, cgeo, cgeo64, cx64 }; }
+	VertexTables view(const DeviceMesh *dm) const
+	{
+		return VertexTables{ dm->fan_off, dm->closed, cid, cgeo, cgeo64, cx64 };
+	}
 	~MasterLayout()
 	{
-		for (void *p : { (void *) coff, (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
+		for (void *p : { (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
 	}
 };
 
@@ -560,7 +565,58 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	return GP_OK;
 }
 
-// Vertex-major slot tables of the mesh (built on the host once per mesh and
+// The mesh's vertex-major slot tables, one thread per vertex: the values of
+// the host layout builder (che.cpp, build_layout) -- the same FP64 operations
+// in the same order, contraction off, so the same bits and the same FP64
+// flags.  FP32 solves: neighbour id (F64_FLAG for a flagged corner, SLOT_WRAP
+// on slot 0 of a closed fan), FP32 and FP64 triangle geometry in units of the
+// mean edge; FP64 solves: neighbour id and the FP64 edge vector.
+template <bool F64>
+__global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, const uint8_t *closed, const double *gt,
+                                uint32_t nv, double ih2, uint32_t *cid, float4 *cgeo, double4 *cgeo64, Edge64 *cx64)
+{
+#pragma clang fp contract(off)
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+		const uint32_t s0 = fan_off[v], n = fan_off[v + 1] - s0;
+		const uint32_t ns = (n && closed[v]) ? n - 1 : n;
+		const double pv[3] = { gt[3 * (size_t) v], gt[3 * (size_t) v + 1], gt[3 * (size_t) v + 2] };
+		for (uint32_t k = 0; k < ns; k++) {
+			const size_t at = (size_t) s0 + k;
+			const uint32_t a = fan[s0 + k];
+			const double *pa = gt + 3 * (size_t) a;
+			const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+			uint32_t id = a;
+			if constexpr (F64) {
+				cx64[at] = Edge64{ X0[0], X0[1], X0[2] };
+			} else {
+				const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+				double4 gd = make_double4(q0 * ih2, 0, 0, 0);
+				if (k + 1 < n) {
+					const double *pb = gt + 3 * (size_t) fan[s0 + k + 1];
+					const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
+					const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
+					const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
+					const double cx = X0[1] * X1[2] - X0[2] * X1[1];
+					const double cy = X0[2] * X1[0] - X0[0] * X1[2];
+					const double cz = X0[0] * X1[1] - X0[1] * X1[0];
+					const double det = cx * cx + cy * cy + cz * cz;
+					const double E[3] = { pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2] };
+					gd.y = g * ih2;
+					gd.z = det * ih2 * ih2;
+					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
+					const double qq = q0 * q1;
+					if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq)) id |= F64_FLAG;
+				}
+				cgeo[at] = make_float4((float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w);
+				cgeo64[at] = gd;
+			}
+			if (k == 0 && ns < n) id |= SLOT_WRAP;
+			cid[at] = id;
+		}
+	}
+}
+
+// Vertex-major slot tables of the mesh (built on the device once per mesh and
 // precision, kept with the device mesh) and the longest fan's slot count.
 int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 {
@@ -569,26 +625,31 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 	DeviceMesh *dm = mesh.dev;
 	MasterLayout *ml = dm->master[f64 ? 1 : 0];
 	if (!ml) {
-		Layout L;
-		if ((rc = build_layout(mesh, nullptr, mesh.nv, f64, L, 0, 1, 0, true))) return rc;
+		if (mesh.nv >= SLOT_WRAP) return fail(GP_ERR_CAPACITY, "layout: vertex ids must stay below 2^29");
 		auto m = std::make_unique<MasterLayout>();
-		m->max_slots = L.max_slots;
-		auto as_d4 = [](const std::vector<Geo64> &v) {
-			std::vector<double4> o(v.size());
-			std::memcpy(o.data(), v.data(), sizeof(double4) * v.size());
-			return o;
-		};
-		if ((rc = upload(m->coff, L.coff)) || (rc = upload(m->cid, L.cid))) return rc;
-		if (f64) {
-			if ((rc = upload(m->cx64, L.cx64))) return rc;
-		} else {
-			std::vector<float4> cg(L.cgeo.size());
-			std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
-			if ((rc = upload(m->cgeo, cg)) || (rc = upload(m->cgeo64, as_d4(L.cgeo64)))) return rc;
+		for (uint32_t v = 0; v < mesh.nv; v++) {
+			const uint32_t n = mesh.fan_off[v + 1] - mesh.fan_off[v];
+			m->max_slots = std::max(m->max_slots, (n && mesh.fan_closed[v]) ? n - 1 : n);
 		}
+		const size_t nf = std::max<size_t>(dm->nfan, 1);
+		if (hipMalloc(&m->cid, 4 * nf) != hipSuccess ||
+		    (f64 ? hipMalloc(&m->cx64, sizeof(Edge64) * nf)
+		         : (hipMalloc(&m->cgeo, sizeof(float4) * nf) == hipSuccess ? hipMalloc(&m->cgeo64, sizeof(double4) * nf)
+		                                                                  : hipErrorOutOfMemory)) != hipSuccess)
+			return fail(GP_ERR_NOMEM, "layout: device allocation failed");
+		const double ih = 1.0 / mesh.mean_edge, ih2 = ih * ih;
+		const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(2048, (mesh.nv + 255) / 256));
+		if (f64)
+			k_master_tables<true><<<g, 256>>>(dm->fan_off, dm->fan, dm->closed, dm->pos, mesh.nv, ih2, m->cid, nullptr,
+			                                  nullptr, m->cx64);
+		else
+			k_master_tables<false><<<g, 256>>>(dm->fan_off, dm->fan, dm->closed, dm->pos, mesh.nv, ih2, m->cid,
+			                                   m->cgeo, m->cgeo64, nullptr);
+		GP_HIP(hipGetLastError());
+		GP_HIP(hipDeviceSynchronize());
 		ml = dm->master[f64 ? 1 : 0] = m.release();
 	}
-	out = ml->view();
+	out = ml->view(dm);
 	max_slots = ml->max_slots;
 	return GP_OK;
 }

@@ -178,10 +178,14 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 	return best;
 }
 
-// Vertex-major slot tables of a mesh in its own numbering (closed fans
-// without their repeated first entry): slot k of vertex v at coff[v] + k.
+// Vertex-major slot tables of a mesh in its own numbering: slot k of vertex v
+// at coff[v] + k (coff = the fan offsets), coff[v + 1] - coff[v] - 1 slots for
+// a closed fan (stored without its repeated first entry), else
+// coff[v + 1] - coff[v].
 struct VertexTables {
-	const uint32_t *coff, *cid;
+	const uint32_t *coff;
+	const uint8_t *closed;
+	const uint32_t *cid;
 	const float4 *cgeo;
 	const double4 *cgeo64;
 	const Edge64 *cx64;
@@ -216,7 +220,8 @@ __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, con
 		if (p < nr) {
 			const uint32_t v = sorted[p];
 			mb = m.coff[v];
-			n = m.coff[v + 1] - mb;
+			const uint32_t nf = m.coff[v + 1] - mb;
+			n = (nf && m.closed[v]) ? nf - 1 : nf;
 		}
 		for (uint32_t k = 0; k < S; k++) {
 			const size_t at = base + 64 * (size_t) k;
