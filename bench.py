@@ -432,15 +432,20 @@ def next_rows(args, L, P, G):
 def stress(args, L, P, G):
     """2048 x 2048 grid stretched 1:100 in x, jitter U(-0.4, 0.4) cells, 4
     sources: FP32 with the FP64 fallback vs all-FP64 GPU vs the FP64 oracle
-    (the reference schedule; the oracle runs on the host cores)."""
+    (the reference schedule; the oracle runs on the host cores).  The FP32
+    row reads the geometry table the plan picks (the FP64 one here: most
+    corners are flagged); f32_fallback_geo32 forces the FP32 table (same
+    bits, checked)."""
     t = time.time()
     xyz, faces = G.anisotropic_grid(2048, 100.0, 0.4, args.seed)
     src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), 4)
     mesh = P.che(xyz, faces)
     log(f"[stress] V={mesh.n_vertices()} sources={src.tolist()} ({time.time() - t:.1f}s)")
     rows, traces = {}, {}
-    for name, prec in (("f32_fallback", L.PREC_F32), ("f64", L.PREC_F64)):
-        plan = P.Plan(mesh, src, precision=prec)
+    for name, kw in (("f32_fallback", dict(precision=L.PREC_F32)),
+                     ("f32_fallback_geo32", dict(precision=L.PREC_F32, fp32_geometry=L.GEO_FP32)),
+                     ("f64", dict(precision=L.PREC_F64))):
+        plan = P.Plan(mesh, src, **kw)
         rows[name] = time_plan(plan, args.steps, args.warmup)
         traces[name] = plan.trace()
         plan.close()
@@ -461,7 +466,10 @@ def stress(args, L, P, G):
            "f32_schedule_matches_f64": first_diff is None, "f32_first_differing_sweep": first_diff,
            "config": {"workload": "anisotropic grid 2048x2048 (4194304 vertices), stretch 1:100, jitter 0.4, "
                                   f"4 sources, seed {args.seed}", "sources": src.tolist()},
-           "oracle": {"sweeps": nsw, "seconds": round(cpu_s, 2), "cores": os.cpu_count()}, "rows": {}}
+           "oracle": {"sweeps": nsw, "seconds": round(cpu_s, 2), "cores": os.cpu_count()},
+           "geometry_tables_identical": bool(np.array_equal(rows["f32_fallback"]["dist"].view(np.uint64),
+                                                            rows["f32_fallback_geo32"]["dist"].view(np.uint64))),
+           "rows": {}}
     for name, r in rows.items():
         mx, mn = rel_err(r["dist"], ref)
         st = r["st"]

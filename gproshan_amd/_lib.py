@@ -22,6 +22,7 @@ PREC_F32, PREC_F64 = 0, 1
 RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
 SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND, SCHED_RING_BAND, SCHED_PATCHES = 0, 1, 2, 3, 4
 LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
+GEO_AUTO, GEO_FP32, GEO_FP64 = 0, 1, 2
 
 # Every symbol include/gproshan_ptp.h declares.
 EXPORTS = [
@@ -53,7 +54,7 @@ class PtpOpts(C.Structure):
                 ("coalesce", C.c_int), ("tol", C.c_double), ("max_sweeps", C.c_uint32),
                 ("block", C.c_uint32), ("grid", C.c_uint32), ("launch_mode", C.c_int),
                 ("band_width", C.c_double), ("local_iters", C.c_uint32), ("uncapped", C.c_int), ("clusters", C.c_int),
-                ("device_setup", C.c_int)]
+                ("device_setup", C.c_int), ("fp32_geometry", C.c_int)]
 
 
 class PtpStats(C.Structure):

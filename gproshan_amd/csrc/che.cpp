@@ -270,6 +270,8 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 	}
 	for (uint32_t pos = 0; pos < nr; pos++) L.fcum[pos + 1] = L.fcum[pos] + flagged_at[pos];
 	L.n_flagged = L.fcum[nr];
+	L.n_slots = 0;
+	for (uint32_t pos = 0; pos < nr; pos++) L.n_slots += slots(order ? order[pos] : pos);
 	L.lidx.clear();
 	if (tile) {
 		if (order) return fail(GP_ERR_INVALID, "layout: tiles need a mesh numbered by positions");

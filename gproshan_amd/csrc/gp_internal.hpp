@@ -93,6 +93,7 @@ struct Layout {
 	uint32_t nchunks = 0, max_slots = 0;
 	uint32_t uniform = 0;                  // every chunk has this many slots (0: see coff)
 	unsigned long long n_flagged = 0;
+	unsigned long long n_slots = 0;        // real slots of the positions
 	std::vector<uint32_t> coff;
 	std::vector<uint32_t> cid;
 	std::vector<Geo32> cgeo;

@@ -166,7 +166,7 @@ struct KArgs {
 
 // Relaxes the window [start, end) (positions); returns the wave's count of
 // non-converged vertices among positions [start, cend).
-template <bool F64, bool CL, uint32_t MS>
+template <bool F64, bool CL, uint32_t MS, bool G64>
 __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
                                                const typename std::conditional<F64, double, float>::type *od,
                                                typename std::conditional<F64, double, float>::type *nd)
@@ -197,7 +197,7 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 			const T old = od[v];
 			const size_t base = (size_t) cb * 64 + lane;
 			uint32_t bk = NO_TRI;
-			const T best = relax_lane<F64, 64, true, CL, MS>(a, base, S, od, old, &bk);
+			const T best = relax_lane<F64, 64, true, CL, MS, G64>(a, base, S, od, old, &bk);
 			nd[v] = best;
 			if constexpr (CL) {
 				// The label of the nearer of the winning triangle's two other
@@ -222,7 +222,7 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 }
 
 // Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
-template <bool F64, bool CL, uint32_t MS>
+template <bool F64, bool CL, uint32_t MS, bool G64 = false>
 __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 {
 	using T = typename std::conditional<F64, double, float>::type;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	const uint32_t start = cur.ws, cend = cur.wc, end = cur.we;
 	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
 	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = sweep_body<F64, CL, MS>(a, start, end, cend, od, nd);
+	const uint32_t bad = sweep_body<F64, CL, MS, G64>(a, start, end, cend, od, nd);
 	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
 	if (lead) {
 		Slot *sl = &st->slot[k4];
@@ -372,6 +372,7 @@ struct gp_plan {
 	uint32_t n = 0, nv_orig = 0, nl = 0, ns = 0, nr = 0, max_iter = 0, grid = 0, block = 256;
 	uint32_t nchunks = 0, max_slots = 0, sb = 0;
 	bool ms8 = false;                        // every fan fits 8 slots: the 8-slot kernels
+	bool g64 = false;                        // FP32 sweeps read the FP64 geometry table (GP_GEO_FP64)
 	unsigned long long n_flagged = 0;
 	gp::PatchPlan *pp = nullptr;             // patch / band schedules
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
@@ -432,7 +433,9 @@ static int sweep_grid(gp_plan *p, bool f64)
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	p->ms8 = p->max_slots <= 8;
 	const void *kfn = f64 ? (p->ms8 ? (const void *) k_sweep<true, false, 8> : (const void *) k_sweep<true, false, MAXS>)
-	                      : (p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>);
+	                : p->g64 ? (p->ms8 ? (const void *) k_sweep<false, false, 8, true>
+	                                   : (const void *) k_sweep<false, false, MAXS, true>)
+	                         : (p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>);
 	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, 256, 0));
 	const uint32_t gmax = (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	p->grid = p->opts.grid ? p->opts.grid : std::max<uint32_t>(1, std::min<uint32_t>(gmax, (p->nchunks + 3) / 4));
@@ -531,6 +534,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	p->max_slots = L.max_slots;
 	p->sb = L.uniform;
 	p->n_flagged = L.n_flagged;
+	p->g64 = !f64 && (opts.fp32_geometry == GP_GEO_FP64 ||
+	                  (opts.fp32_geometry == GP_GEO_AUTO && 4 * L.n_flagged > L.n_slots));
 	// The drivers cap the loop at 2*|limits| sweeps (src/geodesics_ptp.cpp:66-67);
 	// the uncapped loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248)
 	// has no cap, so a generous one stands in as a guard.
@@ -718,6 +723,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	p->max_slots = S;
 	p->sb = S;
 	p->n_flagged = 0;
+	p->g64 = !f64 && opts.fp32_geometry == GP_GEO_FP64;
 	const size_t lanes = (size_t) p->nchunks * S * 64;
 	if (hipMalloc(&p->d_limits, 4ull * nl) != hipSuccess || hipMalloc(&p->d_src, 4ull * ns) != hipSuccess ||
 	    hipMalloc(&p->d_inv, 4ull * mesh->nv) != hipSuccess || hipMalloc(&p->d_coff, 4) != hipSuccess ||
@@ -775,7 +781,12 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 			else if (p->ms8) k_sweep<true, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 			else k_sweep<true, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		} else {
-			if (cl && p->ms8) k_sweep<false, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			if (p->g64) {
+				if (cl && p->ms8) k_sweep<false, true, 8, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+				else if (cl) k_sweep<false, true, MAXS, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+				else if (p->ms8) k_sweep<false, false, 8, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+				else k_sweep<false, false, MAXS, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
+			} else if (cl && p->ms8) k_sweep<false, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 			else if (cl) k_sweep<false, true, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 			else if (p->ms8) k_sweep<false, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 			else k_sweep<false, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);

@@ -44,9 +44,16 @@ __device__ __forceinline__ double tri_f64(const Edge64 &xa, const Edge64 &xb, do
 // CL: also report in *bk the triangle that set the new value -- the first in
 // fan order among those reaching the minimum, NO_TRI if the value did not
 // drop -- for the cluster label.
+//
+// G64 (FP32, register path): the slots' FP64 geometry is read instead of the
+// FP32 one; an unflagged triangle runs the FP32 update on its FP32 rounding
+// (the FP32 table holds exactly that rounding), a flagged one the FP64 update
+// on the FP64 values (what the deferred path loads), so the value is the same
+// bits with one 32-byte read per slot and no second pass.
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
 
-template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, uint32_t MS = MAXS, typename A>
+template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, uint32_t MS = MAXS, bool G64 = false,
+          typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
 relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
            typename std::conditional<F64, double, float>::type best, uint32_t *bk_out = nullptr)
@@ -69,6 +76,42 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 	const G *geo;
 	if constexpr (F64) geo = a.cx64; else geo = a.cgeo;
 	uint32_t fmask = 0;
+	if constexpr (!F64 && G64) {
+		if (S <= MS) {
+			uint32_t id[MS];
+			double4 g[MS];
+			T t[MS];
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) {
+				id[k] = SLOT_EMPTY;
+				if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = a.cgeo64[base + ST * k]; }
+			}
+			const bool wrap = (id[0] & SLOT_WRAP) != 0u;
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++)
+				t[k] = (k < S) ? od[id[k] & SLOT_ID_MASK] : (T) INFINITY;
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) {
+				if (k < S) {
+					const bool nr = k + 1 < MS && k + 1 < S && !(id[(k + 1) % MS] & SLOT_EMPTY);
+					const bool tri = !(id[k] & SLOT_EMPTY) && (nr || wrap);
+					const bool fl = (id[k] & F64_FLAG) != 0u;
+					const T tb = nr ? t[(k + 1) % MS] : t[0];
+					const double q1 = nr ? g[(k + 1) % MS].x : g[0].x;
+					T p = (T) INFINITY;
+					if (tri && fl)
+						p = (float) update_stable<double>(g[k].x, q1, g[k].y, g[k].z, g[k].w, (double) t[k], (double) tb,
+						                                  (double) a.h, (double) a.inv_h);
+					if (tri && !fl)
+						p = update_stable<float>((float) g[k].x, (float) q1, (float) g[k].y, (float) g[k].z,
+						                         (float) g[k].w, t[k], tb, a.h, a.inv_h);
+					take(p, k);
+				}
+			}
+			if constexpr (CL) *bk_out = bk;
+			return best;
+		}
+	}
 	if (S <= MS) {
 		uint32_t id[MS];
 		G g[MS];

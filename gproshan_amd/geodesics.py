@@ -223,7 +223,7 @@ class Plan:
     def __init__(self, mesh, sources, dhat=None, **kw):
         self.mesh = mesh
         self._src = _u32(sources)
-        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup") else v) for k, v in kw.items()})
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup", "fp32_geometry") else v) for k, v in kw.items()})
         h = C.c_void_p()
         if dhat is None:
             check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))

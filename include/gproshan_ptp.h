@@ -79,6 +79,21 @@ enum gp_launch_mode {
 	GP_LAUNCH_STREAM = 1   /* the same kernels launched one by one on the stream                 */
 };
 
+/* Geometry table the FP32 ring sweeps read.  Both forms give the same bits:
+ * the FP32 triangle updates use the FP32 rounding of the FP64 geometry and the
+ * flagged (near-degenerate or obtuse) corners the FP64 geometry itself.
+ * GP_GEO_FP32 reads 16 bytes per slot and loads the FP64 geometry of flagged
+ * corners on the side (the first per vertex with the neighbour distances, the
+ * others after the fan loop); GP_GEO_FP64 reads the 32-byte FP64 geometry of
+ * every slot and runs each triangle once, which is faster when many corners
+ * are flagged.  GP_GEO_AUTO picks GP_GEO_FP64 for host-built plans with more
+ * than a quarter of their triangles flagged. */
+enum gp_fp32_geometry {
+	GP_GEO_AUTO = 0,
+	GP_GEO_FP32 = 1,
+	GP_GEO_FP64 = 2
+};
+
 typedef struct gp_ptp_opts {
 	int precision;      /* gp_precision                                          */
 	int result_buffer;  /* gp_result_buffer                                      */
@@ -109,6 +124,7 @@ typedef struct gp_ptp_opts {
 	                       f64_updates is not counted); falls back to the host
 	                       build when a fan is longer than the sweep kernels'
 	                       register-held slots                                */
+	int fp32_geometry;  /* gp_fp32_geometry (precision F32, ring schedules)     */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {

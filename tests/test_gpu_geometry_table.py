@@ -1,0 +1,58 @@
+"""The two geometry tables of the FP32 ring sweeps (gp_ptp_opts.fp32_geometry).
+
+GP_GEO_FP32 reads each slot's FP32 triangle geometry and loads the FP64
+geometry of flagged corners on the side; GP_GEO_FP64 reads the FP64 geometry
+of every slot and rounds it to FP32 for the unflagged triangles.  The FP32
+table holds exactly that rounding, so both must give the same bits: distances,
+schedule trace and cluster labels, on meshes with no flagged corner, with most
+corners flagged (a grid stretched 1:100) and with fans longer than the
+register-held slots (the streaming loop, shared by both).
+"""
+import numpy as np
+import pytest
+
+from gproshan_amd import _lib as L
+from gproshan_amd import geodesics as P
+from gproshan_amd import meshgen as G
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ("hf128", lambda: G.height_field(128, 1), [777, 9000]),
+    ("aniso128_1to20", lambda: G.anisotropic_grid(128, 20.0, 0.4, 3), [4000]),
+    ("aniso96_1to100", lambda: G.anisotropic_grid(96, 100.0, 0.4, 5), [10, 5000, 9000]),
+    ("ico5", lambda: G.icosphere(5, 0.01, 0), [0]),
+    ("polar24", lambda: G.polar_disk(30, 24, 0), [0]),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request):
+    name, gen, src = request.param
+    xyz, f = gen()
+    print(f"\n[case] {name} V={len(xyz)} sources={src}", flush=True)
+    return name, np.array(src, np.uint32), P.che(xyz, f)
+
+
+def _solve(C, src, geo, **kw):
+    plan = P.Plan(C, src, precision=L.PREC_F32, fp32_geometry=geo, **kw)
+    st = plan.run()
+    d = plan.result()
+    tr = plan.trace()
+    cl = plan.clusters() if kw.get("clusters") else None
+    plan.close()
+    return d, tr, cl, st
+
+
+@pytest.mark.parametrize("sched", [dict(), dict(uncapped=True, tol=1e-6), dict(clusters=True)],
+                         ids=["ref", "uncapped", "clusters"])
+def test_geometry_tables_identical(case, sched):
+    name, src, C = case
+    d32, tr32, cl32, st32 = _solve(C, src, L.GEO_FP32, **sched)
+    d64, tr64, cl64, st64 = _solve(C, src, L.GEO_FP64, **sched)
+    print(f"[geo] {name} {sched}: sweeps {st32['sweeps']}, flagged updates {st32['f64_updates']} "
+          f"of {st32['relaxations']} relaxations", flush=True)
+    assert np.array_equal(d32.view(np.uint64), d64.view(np.uint64))
+    assert np.array_equal(tr32, tr64)
+    if cl32 is not None:
+        assert np.array_equal(cl32, cl64)
