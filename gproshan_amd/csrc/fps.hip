@@ -62,7 +62,7 @@ struct FTables {
 
 struct FArgs {
 	FTables t;
-	uint32_t S;                             // slots per chunk
+	const uint32_t *coff;                   // slot rows of chunk c: coff[c] .. coff[c + 1]
 	const uint32_t *limits, *nl;            // rings of the current samples (device)
 	void *dist0, *dist1;                    // by position
 	FState *st;
@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
 		bool lbad = false;
 		if (pos >= start && pos < end) {
 			const T old = od[pos];
-			const T best = relax_lane<F64, 64, false, false, MS>(a.t, (size_t) c * a.S * 64 + lane, a.S, od, old);
+			const uint32_t cb = a.coff[c];
+			const T best = relax_lane<F64, 64, false, false, MS>(a.t, (size_t) cb * 64 + lane, a.coff[c + 1] - cb, od, old);
 			nd[pos] = best;
 			if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
 		}
@@ -281,8 +282,10 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	int rc = master_tables(m, F64, mt, S);
 	if (rc) return rc;
 	FpsRun r;
-	// Ring-ordered chunked tables of a round, S slots per chunk (the longest
-	// fan; fans past the register-held slots take the streaming loop).
+	// Ring-ordered chunked tables of a round: each chunk as many slot rows as
+	// its longest fan (each ring sorted longest fans first), room for the
+	// longest fan in every chunk; fans past the register-held slots take the
+	// streaming loop.
 	const uint32_t nchunks = (nv + 63) / 64, npos = nchunks * 64;
 	const size_t lanes = (size_t) nchunks * S * 64;
 	if (hipMalloc(&r.cid, 4 * lanes) != hipSuccess || hipMalloc(&r.inv, 4ull * nv) != hipSuccess ||
@@ -305,7 +308,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	FArgs a{};
 	a.t.cid = r.cid; a.t.cgeo = r.cgeo; a.t.cgeo64 = GeoRemap64{ mt.cgeo64, r.midx }; a.t.cx64 = r.cx64;
 	a.t.h = m.dev->h; a.t.inv_h = m.dev->inv_h;
-	a.S = S; a.limits = lv.limits; a.nl = lv.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
+	a.coff = lv.coff; a.limits = lv.limits; a.nl = lv.nl; a.dist0 = r.d0; a.dist1 = r.d1; a.st = r.st;
 	a.tol = GP_PTP_TOL;
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
@@ -348,10 +351,10 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		k_fps_inv_clear<<<ig, 256, 0, r.stream>>>(r.inv, nv);
 		k_fps_inv<<<ig, 256, 0, r.stream>>>(lv.sorted, r.st, r.inv);
 		if (F64)
-			k_gather_tables<true, false><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, mt, r.cid,
+			k_gather_tables<true, false><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, lv.coff, mt, r.cid,
 			                                                        nullptr, nullptr, nullptr, r.cx64);
 		else
-			k_gather_tables<false, true><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, mt, r.cid,
+			k_gather_tables<false, true><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, lv.coff, mt, r.cid,
 			                                                        r.cgeo, nullptr, r.midx, nullptr);
 		GP_HIP(hipGetLastError());
 		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));

@@ -128,22 +128,43 @@ __global__ __launch_bounds__(256) void k_lv_step(const uint32_t *fan_off, const 
 	}
 }
 
-// Counting sort of the reached vertices by level: histogram (and the largest
-// level), exclusive scan into limits (one workgroup), scatter into sorted[].
-// Global atomics on a few hundred level counters serialise, so each workgroup
-// counts a block of SORT_BLOCK vertices in LDS bins (levels lo .. lo + SORT_BINS
-// - 1 of the block; levels past them, rare, count globally) and adds or
-// reserves one global count per level present in the block.
-constexpr uint32_t SORT_BLOCK = 8192, SORT_BINS = 4096, SORT_T = 1024;
+// Counting sort of the reached vertices by level and, inside a level, by slot
+// count (longest fans first), so that a 64-position chunk of a ring holds fans
+// of one length: histogram of the keys (and the largest level), exclusive scan
+// into the cursors and limits (one workgroup), scatter into sorted[].  The
+// order inside a level is free (a sweep relaxes whole rings from the previous
+// buffer).  Global atomics on a few thousand counters serialise, so each
+// workgroup counts a block of SORT_BLOCK vertices in LDS bins (keys lo .. lo +
+// SORT_BINS - 1 of the block; keys past them, rare, count globally) and adds
+// or reserves one global count per key present in the block.
+constexpr uint32_t SORT_BLOCK = 8192, SORT_BINS = 12288, SORT_T = 1024;
 
-// Smallest and largest reached level of the block [b0, b1) (every thread).
-__device__ void block_level_range(const uint32_t *level, uint32_t b0, uint32_t b1, uint32_t *red, uint32_t &lo,
-                                  uint32_t &hi)
+// Sort key of vertex v: level * KS + slot class (KS or more slots first, then
+// KS - 1, ..., 1), NIL when unreached.
+__device__ __forceinline__ uint32_t sort_key(const uint32_t *level, const uint32_t *fan_off, const uint8_t *closed,
+                                             uint32_t v)
+{
+	const uint32_t l = level[v];
+	if (l == GP_NIL) return GP_NIL;
+	const uint32_t n = fan_off[v + 1] - fan_off[v];
+	const uint32_t sl = (n && closed[v]) ? n - 1 : n;
+	return l * LEVEL_KEYS + (LEVEL_KEYS - min(max(sl, 1u), LEVEL_KEYS));
+}
+
+struct KeySrc {
+	const uint32_t *level, *fan_off;
+	const uint8_t *closed;
+	__device__ uint32_t operator()(uint32_t v) const { return sort_key(level, fan_off, closed, v); }
+};
+
+// Smallest and largest key of the block [b0, b1) (every thread).
+__device__ void block_key_range(const KeySrc &key, uint32_t b0, uint32_t b1, uint32_t *red, uint32_t &lo,
+                                uint32_t &hi)
 {
 	lo = GP_NIL; hi = 0;
 	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
-		const uint32_t l = level[v];
-		if (l != GP_NIL) { lo = min(lo, l); hi = max(hi, l); }
+		const uint32_t k = key(v);
+		if (k != GP_NIL) { lo = min(lo, k); hi = max(hi, k); }
 	}
 	for (uint32_t o = 32; o > 0; o >>= 1) {
 		lo = min(lo, (uint32_t) __shfl_xor(lo, o));
@@ -156,41 +177,43 @@ __device__ void block_level_range(const uint32_t *level, uint32_t b0, uint32_t b
 	for (uint32_t q = 0; q < SORT_T / 64; q++) { lo = min(lo, red[q]); hi = max(hi, red[16 + q]); }
 }
 
-// LDS counts of the block's levels lo .. lo + SORT_BINS - 1; the others are
+// LDS counts of the block's keys lo .. lo + SORT_BINS - 1; the others are
 // counted in glob (when given).
-__device__ void block_level_counts(const uint32_t *level, uint32_t b0, uint32_t b1, uint32_t lo, uint32_t *bins,
-                                   uint32_t *glob)
+__device__ void block_key_counts(const KeySrc &key, uint32_t b0, uint32_t b1, uint32_t lo, uint32_t *bins,
+                                 uint32_t *glob)
 {
 	for (uint32_t q = threadIdx.x; q < SORT_BINS; q += SORT_T) bins[q] = 0;
 	__syncthreads();
 	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
-		const uint32_t l = level[v];
-		if (l == GP_NIL) continue;
-		if (l - lo < SORT_BINS) atomicAdd(&bins[l - lo], 1u);
-		else if (glob) atomicAdd(&glob[l], 1u);
+		const uint32_t k = key(v);
+		if (k == GP_NIL) continue;
+		if (k - lo < SORT_BINS) atomicAdd(&bins[k - lo], 1u);
+		else if (glob) atomicAdd(&glob[k], 1u);
 	}
 	__syncthreads();
 }
 
-__global__ __launch_bounds__(SORT_T) void k_lv_hist(const uint32_t *level, uint32_t n, uint32_t *hist, LevelState *ls)
+__global__ __launch_bounds__(SORT_T) void k_lv_hist(KeySrc key, uint32_t n, uint32_t *hist, LevelState *ls)
 {
 	__shared__ uint32_t bins[SORT_BINS];
 	__shared__ uint32_t red[32];
 	const uint32_t b0 = blockIdx.x * SORT_BLOCK, b1 = min(n, b0 + SORT_BLOCK);
 	uint32_t lo, hi;
-	block_level_range(level, b0, b1, red, lo, hi);
+	block_key_range(key, b0, b1, red, lo, hi);
 	if (lo == GP_NIL) return;   // nothing reached in the block (uniform)
-	block_level_counts(level, b0, b1, lo, bins, hist);
+	block_key_counts(key, b0, b1, lo, bins, hist);
 	const uint32_t range = min(SORT_BINS, hi - lo + 1);
 	for (uint32_t q = threadIdx.x; q < range; q += SORT_T)
 		if (bins[q]) atomicAdd(&hist[lo + q], bins[q]);
-	if (threadIdx.x == 0) atomicMax(&ls->max_level, hi);
+	if (threadIdx.x == 0) atomicMax(&ls->max_level, hi / LEVEL_KEYS);
 }
 
+// Exclusive scan of the key counts into the scatter cursors; limits[l] = the
+// first position of level l, limits[max_level + 1] = the reached count.
 __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limits, uint32_t *nl, const LevelState *ls)
 {
 	__shared__ uint32_t part[1024];
-	const uint32_t nb = ls->max_level + 1;   // levels 0 .. max_level
+	const uint32_t nlev = ls->max_level + 1, nb = nlev * LEVEL_KEYS;
 	uint32_t carry = 0;
 	for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
 		const uint32_t q = b0 + threadIdx.x;
@@ -203,38 +226,82 @@ __global__ __launch_bounds__(1024) void k_lv_scan(uint32_t *hist, uint32_t *limi
 			part[threadIdx.x] += y;
 			__syncthreads();
 		}
-		const uint32_t incl = part[threadIdx.x] + carry;
+		const uint32_t excl = part[threadIdx.x] + carry - x;
 		if (q < nb) {
-			limits[q + 1] = incl;
-			hist[q] = incl - x;   // becomes the scatter cursor of level q
+			hist[q] = excl;   // becomes the scatter cursor of key q
+			if (q % LEVEL_KEYS == 0) limits[q / LEVEL_KEYS] = excl;
 		}
 		const uint32_t tot = part[1023];
 		__syncthreads();
 		carry += tot;
 	}
-	if (threadIdx.x == 0) { limits[0] = 0; *nl = nb + 1; }
+	if (threadIdx.x == 0) { limits[nlev] = carry; *nl = nlev + 1; }
 }
 
-__global__ __launch_bounds__(SORT_T) void k_lv_scatter(const uint32_t *level, uint32_t n, uint32_t *cursor,
-                                                       uint32_t *sorted)
+__global__ __launch_bounds__(SORT_T) void k_lv_scatter(KeySrc key, uint32_t n, uint32_t *cursor, uint32_t *sorted)
 {
 	__shared__ uint32_t bins[SORT_BINS];
 	__shared__ uint32_t red[32];
 	const uint32_t b0 = blockIdx.x * SORT_BLOCK, b1 = min(n, b0 + SORT_BLOCK);
 	uint32_t lo, hi;
-	block_level_range(level, b0, b1, red, lo, hi);
+	block_key_range(key, b0, b1, red, lo, hi);
 	if (lo == GP_NIL) return;
-	block_level_counts(level, b0, b1, lo, bins, nullptr);
-	// Reserve the block's run of each level; the bins become its cursors.
+	block_key_counts(key, b0, b1, lo, bins, nullptr);
+	// Reserve the block's run of each key; the bins become its cursors.
 	const uint32_t range = min(SORT_BINS, hi - lo + 1);
 	for (uint32_t q = threadIdx.x; q < range; q += SORT_T)
 		if (bins[q]) bins[q] = atomicAdd(&cursor[lo + q], bins[q]);
 	__syncthreads();
 	for (uint32_t v = b0 + threadIdx.x; v < b1; v += SORT_T) {
-		const uint32_t l = level[v];
-		if (l == GP_NIL) continue;
-		sorted[l - lo < SORT_BINS ? atomicAdd(&bins[l - lo], 1u) : atomicAdd(&cursor[l], 1u)] = v;
+		const uint32_t k = key(v);
+		if (k == GP_NIL) continue;
+		sorted[k - lo < SORT_BINS ? atomicAdd(&bins[k - lo], 1u) : atomicAdd(&cursor[k], 1u)] = v;
 	}
+}
+
+// Slot rows of each 64-position chunk of sorted[] (its longest fan; 0 past the
+// reached vertices), one wave per chunk.
+__global__ void k_lv_chunk_slots(const uint32_t *sorted, const uint32_t *limits, const uint32_t *nl,
+                                 const uint32_t *fan_off, const uint8_t *closed, uint32_t nchunks, uint32_t *cslots)
+{
+	const uint32_t nr = limits[*nl - 1];
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+	for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nchunks; c += nw) {
+		const uint32_t p = (c << 6) + lane;
+		uint32_t sl = 0;
+		if (p < nr) {
+			const uint32_t v = sorted[p];
+			const uint32_t n = fan_off[v + 1] - fan_off[v];
+			sl = (n && closed[v]) ? n - 1 : n;
+		}
+		for (uint32_t o = 32; o > 0; o >>= 1) sl = max(sl, (uint32_t) __shfl_xor(sl, o));
+		if (lane == 0) cslots[c] = sl;
+	}
+}
+
+// out[q] = in[0] + ... + in[q - 1] for q <= n (one workgroup).
+__global__ __launch_bounds__(1024) void k_exclusive_scan(const uint32_t *in, uint32_t n, uint32_t *out)
+{
+	__shared__ uint32_t part[1024];
+	uint32_t carry = 0;
+	for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+		const uint32_t q = b0 + threadIdx.x;
+		const uint32_t x = q < n ? in[q] : 0u;
+		part[threadIdx.x] = x;
+		__syncthreads();
+		for (uint32_t o = 1; o < 1024; o <<= 1) {
+			const uint32_t y = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+			__syncthreads();
+			part[threadIdx.x] += y;
+			__syncthreads();
+		}
+		if (q < n) out[q] = part[threadIdx.x] + carry - x;
+		const uint32_t tot = part[1023];
+		__syncthreads();
+		carry += tot;
+	}
+	if (threadIdx.x == 0) out[n] = carry;
 }
 
 constexpr uint32_t LV = 32;   // level launches per graph replay
@@ -242,7 +309,7 @@ constexpr uint32_t LV = 32;   // level launches per graph replay
 Levels::~Levels()
 {
 	for (void *p : { (void *) level, (void *) queue, (void *) hist, (void *) sorted, (void *) limits, (void *) nl,
-	                 (void *) ls })
+	                 (void *) ls, (void *) cslots, (void *) coff })
 		(void) hipFree(p);
 	if (h_ls) (void) hipHostFree(h_ls);
 	if (graph) (void) hipGraphExecDestroy(graph);
@@ -254,7 +321,9 @@ int Levels::create(const DeviceMesh *mesh, hipStream_t s)
 	nv = mesh->nv;
 	stream = s;
 	if (hipMalloc(&level, 4ull * nv) != hipSuccess || hipMalloc(&queue, 4ull * nv) != hipSuccess ||
-	    hipMalloc(&hist, 4ull * (nv + 1)) != hipSuccess || hipMalloc(&sorted, 4ull * nv) != hipSuccess ||
+	    hipMalloc(&hist, 4ull * ((size_t) nv * LEVEL_KEYS + 1)) != hipSuccess ||
+	    hipMalloc(&sorted, 4ull * nv) != hipSuccess || hipMalloc(&cslots, 4ull * (nchunks() + 1)) != hipSuccess ||
+	    hipMalloc(&coff, 4ull * (nchunks() + 1)) != hipSuccess ||
 	    hipMalloc(&limits, 4ull * (nv + 2)) != hipSuccess || hipMalloc(&nl, 4) != hipSuccess ||
 	    hipMalloc(&ls, sizeof(LevelState)) != hipSuccess || hipHostMalloc(&h_ls, sizeof(LevelState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "levels: device allocation failed");
@@ -294,11 +363,16 @@ int Levels::expand()
 
 void Levels::sort()
 {
-	const uint32_t g = fill_grid(nv + 1), gb = (nv + SORT_BLOCK - 1) / SORT_BLOCK;
-	k_lv_zero<<<g, 256, 0, stream>>>(hist, nv + 1);
-	k_lv_hist<<<gb, SORT_T, 0, stream>>>(level, nv, hist, ls);
+	const uint32_t nk = nv * LEVEL_KEYS + 1;
+	const uint32_t g = fill_grid(nk), gb = (nv + SORT_BLOCK - 1) / SORT_BLOCK;
+	const KeySrc key{ level, dm->fan_off, dm->closed };
+	k_lv_zero<<<g, 256, 0, stream>>>(hist, nk);
+	k_lv_hist<<<gb, SORT_T, 0, stream>>>(key, nv, hist, ls);
 	k_lv_scan<<<1, 1024, 0, stream>>>(hist, limits, nl, ls);
-	k_lv_scatter<<<gb, SORT_T, 0, stream>>>(level, nv, hist, sorted);
+	k_lv_scatter<<<gb, SORT_T, 0, stream>>>(key, nv, hist, sorted);
+	const uint32_t nc = nchunks();
+	k_lv_chunk_slots<<<fill_grid(64 * nc), 256, 0, stream>>>(sorted, limits, nl, dm->fan_off, dm->closed, nc, cslots);
+	k_exclusive_scan<<<1, 1024, 0, stream>>>(cslots, nc, coff);
 }
 
 }  // namespace gp

@@ -4,9 +4,11 @@
 // launch per level, replayed from a captured graph, the host polling a done
 // flag) lowers the levels from the seeded sources, stopping where the levels
 // are already as low -- so seeding one more source into existing levels
-// updates them incrementally -- and a counting sort by level rebuilds sorted[]
-// and limits.  The order inside a ring is not the reference's discovery order
-// (the window sweeps relax whole rings, so it changes no value).
+// updates them incrementally -- and a counting sort by level and, inside a
+// level, by slot count (longest fans first) rebuilds sorted[], limits and the
+// slot rows of each 64-position chunk.  The order inside a ring is not the
+// reference's discovery order (the window sweeps relax whole rings, so it
+// changes no value).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,14 +19,18 @@ namespace gp {
 
 struct LevelState;
 
+constexpr uint32_t LEVEL_KEYS = 8;   // slot classes per level in the sort key
+
 struct Levels {
 	uint32_t nv = 0;
 	uint32_t *level = nullptr;    // [nv], NIL = unreached
 	uint32_t *queue = nullptr;    // [nv]
-	uint32_t *hist = nullptr;     // [nv + 1]
-	uint32_t *sorted = nullptr;   // [nv]   vertices by level
+	uint32_t *hist = nullptr;     // [nv * LEVEL_KEYS + 1]  key counts, then cursors
+	uint32_t *sorted = nullptr;   // [nv]   vertices by level, longest fans first in a level
 	uint32_t *limits = nullptr;   // [nv + 2]
 	uint32_t *nl = nullptr;       // |limits| (device)
+	uint32_t *cslots = nullptr;   // [nchunks() + 1] slot rows of each 64-position chunk of sorted
+	uint32_t *coff = nullptr;     // [nchunks() + 1] their exclusive prefix (chunk c at row coff[c])
 	LevelState *ls = nullptr, *h_ls = nullptr;
 	hipGraphExec_t graph = nullptr;
 	hipStream_t stream = nullptr;   // borrowed
@@ -38,7 +44,8 @@ struct Levels {
 	void clear();                                               // every level unreached
 	void seed(const uint32_t *d_src, uint32_t first, uint32_t ns);   // d_src[first, ns) to level 0
 	int expand();                                               // BFS to completion
-	void sort();                                                // sorted, limits, nl on the device
+	void sort();                                                // sorted, limits, nl, coff on the device
+	uint32_t nchunks() const { return (nv + 63) / 64; }
 };
 
 }  // namespace gp

@@ -721,12 +721,16 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	p->block = 256;
 	p->nchunks = (nr + 63) / 64;
 	p->max_slots = S;
-	p->sb = S;
+	p->sb = 0;                               // per-chunk slot rows (each ring sorted longest fans first)
 	p->n_flagged = 0;
 	p->g64 = !f64 && opts.fp32_geometry == GP_GEO_FP64;
-	const size_t lanes = (size_t) p->nchunks * S * 64;
+	uint32_t rows = 0;
+	GP_HIP(hipMemcpy(&rows, lv.coff + p->nchunks, 4, hipMemcpyDeviceToHost));
+	if (rows > p->nchunks * S) return fail(GP_ERR_KERNEL, "plan: bad chunk slot rows %u", rows);
+	const size_t lanes = (size_t) rows * 64;
 	if (hipMalloc(&p->d_limits, 4ull * nl) != hipSuccess || hipMalloc(&p->d_src, 4ull * ns) != hipSuccess ||
-	    hipMalloc(&p->d_inv, 4ull * mesh->nv) != hipSuccess || hipMalloc(&p->d_coff, 4) != hipSuccess ||
+	    hipMalloc(&p->d_inv, 4ull * mesh->nv) != hipSuccess ||
+	    hipMalloc(&p->d_coff, 4ull * (p->nchunks + 1)) != hipSuccess ||
 	    hipMalloc(&p->d_cid, 4 * std::max<size_t>(lanes, 1)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	if (f64) {
@@ -737,6 +741,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	}
 	GP_HIP(hipMemcpyAsync(p->d_limits, lv.limits, 4ull * nl, hipMemcpyDeviceToDevice, p->stream));
+	GP_HIP(hipMemcpyAsync(p->d_coff, lv.coff, 4ull * (p->nchunks + 1), hipMemcpyDeviceToDevice, p->stream));
 	const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(2048, (mesh->nv + 255) / 256));
 	k_fill_u32<<<g, 256, 0, p->stream>>>(p->d_inv, mesh->nv, GP_NIL);
 	k_positions<<<g, 256, 0, p->stream>>>(lv.sorted, nr, p->d_inv, p->d_src, ns);
@@ -744,10 +749,10 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	const uint32_t gg = std::max<uint32_t>(1, std::min<uint32_t>(4096, (npos + 255) / 256));
 	const uint32_t *nr_at = lv.limits + (nl - 1);
 	if (f64)
-		k_gather_tables<true, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, mt,
+		k_gather_tables<true, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, lv.coff, mt,
 		                                                         p->d_cid, nullptr, nullptr, nullptr, p->d_cx64);
 	else
-		k_gather_tables<false, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, mt,
+		k_gather_tables<false, false><<<gg, 256, 0, p->stream>>>(lv.sorted, p->d_inv, nr_at, npos, S, lv.coff, mt,
 		                                                          p->d_cid, p->d_cgeo, p->d_cgeo64, nullptr, nullptr);
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipStreamSynchronize(p->stream));

@@ -247,18 +247,22 @@ struct GeoRemap64 {
 };
 
 // Ring-ordered chunked tables gathered from vertex-major ones: position p
-// (vertex sorted[p], p < *nr_at) takes lane p % 64 of chunk p / 64, S slots
-// per chunk; neighbour ids become positions (flags kept), missing slots are
-// empty (id = p).  FP32 tables keep the FP64 geometry as a copy (cgeo64) or,
-// MIDX, as the slot's index in the vertex-major tables (midx).
+// (vertex sorted[p], p < *nr_at) takes lane p % 64 of chunk p / 64, whose
+// slot rows start at row coff[c] (c * S when coff is null); neighbour ids
+// become positions (flags kept), missing slots are empty (id = p).  FP32
+// tables keep the FP64 geometry as a copy (cgeo64) or, MIDX, as the slot's
+// index in the vertex-major tables (midx).
 template <bool F64, bool MIDX>
 __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, const uint32_t *nr_at, uint32_t npos,
-                                uint32_t S, VertexTables m, uint32_t *cid, float4 *cgeo, double4 *cgeo64,
-                                uint32_t *midx, Edge64 *cx64)
+                                uint32_t S, const uint32_t *coff, VertexTables m, uint32_t *cid, float4 *cgeo,
+                                double4 *cgeo64, uint32_t *midx, Edge64 *cx64)
 {
 	const uint32_t nr = *nr_at;
 	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npos; p += gridDim.x * blockDim.x) {
-		const size_t base = (size_t) (p >> 6) * S * 64 + (p & 63u);
+		const uint32_t c = p >> 6;
+		const uint32_t row = coff ? coff[c] : c * S;
+		const uint32_t Sc = coff ? coff[c + 1] - row : S;
+		const size_t base = (size_t) row * 64 + (p & 63u);
 		uint32_t mb = 0, n = 0;
 		if (p < nr) {
 			const uint32_t v = sorted[p];
@@ -266,7 +270,7 @@ __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, con
 			const uint32_t nf = m.coff[v + 1] - mb;
 			n = (nf && m.closed[v]) ? nf - 1 : nf;
 		}
-		for (uint32_t k = 0; k < S; k++) {
+		for (uint32_t k = 0; k < Sc; k++) {
 			const size_t at = base + 64 * (size_t) k;
 			const bool real = k < n;
 			const uint32_t id = real ? m.cid[mb + k] : 0u;
