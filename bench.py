@@ -209,6 +209,17 @@ def main():
             e2e[name] = {"ms_min": round(min(walls), 2), "ms_mean": round(float(np.mean(walls)), 2),
                          "kernel_ms": round(float(np.mean(kms)), 4)}
             log(f"[bench] end to end, plain_ref_f32, {name}: {e2e[name]}")
+        # The drop-in call (gp_ptp: the front end's PTP_GPU), which also
+        # returns the reference's BFS order (sorted_index).
+        walls = []
+        for _ in range(3):
+            C = P.che(xyz, faces)
+            t0 = time.perf_counter()
+            P.ptp(C, src, device_setup=True)
+            walls.append((time.perf_counter() - t0) * 1e3)
+            del C
+        e2e["gp_ptp_with_sorted_index"] = {"ms_min": round(min(walls), 2), "ms_mean": round(float(np.mean(walls)), 2)}
+        log(f"[bench] end to end, gp_ptp with sorted_index: {e2e['gp_ptp_with_sorted_index']}")
     # The 16-source field of the same mesh (one multi-source field).
     src16 = G.draw_sources(np.random.default_rng(args.seed + 2000), len(xyz), 16)
     multi = {}
@@ -255,6 +266,13 @@ def main():
                 e = rel_err(r["dist"], ref16)
                 r["max_vs_ref_schedule"], r["mean_vs_ref_schedule"] = e
                 log(f"[bench] 16 sources, {name}: vs reference schedule {e[0]:.2e}")
+
+    if not rows:
+        print(json.dumps({"metric": "end to end and 16-source rows only (no timed field rows selected)",
+                          "end_to_end": e2e or None,
+                          "sources16": {n: {"kernel_ms": round(r["kernel_ms"], 4), "sweeps": r["st"]["sweeps"]}
+                                        for n, r in multi.items()} or None}), flush=True)
+        return
 
     # Headline: learned bands if they meet the bar, else the fastest row that does.
     def meets(name):

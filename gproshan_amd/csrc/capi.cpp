@@ -5,6 +5,8 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
+#include <thread>
 
 #include "gp_internal.hpp"
 
@@ -263,18 +265,38 @@ int gp_ptp(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const gp_p
 		gp_ptp_opts o;
 		if (opts) o = *opts; else gp_ptp_opts_default(&o);
 		const gp::Che &m = mesh->m;
+		// The BFS order the caller asked for (the reference's sorted_index)
+		// is built on a host thread while the solve runs.
+		int rc0 = GP_OK;
+		std::string err0;
+		std::thread bfs;
 		if (sorted_out) {
-			std::vector<uint32_t> lev(m.nv), lim;
-			int rc0 = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted_out, lim);
-			if (rc0) return rc0;
+			bfs = std::thread([&] {
+				try {
+					std::vector<uint32_t> lev(m.nv), lim;
+					rc0 = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted_out, lim);
+					if (rc0) err0 = gp::last_error();
+				} catch (...) {
+					rc0 = GP_ERR_NOMEM;
+					err0 = "toplesets: allocation failed";
+				}
+			});
 		}
+		struct Join {
+			std::thread &t;
+			~Join() { if (t.joinable()) t.join(); }
+		} join{ bfs };
 		gp_plan *p = nullptr;
 		int rc = plan_create_any(mesh, sources, n_sources, nullptr, o, &p);
 		if (rc) return rc;
 		std::unique_ptr<gp_plan, void (*)(gp_plan *)> guard(p, gp_plan_destroy);
 		rc = gp::plan_run(p, stats);
 		if (rc) return rc;
-		return gp::plan_result(p, dist);
+		rc = gp::plan_result(p, dist);
+		if (rc) return rc;
+		if (bfs.joinable()) bfs.join();
+		if (rc0) return gp::fail(rc0, "%s", err0.c_str());
+		return GP_OK;
 	)
 }
 

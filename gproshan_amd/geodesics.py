@@ -204,12 +204,14 @@ def normalize_ptp(dist):
 
 
 def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRITTEN, schedule=L.SCHED_RINGS,
-        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH, uncapped=False):
-    """One PTP solve with explicit options; returns (dist, sorted, stats dict)."""
+        coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH, uncapped=False,
+        device_setup=False):
+    """One PTP solve with explicit options; returns (dist, sorted, stats dict).
+    sorted is the reference's BFS order, built on the host while the solve runs."""
     src = _u32(sources)
     o = L.default_opts(precision=precision, result_buffer=result_buffer, schedule=schedule,
                        coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid,
-                       launch_mode=launch_mode, uncapped=int(uncapped))
+                       launch_mode=launch_mode, uncapped=int(uncapped), device_setup=int(device_setup))
     dist = np.zeros(mesh.n_vertices())
     srt = np.zeros(mesh.n_vertices(), np.uint32)
     st = L.PtpStats()
@@ -347,9 +349,13 @@ class geodesics:
             self._dist[:n] = d
             self.clusters = cl
         elif opt == self.PTP_GPU:
-            # One source: ring-renumbered (coalesced) sweeps; several: sorted[]
-            # indirection (src/geodesics.cpp:225-240).
-            d, srt, _ = ptp(mesh, src, precision=precision, schedule=schedule, coalesce=len(src) == 1)
+            # The reference runs the coalescence driver for one source and the
+            # sorted[] driver for several (src/geodesics.cpp:225-240); both give
+            # the same field.  Here both run ring-numbered sweeps whose rings
+            # and tables are built on the device (the host builds them for
+            # other schedules and for fans past the register-held slots).
+            d, srt, _ = ptp(mesh, src, precision=precision, schedule=schedule, coalesce=True,
+                            device_setup=schedule == L.SCHED_RINGS)
             self._sorted[:] = srt
             self._dist[:n] = d
         else:

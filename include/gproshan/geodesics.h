@@ -240,9 +240,15 @@ class geodesics {
 				else
 					parallel_toplesets_propagation_gpu({ dist_, clusters }, mesh, sources, { limits, sorted_index_ });
 			} else {
+				// The reference runs the coalescence driver for one source and
+				// the sorted[] driver for several (src/geodesics.cpp:232-235);
+				// both give the same field.  Here both run ring-numbered sweeps
+				// whose rings and tables are built on the device (host build
+				// for fans past the register-held slots).
 				gp_ptp_opts o;
 				gp_ptp_opts_default(&o);
-				o.coalesce = sources.size() == 1;   // src/geodesics.cpp:232-235
+				o.coalesce = 1;
+				o.device_setup = 1;
 				check(gp_ptp(mesh->handle(), sources.data(), (uint32_t) sources.size(), &o, dist_, sorted_index_,
 				             nullptr));
 			}

@@ -13,7 +13,8 @@ EXE = os.path.join(ROOT, "tests", "cpp", "frontend")
 def exe():
     src = os.path.join(ROOT, "tests", "cpp", "frontend.cpp")
     lib = os.path.join(ROOT, "gproshan_amd")
-    if os.path.exists(EXE) and os.path.getmtime(EXE) >= os.path.getmtime(src):
+    deps = [src, os.path.join(ROOT, "include", "gproshan", "geodesics.h"), os.path.join(ROOT, "include", "gproshan_ptp.h")]
+    if os.path.exists(EXE) and all(os.path.getmtime(EXE) >= os.path.getmtime(d) for d in deps):
         return EXE
     subprocess.run(["g++", "-std=c++17", "-O1", f"-I{ROOT}/include", src, "-o", EXE, f"-L{lib}", "-lgproshan_ptp",
                     f"-Wl,-rpath,{lib}"], check=True)
