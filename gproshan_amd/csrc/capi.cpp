@@ -308,6 +308,8 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
                            const gp_ptp_opts &o, gp_plan **out)
 {
 	const gp::Che &m = mesh->m;
+	if (o.fp32_geometry != GP_GEO_AUTO && o.fp32_geometry != GP_GEO_FP32 && o.fp32_geometry != GP_GEO_FP64)
+		return gp::fail(GP_ERR_INVALID, "unknown fp32_geometry %d", o.fp32_geometry);
 	if (o.schedule == GP_SCHED_PATCHES) {
 		gp_ptp_opts op = o;
 		if (!(op.band_width > 0)) op.band_width = 128.0;

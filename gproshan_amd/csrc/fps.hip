@@ -178,25 +178,29 @@ __global__ void k_fps_state_init(FState *st, const uint32_t *nl, const uint32_t 
 __device__ __forceinline__ unsigned long long ordered_bits(float x) { return __float_as_uint(x); }
 __device__ __forceinline__ unsigned long long ordered_bits(double x) { return (unsigned long long) __double_as_longlong(x); }
 
-// Largest |distance| of the buffer the last sweep wrote, over the vertices in
-// their own numbering (unreached ones are +inf, as in the reference's field;
-// distances are >= 0 or +inf, so their IEEE bit patterns order like the
-// values).
+// Distance of vertex v in the buffer d (by position); unreached vertices are
+// +inf, as in the reference's field.
 template <typename T>
 __device__ __forceinline__ T vertex_dist(const T *d, const uint32_t *inv, uint32_t v)
 {
 	return inv[v] != GP_NIL ? d[inv[v]] : (T) INFINITY;
 }
 
+// Largest |distance| of the buffer the last sweep wrote, read over the
+// positions (contiguous), +inf when fewer than n vertices were reached
+// (distances are >= 0 or +inf, so their IEEE bit patterns order like the
+// values).
 template <typename T>
-__global__ void k_fps_max(const void *d0, const void *d1, const uint32_t *inv, uint32_t n, FState *st)
+__global__ void k_fps_max(const void *d0, const void *d1, uint32_t n, FState *st)
 {
 	const T *d = (const T *) (st->final_d ? d1 : d0);
+	const uint32_t nr = st->nr;
 	unsigned long long m = 0;
-	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-		const unsigned long long b = ordered_bits(fabs(vertex_dist(d, inv, v)));
+	for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < nr; p += gridDim.x * blockDim.x) {
+		const unsigned long long b = ordered_bits(fabs(d[p]));
 		m = b > m ? b : m;
 	}
+	if (nr < n) m = ordered_bits((T) INFINITY);
 	for (uint32_t o = 32; o > 0; o >>= 1) {
 		const unsigned long long y = __shfl_xor(m, o);
 		m = y > m ? y : m;
@@ -237,10 +241,10 @@ struct FpsRun {
 	float4 *cgeo = nullptr;
 	Edge64 *cx64 = nullptr;
 	void *d0 = nullptr, *d1 = nullptr;
-	FState *st = nullptr, *h_st = nullptr;
+	FState *st = nullptr, *h_st = nullptr;   // h_st: the round's snapshot, then two sweep snapshots
 	BfsJob *job = nullptr;
 	hipStream_t stream = nullptr;
-	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = { nullptr, nullptr };
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	~FpsRun()
 	{
@@ -252,6 +256,8 @@ struct FpsRun {
 		if (g_small) (void) hipGraphExecDestroy(g_small);
 		if (e0) (void) hipEventDestroy(e0);
 		if (e1) (void) hipEventDestroy(e1);
+		for (hipEvent_t e : ev)
+			if (e) (void) hipEventDestroy(e);
 		if (stream) (void) hipStreamDestroy(stream);
 	}
 };
@@ -296,7 +302,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	if ((rc = fup(r.samples, samples, cap))) return rc;
 	if (hipMalloc(&r.d0, sizeof(T) * nv) != hipSuccess || hipMalloc(&r.d1, sizeof(T) * nv) != hipSuccess ||
 	    hipMalloc(&r.st, sizeof(FState)) != hipSuccess || hipMalloc(&r.job, sizeof(BfsJob)) != hipSuccess ||
-	    hipHostMalloc(&r.h_st, sizeof(FState)) != hipSuccess)
+	    hipHostMalloc(&r.h_st, 3 * sizeof(FState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "fps: device allocation failed");
 	GP_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
 	Levels lv;
@@ -305,6 +311,8 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	GP_HIP(hipMemcpy(r.job, &job, sizeof(job), hipMemcpyHostToDevice));
 	GP_HIP(hipEventCreate(&r.e0));
 	GP_HIP(hipEventCreate(&r.e1));
+	GP_HIP(hipEventCreateWithFlags(&r.ev[0], hipEventDisableTiming));
+	GP_HIP(hipEventCreateWithFlags(&r.ev[1], hipEventDisableTiming));
 	FArgs a{};
 	a.t.cid = r.cid; a.t.cgeo = r.cgeo; a.t.cgeo64 = GeoRemap64{ mt.cgeo64, r.midx }; a.t.cx64 = r.cx64;
 	a.t.h = m.dev->h; a.t.inv_h = m.dev->inv_h;
@@ -362,19 +370,35 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		const uint32_t nl = r.h_st->nl;
 		if (nl < 2 || nl > nv + 2) return fail(GP_ERR_KERNEL, "fps: bad ring count %u", nl);
 		// Sweeps: graph replays until the device reports done (at most
-		// 2*|limits| sweeps plus the replay that records the last outcome).
+		// 2*|limits| sweeps plus the replay that records the last outcome),
+		// two in flight: each replay is followed by a copy of the state into
+		// its pinned snapshot and an event, and the host waits for the older
+		// replay while the newer one runs (a replay queued behind the one that
+		// finishes meets done slots and returns at once).  j grows by one ring
+		// per sweep, so big graphs run while the last seen j, one replay
+		// behind, leaves room for them.
+		FState *snap = r.h_st + 1;
 		uint32_t launched = 0, j_seen = 2;
-		while (!r.h_st->done) {
-			if (launched > 2 * nl + BIG) return fail(GP_ERR_KERNEL, "fps: schedule did not finish");
-			const bool big = nl - 1 > j_seen + BIG;
+		auto replay = [&](uint32_t s) -> int {
+			const bool big = nl - 1 > j_seen + BIG + (launched ? BIG : 0);
 			GP_HIP(hipGraphLaunch(big ? r.g_big : r.g_small, r.stream));
 			launched += big ? BIG : SMALL;
-			GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
-			GP_HIP(hipStreamSynchronize(r.stream));
-			j_seen = r.h_st->slot[3].j;
+			GP_HIP(hipMemcpyAsync(snap + s, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
+			GP_HIP(hipEventRecord(r.ev[s], r.stream));
+			return GP_OK;
+		};
+		if (!r.h_st->done) {
+			if ((rc = replay(0))) return rc;
+			for (uint32_t cur = 0;; cur ^= 1u) {
+				if (launched > 2 * nl + 2 * BIG) return fail(GP_ERR_KERNEL, "fps: schedule did not finish");
+				if ((rc = replay(cur ^ 1u))) return rc;
+				GP_HIP(hipEventSynchronize(r.ev[cur]));
+				if (snap[cur].done) break;
+				j_seen = snap[cur].slot[3].j;
+			}
 		}
 		k_fps_arg_init<<<1, 1, 0, r.stream>>>(r.st);
-		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, r.inv, nv, r.st);
+		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
 		k_fps_arg<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, r.inv, nv, r.st);
 		k_fps_append<T><<<1, 1, 0, r.stream>>>(r.job, r.samples, r.st);
 		GP_HIP(hipGetLastError());

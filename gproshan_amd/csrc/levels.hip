@@ -313,6 +313,8 @@ Levels::~Levels()
 		(void) hipFree(p);
 	if (h_ls) (void) hipHostFree(h_ls);
 	if (graph) (void) hipGraphExecDestroy(graph);
+	for (hipEvent_t e : ev)
+		if (e) (void) hipEventDestroy(e);
 }
 
 int Levels::create(const DeviceMesh *mesh, hipStream_t s)
@@ -325,8 +327,10 @@ int Levels::create(const DeviceMesh *mesh, hipStream_t s)
 	    hipMalloc(&sorted, 4ull * nv) != hipSuccess || hipMalloc(&cslots, 4ull * (nchunks() + 1)) != hipSuccess ||
 	    hipMalloc(&coff, 4ull * (nchunks() + 1)) != hipSuccess ||
 	    hipMalloc(&limits, 4ull * (nv + 2)) != hipSuccess || hipMalloc(&nl, 4) != hipSuccess ||
-	    hipMalloc(&ls, sizeof(LevelState)) != hipSuccess || hipHostMalloc(&h_ls, sizeof(LevelState)) != hipSuccess)
+	    hipMalloc(&ls, sizeof(LevelState)) != hipSuccess || hipHostMalloc(&h_ls, 2 * sizeof(LevelState)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "levels: device allocation failed");
+	GP_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+	GP_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 	// Enough workgroups for a frontier of 64k vertices per pass.
 	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(256, (nv + 255) / 256));
 	hipGraph_t g = nullptr;
@@ -348,17 +352,30 @@ void Levels::seed(const uint32_t *d_src, uint32_t first, uint32_t ns)
 	k_lv_seed<<<1, 256, 0, stream>>>(d_src, first, ns, level, queue, ls);
 }
 
+// Replays until the state reports done, two in flight: each replay is
+// followed by a copy of the state into its pinned snapshot and an event; the
+// host waits for the older replay while the newer one runs (a replay queued
+// behind the one that finishes meets done slots and returns at once).
 int Levels::expand()
 {
 	GP_HIP(hipGetLastError());
-	for (uint32_t reps = 0;; reps++) {
+	auto replay = [&](uint32_t s) -> int {
+		GP_HIP(hipGraphLaunch(graph, stream));
+		GP_HIP(hipMemcpyAsync(h_ls + s, ls, sizeof(LevelState), hipMemcpyDeviceToHost, stream));
+		GP_HIP(hipEventRecord(ev[s], stream));
+		return GP_OK;
+	};
+	int rc = replay(0);
+	if (rc) return rc;
+	for (uint32_t reps = 0, cur = 0;; reps++, cur ^= 1u) {
 		// Every replay but an empty one lowers at least one vertex.
 		if (reps > nv / LV + 2) return fail(GP_ERR_KERNEL, "levels: BFS did not finish");
-		GP_HIP(hipGraphLaunch(graph, stream));
-		GP_HIP(hipMemcpyAsync(h_ls, ls, sizeof(LevelState), hipMemcpyDeviceToHost, stream));
-		GP_HIP(hipStreamSynchronize(stream));
-		if (h_ls->done) return GP_OK;
+		if ((rc = replay(cur ^ 1u))) return rc;
+		GP_HIP(hipEventSynchronize(ev[cur]));
+		if (h_ls[cur].done) break;
 	}
+	GP_HIP(hipStreamSynchronize(stream));
+	return GP_OK;
 }
 
 void Levels::sort()

@@ -31,7 +31,8 @@ struct Levels {
 	uint32_t *nl = nullptr;       // |limits| (device)
 	uint32_t *cslots = nullptr;   // [nchunks() + 1] slot rows of each 64-position chunk of sorted
 	uint32_t *coff = nullptr;     // [nchunks() + 1] their exclusive prefix (chunk c at row coff[c])
-	LevelState *ls = nullptr, *h_ls = nullptr;
+	LevelState *ls = nullptr, *h_ls = nullptr;   // h_ls: two pinned snapshots
+	hipEvent_t ev[2] = { nullptr, nullptr };
 	hipGraphExec_t graph = nullptr;
 	hipStream_t stream = nullptr;   // borrowed
 	const DeviceMesh *dm = nullptr;

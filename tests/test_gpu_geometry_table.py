@@ -56,3 +56,11 @@ def test_geometry_tables_identical(case, sched):
     assert np.array_equal(tr32, tr64)
     if cl32 is not None:
         assert np.array_equal(cl32, cl64)
+
+
+def test_geometry_table_option_checked():
+    xyz, f = G.height_field(16, 0)
+    C = P.che(xyz, f)
+    with pytest.raises(P.GPError) as e:
+        P.Plan(C, [3], fp32_geometry=7)
+    assert e.value.status == L.ERR_INVALID
