@@ -662,9 +662,10 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 // Plan of the ring schedule on a ring-numbered layout built on the device:
 // rings by the frontier BFS and counting sort (no host BFS, no host
 // renumbering), tables gathered from the mesh's vertex-major tables
-// (k_gather_tables; every chunk at the longest fan's width).  The
-// rings are the reference's as sets, the order inside a ring is not (nothing
-// depends on it); no FP64-update count.  GP_ERR_CAPACITY when a fan needs
+// (k_gather_tables; each ring ordered longest fans first, each chunk as many
+// slot rows as its own longest fan).  The rings are the reference's as sets,
+// the order inside a ring is not (nothing depends on it); no FP64-update
+// count.  GP_ERR_CAPACITY when a fan needs
 // more register-held slots than the sweep kernels hold (the caller then takes
 // the host-built path).
 int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out)
