@@ -115,4 +115,6 @@ def test_frontend_ptp_gpu_dispatch(case, oracle):
     mx, _ = rel_err(g.dist, ref)
     assert mx <= 1e-12
     _, srt, _ = M.toplesets(src)
-    assert all(g(i) == srt[i] for i in range(0, len(srt), 101))
+    # sorted_index is the reference's BFS order (built on a host thread while
+    # the device-built plan solves).
+    assert np.array_equal(np.array([g(i) for i in range(len(srt))], np.uint32), np.asarray(srt, np.uint32))
