@@ -220,6 +220,30 @@ def main():
             del C
         e2e["gp_ptp_with_sorted_index"] = {"ms_min": round(min(walls), 2), "ms_mean": round(float(np.mean(walls)), 2)}
         log(f"[bench] end to end, gp_ptp with sorted_index: {e2e['gp_ptp_with_sorted_index']}")
+        # The learned-band field end to end: prediction (features need the
+        # rings), band plan (bands, renumbering, tables, upload), one run,
+        # result download.
+        for tag, dev in (("band_learned", False), ("band_learned_device_setup", True)):
+            if net is None:
+                break
+            walls, parts = [], []
+            for _ in range(3):
+                C = P.che(xyz, faces)
+                t0 = time.perf_counter()
+                dhat, _ = net.predict(C, src)
+                t1 = time.perf_counter()
+                plan = P.Plan(C, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, device_setup=dev, **band_kw)
+                t2 = time.perf_counter()
+                stx = plan.run()
+                plan.result()
+                plan.close()
+                walls.append((time.perf_counter() - t0) * 1e3)
+                parts.append(((t1 - t0) * 1e3, (t2 - t1) * 1e3, stx["kernel_ms"]))
+                del C
+            k = int(np.argmin(walls))
+            e2e[tag] = {"ms_min": round(walls[k], 2), "predict_ms": round(parts[k][0], 2),
+                        "plan_ms": round(parts[k][1], 2), "kernel_ms": round(parts[k][2], 4)}
+            log(f"[bench] end to end, {tag}: {e2e[tag]}")
     # The 16-source field of the same mesh (one multi-source field).
     src16 = G.draw_sources(np.random.default_rng(args.seed + 2000), len(xyz), 16)
     multi = {}

@@ -323,43 +323,51 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 		if (rd != GP_ERR_CAPACITY) return rd;
 		gp::clear_error();   // a fan too long for the device build: build on the host
 	}
+	std::vector<double> d;
+	if (!dhat && o.schedule == GP_SCHED_EUCLID_BAND) {
+		d.assign(m.nv, INFINITY);
+		#pragma omp parallel for schedule(static)
+		for (int64_t v = 0; v < (int64_t) m.nv; v++) {
+			double best = INFINITY;
+			for (uint32_t q = 0; q < n_sources; q++) {
+				const double *a = &m.gt[3 * (size_t) v], *b = &m.gt[3 * (size_t) sources[q]];
+				best = std::min(best, std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+				                                (a[2] - b[2]) * (a[2] - b[2])));
+			}
+			d[v] = best;
+		}
+		dhat = d.data();
+	}
+	// Bands of predicted distance take the place of the rings in the
+	// reference's uncapped window loop.
+	const double width = (o.band_width > 0 ? o.band_width : 1.0) * m.mean_edge;
+	gp_ptp_opts ob = o;
+	ob.schedule = GP_SCHED_RINGS;
+	ob.coalesce = 1;
+	ob.uncapped = 1;
+	if (o.device_setup && !o.clusters && dhat &&
+	    (o.schedule == GP_SCHED_LEARNED_BAND || o.schedule == GP_SCHED_EUCLID_BAND)) {
+		const int rd = gp::plan_create_device(&mesh->m, sources, n_sources, ob, out, dhat, width);
+		if (rd != GP_ERR_CAPACITY) return rd;
+		gp::clear_error();   // a fan too long or a band index too large: build on the host
+	}
 	std::vector<uint32_t> lev(m.nv), sorted(m.nv), lim;
 	int rc = gp::toplesets_host(m, sources, n_sources, lev.data(), sorted.data(), lim);
 	if (rc) return rc;
 	if (o.schedule == GP_SCHED_RINGS)
 		return gp::plan_create_rings(&mesh->m, sources, n_sources, lim.data(), (uint32_t) lim.size(), sorted.data(), o, out);
-	std::vector<double> d;
 	if (!dhat) {
-		d.assign(m.nv, INFINITY);
-		if (o.schedule == GP_SCHED_EUCLID_BAND) {
-			#pragma omp parallel for schedule(static)
-			for (int64_t v = 0; v < (int64_t) m.nv; v++) {
-				double best = INFINITY;
-				for (uint32_t q = 0; q < n_sources; q++) {
-					const double *a = &m.gt[3 * (size_t) v], *b = &m.gt[3 * (size_t) sources[q]];
-					best = std::min(best, std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
-					                                (a[2] - b[2]) * (a[2] - b[2])));
-				}
-				d[v] = best;
-			}
-		} else if (o.schedule == GP_SCHED_RING_BAND) {
-			for (uint32_t v = 0; v < m.nv; v++) d[v] = lev[v] == gp::NIL ? INFINITY : (double) lev[v] * m.mean_edge;
-		} else {
+		if (o.schedule != GP_SCHED_RING_BAND)
 			return gp::fail(GP_ERR_INVALID, "learned bands need predicted distances (gp_plan_create_bands)");
-		}
+		d.assign(m.nv, INFINITY);
+		for (uint32_t v = 0; v < m.nv; v++) d[v] = lev[v] == gp::NIL ? INFINITY : (double) lev[v] * m.mean_edge;
 		dhat = d.data();
 	}
-	// Bands of predicted distance take the place of the rings in the
-	// reference's uncapped window loop; the vertices of a band are in Morton
-	// order (the coalesced numbering of the plan).
-	const double width = (o.band_width > 0 ? o.band_width : 1.0) * m.mean_edge;
+	// Host-built bands: the vertices of a band in Morton order (the coalesced
+	// numbering of the plan).
 	std::vector<uint32_t> order, blim;
 	rc = gp::band_order(m, sources, n_sources, lev.data(), dhat, width, order, blim);
 	if (rc) return rc;
-	gp_ptp_opts ob = o;
-	ob.schedule = GP_SCHED_RINGS;
-	ob.coalesce = 1;
-	ob.uncapped = 1;
 	return gp::plan_create_rings(&mesh->m, sources, n_sources, blim.data(), (uint32_t) blim.size(), order.data(), ob, out);
 }
 

@@ -153,7 +153,10 @@ int toplesets_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels, ui
                   std::vector<uint32_t> &limits);
 int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_t *limits, uint32_t nl,
                       const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out);
-int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out);
+// Ring plan built on the device; with dhat, the rings are the nonempty bands
+// of width `width` of dhat (the sources first), as the host band builder.
+int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out,
+                       const double *dhat = nullptr, double width = 0);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
 int plan_result(gp_plan *p, double *dist);
 int plan_clusters(gp_plan *p, uint32_t *clusters);

@@ -350,6 +350,56 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t x)
 
 // inv[sorted[p]] = p for the nr reached vertices; src_pos = 0 .. ns - 1 (ring 0
 // holds exactly the sources).
+// Band plans built on the device: a reached vertex other than a source (BFS
+// level 0) takes band 1 + floor(dhat / width) (clamped to nv; the host band
+// builder's index, che.cpp band_order), the sources band 0.  *bmax gets the
+// largest band.
+__global__ void k_band_levels(uint32_t *level, const double *dhat, uint32_t nv, double width, uint32_t *bmax)
+{
+	uint32_t m = 0;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
+		const uint32_t l = level[v];
+		if (l == GP_NIL || l == 0) continue;
+		const double d = dhat[v];
+		const double bq = isfinite(d) && d > 0 ? floor(d / width) : 0.0;
+		const unsigned long long band = 1ull + (unsigned long long) fmin(bq, 4.0e9);
+		const uint32_t b = band < nv ? (uint32_t) band : nv;
+		level[v] = b;
+		m = max(m, b);
+	}
+	for (uint32_t o = 32; o > 0; o >>= 1) m = max(m, (uint32_t) __shfl_xor(m, o));
+	if ((threadIdx.x & 63u) == 0) atomicMax(bmax, m);
+}
+
+// Drops the empty bands from limits[0 .. *nl - 1] (a level with no vertex
+// repeats its start), as the host band builder never opens one; tmp holds
+// *nl words.  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_compact_limits(uint32_t *limits, uint32_t *nl, uint32_t *tmp)
+{
+	__shared__ uint32_t part[1024];
+	const uint32_t n = *nl;
+	uint32_t carry = 0;
+	for (uint32_t b0 = 0; b0 < n; b0 += 1024) {
+		const uint32_t q = b0 + threadIdx.x;
+		const uint32_t keep = (q < n && (q == 0 || limits[q] != limits[q - 1])) ? 1u : 0u;
+		part[threadIdx.x] = keep;
+		__syncthreads();
+		for (uint32_t o = 1; o < 1024; o <<= 1) {
+			const uint32_t y = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+			__syncthreads();
+			part[threadIdx.x] += y;
+			__syncthreads();
+		}
+		if (keep) tmp[carry + part[threadIdx.x] - 1] = limits[q];
+		const uint32_t tot = part[1023];
+		__syncthreads();
+		carry += tot;
+	}
+	__syncthreads();
+	for (uint32_t q = threadIdx.x; q < carry; q += 1024) limits[q] = tmp[q];
+	if (threadIdx.x == 0) *nl = carry;
+}
+
 __global__ void k_positions(const uint32_t *sorted, uint32_t nr, uint32_t *inv, uint32_t *src_pos, uint32_t ns)
 {
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nr; q += gridDim.x * blockDim.x) {
@@ -668,7 +718,8 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 // count.  GP_ERR_CAPACITY when a fan needs
 // more register-held slots than the sweep kernels hold (the caller then takes
 // the host-built path).
-int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out)
+int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out,
+                       const double *dhat, double width)
 {
 	*out = nullptr;
 	if (!mesh || !src) return fail(GP_ERR_INVALID, "plan: null argument");
@@ -708,7 +759,30 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	lv.clear();
 	lv.seed(d_srcv, 0, ns);
 	if ((rc = lv.expand())) return rc;
-	lv.sort();
+	if (dhat) {
+		// Bands of dhat in place of the BFS levels of the reached vertices.
+		if (!(width > 0)) return fail(GP_ERR_INVALID, "band width must be positive");
+		std::vector<double> dh(dhat, dhat + mesh->nv);
+		double *d_dhat = nullptr;
+		uint32_t *d_bmax = nullptr;
+		if ((rc = upload(d_dhat, dh))) return rc;
+		std::unique_ptr<double, hipError_t (*)(void *)> dhat_guard(d_dhat, hipFree);
+		if (hipMalloc(&d_bmax, 4) != hipSuccess) return fail(GP_ERR_NOMEM, "plan: device allocation failed");
+		std::unique_ptr<uint32_t, hipError_t (*)(void *)> bmax_guard(d_bmax, hipFree);
+		const uint32_t gb = std::max<uint32_t>(1, std::min<uint32_t>(2048, (mesh->nv + 255) / 256));
+		k_fill_u32<<<1, 64, 0, p->stream>>>(d_bmax, 1, 0);
+		k_band_levels<<<gb, 256, 0, p->stream>>>(lv.level, d_dhat, mesh->nv, width, d_bmax);
+		GP_HIP(hipGetLastError());
+		uint32_t bmax = 0;
+		GP_HIP(hipMemcpyAsync(&bmax, d_bmax, 4, hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipStreamSynchronize(p->stream));
+		if (bmax >= mesh->nv) return fail(GP_ERR_CAPACITY, "plan: band index %u past the vertex count", bmax);
+		lv.sort();
+		k_compact_limits<<<1, 1024, 0, p->stream>>>(lv.limits, lv.nl, lv.hist);
+		GP_HIP(hipGetLastError());
+	} else {
+		lv.sort();
+	}
 	uint32_t nl = 0;
 	GP_HIP(hipMemcpyAsync(&nl, lv.nl, 4, hipMemcpyDeviceToHost, p->stream));
 	GP_HIP(hipStreamSynchronize(p->stream));

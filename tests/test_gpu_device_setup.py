@@ -88,3 +88,34 @@ def test_device_setup_errors():
         P.Plan(C, [3, 3], device_setup=True)
     with pytest.raises(P.GPError):
         P.Plan(C, [len(xyz)], device_setup=True)
+
+
+@pytest.mark.parametrize("kind", ["euclid", "dhat"])
+def test_device_setup_bands_f64_identical(case, kind):
+    """Band plans built on the device (bands of dhat as the rings, empty bands
+    dropped) against the host band builder: same bands as sets, so the FP64
+    field and the schedule trace are identical."""
+    name, src, C = case
+    if kind == "euclid":
+        kw = dict(schedule=L.SCHED_EUCLID_BAND)
+    else:
+        rng = np.random.default_rng(5)
+        xyz = C.gt()
+        e = np.min(np.linalg.norm(xyz[:, None, :] - xyz[src][None, :, :], axis=2), axis=1)
+        kw = dict(schedule=L.SCHED_LEARNED_BAND, dhat=e * rng.uniform(0.8, 1.3, len(e)))
+    host, tr_h, _ = _solve(C, src, precision=L.PREC_F64, tol=1e-6, **kw)
+    dev, tr_d, st = _solve(C, src, precision=L.PREC_F64, tol=1e-6, device_setup=True, **kw)
+    print(f"[device bands f64] {name} {kind} sweeps {st['sweeps']} bands {len(tr_d)}", flush=True)
+    assert np.array_equal(host.view(np.uint64), dev.view(np.uint64))
+    assert np.array_equal(tr_h, tr_d)
+
+
+def test_device_setup_bands_fall_back_on_large_band_index():
+    xyz, f = G.height_field(32, 2)
+    C = P.che(xyz, f)
+    dhat = np.full(len(xyz), 1e300)
+    host, tr_h, _ = _solve(C, [5], precision=L.PREC_F64, tol=1e-6, schedule=L.SCHED_LEARNED_BAND, dhat=dhat)
+    dev, tr_d, _ = _solve(C, [5], precision=L.PREC_F64, tol=1e-6, schedule=L.SCHED_LEARNED_BAND, dhat=dhat,
+                          device_setup=True)
+    assert np.array_equal(host.view(np.uint64), dev.view(np.uint64))
+    assert np.array_equal(tr_h, tr_d)
