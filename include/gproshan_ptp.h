@@ -116,14 +116,16 @@ typedef struct gp_ptp_opts {
 	                       source q is q + 1, a vertex whose value drops takes the
 	                       label of the nearer vertex of the winning triangle
 	                       (old values), NIL where unreached                  */
-	int device_setup;   /* ring schedule, coalesce = 1, clusters = 0: 1 = build the
-	                       rings and the ring-numbered tables on the device (a
-	                       frontier BFS and a counting sort instead of the host
-	                       toplesets and renumbering; the order inside a ring is
-	                       not the reference's, which changes no value, and
-	                       f64_updates is not counted); falls back to the host
-	                       build when a fan is longer than the sweep kernels'
-	                       register-held slots                                */
+	int device_setup;   /* ring schedule (coalesce = 1) and the learned and
+	                       Euclidean band schedules, clusters = 0: 1 = build the
+	                       rings (or bands) and the ring-numbered tables on the
+	                       device (a frontier BFS and a counting sort instead of
+	                       the host toplesets, band sort and renumbering; the
+	                       order inside a ring is not the host's, which changes
+	                       no value, and f64_updates is not counted); falls
+	                       back to the host build when a fan is longer than the
+	                       sweep kernels' register-held slots or a band index
+	                       would pass the vertex count                        */
 	int fp32_geometry;  /* gp_fp32_geometry (precision F32, ring schedules)     */
 } gp_ptp_opts;
 
