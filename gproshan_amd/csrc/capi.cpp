@@ -452,8 +452,9 @@ int gp_mlp_predict(gp_mlp *mlp, gp_che *mesh, const uint32_t *sources, uint32_t 
 {
 	GP_GUARD(
 		if (!mlp || !mesh || !sources || !n_sources || !dhat) return gp::fail(GP_ERR_INVALID, "null argument");
-		std::vector<uint32_t> lev(mesh->m.nv), srt(mesh->m.nv), lim;
-		int rc = gp::toplesets_host(mesh->m, sources, n_sources, lev.data(), srt.data(), lim);
+		// The ring-index feature: BFS levels of the sources, on the device.
+		std::vector<uint32_t> lev(mesh->m.nv);
+		int rc = gp::levels_gpu(mesh->m, sources, n_sources, lev.data());
 		if (rc) return rc;
 		return gp::mlp_predict(mlp, mesh->m, sources, n_sources, lev.data(), dhat, ms);
 	)

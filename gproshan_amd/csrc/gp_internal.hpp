@@ -157,6 +157,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 // of width `width` of dhat (the sources first), as the host band builder.
 int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out,
                        const double *dhat = nullptr, double width = 0);
+// BFS levels of the sources computed on the device (NIL where unreached).
+int levels_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
 int plan_result(gp_plan *p, double *dist);
 int plan_clusters(gp_plan *p, uint32_t *clusters);

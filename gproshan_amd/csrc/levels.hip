@@ -2,6 +2,8 @@
 #include "levels.hpp"
 
 #include <algorithm>
+#include <memory>
+#include <type_traits>
 
 #include "gp_internal.hpp"
 
@@ -390,6 +392,32 @@ void Levels::sort()
 	const uint32_t nc = nchunks();
 	k_lv_chunk_slots<<<fill_grid(64 * nc), 256, 0, stream>>>(sorted, limits, nl, dm->fan_off, dm->closed, nc, cslots);
 	k_exclusive_scan<<<1, 1024, 0, stream>>>(cslots, nc, coff);
+}
+
+// BFS levels of the sources (NIL where unreached) on the device, copied into
+// levels[nv]: the same sets as the host toplesets' levels.
+int levels_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels)
+{
+	if (!ns) return fail(GP_ERR_INVALID, "levels: no sources");
+	for (uint32_t q = 0; q < ns; q++)
+		if (src[q] >= m.nv) return fail(GP_ERR_INVALID, "levels: source %u out of range", src[q]);
+	int rc = upload_mesh(m);
+	if (rc) return rc;
+	hipStream_t stream = nullptr;
+	GP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+	std::unique_ptr<std::remove_pointer<hipStream_t>::type, hipError_t (*)(hipStream_t)> sg(stream, hipStreamDestroy);
+	uint32_t *d_src = nullptr;
+	if (hipMalloc(&d_src, 4ull * ns) != hipSuccess) return fail(GP_ERR_NOMEM, "levels: device allocation failed");
+	std::unique_ptr<uint32_t, hipError_t (*)(void *)> dg(d_src, hipFree);
+	GP_HIP(hipMemcpy(d_src, src, 4ull * ns, hipMemcpyHostToDevice));
+	Levels lv;
+	if ((rc = lv.create(m.dev, stream))) return rc;
+	lv.clear();
+	lv.seed(d_src, 0, ns);
+	if ((rc = lv.expand())) return rc;
+	GP_HIP(hipMemcpyAsync(levels, lv.level, 4ull * m.nv, hipMemcpyDeviceToHost, stream));
+	GP_HIP(hipStreamSynchronize(stream));
+	return GP_OK;
 }
 
 }  // namespace gp
