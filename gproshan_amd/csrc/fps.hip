@@ -322,16 +322,18 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
 	const bool ms8 = S <= 8;
-	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms8 ? k_fps_sweep<F64, 8> : k_fps_sweep<F64, MAXS>, 256, 0));
+	// One wave per workgroup, as the single-field sweeps.
+	constexpr uint32_t FB = 64;
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ms8 ? k_fps_sweep<F64, 8> : k_fps_sweep<F64, MAXS>, FB, 0));
 	const uint32_t grid = std::max<uint32_t>(8, std::min<uint32_t>((uint32_t) ncu * (uint32_t) std::max(1, occ),
-	                                                               (nchunks + 3) / 4));
+	                                                               nchunks));
 	constexpr uint32_t BIG = 64, SMALL = 8;
 	auto capture = [&](uint32_t count, hipGraphExec_t *out) -> int {
 		hipGraph_t g = nullptr;
 		GP_HIP(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
 		for (uint32_t q = 0; q < count; q++) {
-			if (ms8) k_fps_sweep<F64, 8><<<grid, 256, 0, r.stream>>>(a, q & 3u);
-			else k_fps_sweep<F64, MAXS><<<grid, 256, 0, r.stream>>>(a, q & 3u);
+			if (ms8) k_fps_sweep<F64, 8><<<grid, FB, 0, r.stream>>>(a, q & 3u);
+			else k_fps_sweep<F64, MAXS><<<grid, FB, 0, r.stream>>>(a, q & 3u);
 		}
 		GP_HIP(hipStreamEndCapture(r.stream, &g));
 		const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
