@@ -47,6 +47,8 @@ ERR_BAR = 1e-4
 # dispatches of one band_learned run) scaled by what the same counters read for
 # a 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
 # profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources).
+# Measured on an earlier build of the kernel (one slot stride per plan, four
+# waves per workgroup); the current tables hold fewer slot rows.
 MEASURED_TRAFFIC = {("band_learned", 1024, 1): (31678.4 * 2.0 + 1506.5 * 1.0) * 1024}
 
 
@@ -339,6 +341,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": round(traffic) if traffic else None,
+                     "traffic_note": ("rocprofv3 FETCH_SIZE/WRITE_SIZE of an earlier build of this kernel (one slot "
+                                      "stride per plan; the current tables hold fewer slot rows)") if traffic else None,
                      "algorithmic_bytes_per_launch": round(bytes_alg / launches),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false, false, 8u>"},
