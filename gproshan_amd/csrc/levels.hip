@@ -334,10 +334,11 @@ int Levels::create(const DeviceMesh *mesh, hipStream_t s)
 	GP_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
 	GP_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 	// Enough workgroups for a frontier of 64k vertices per pass.
-	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(256, (nv + 255) / 256));
+	// One wave per workgroup, as the sweeps.
+	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(1024, (nv + 63) / 64));
 	hipGraph_t g = nullptr;
 	GP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-	for (uint32_t q = 0; q < LV; q++) k_lv_step<<<grid, 256, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
+	for (uint32_t q = 0; q < LV; q++) k_lv_step<<<grid, 64, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
 	GP_HIP(hipStreamEndCapture(stream, &g));
 	const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
 	(void) hipGraphDestroy(g);
