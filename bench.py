@@ -104,6 +104,8 @@ def main():
                     help="rows around the hot path instead (FPS, clusters, iteration error, mesh readers)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 256 MiB five times (to scale FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--no-c3c5", action="store_true",
+                    help="leave out the batched (C3) and precision-stress (C5) results of the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -129,9 +131,9 @@ def main():
     if args.next:
         return next_rows(args, L, P, G)
 
+    from gproshan_amd import workloads as W
     t = time.time()
-    xyz, faces = G.height_field(args.n, args.seed)
-    src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), args.sources)
+    xyz, faces, src = W.c2(args.seed, args.sources, args.n)
     mesh = P.che(xyz, faces)
     log(f"[bench] mesh V={mesh.n_vertices()} F={mesh.n_faces()} source={src.tolist()} ({time.time() - t:.1f}s)")
 
@@ -151,11 +153,21 @@ def main():
         ("patches", dict(schedule=L.SCHED_PATCHES, **patch_kw)),
     ]
     only = set(filter(None, args.only.split(",")))
+    tuned = {}
     for name, kw in configs:
         if only and name not in only:
             continue
         t = time.time()
-        plan = P.Plan(mesh, src, **kw)
+        if name == "band_euclid":
+            # Bands of the Euclidean distance, width and launch shape tuned.
+            e = np.min(np.linalg.norm(xyz[:, None, :] - xyz[src][None, :, :], axis=2), axis=1)
+            bkw = {k: v for k, v in kw.items() if k != "band_width"}
+            plan, w = P.Plan.tuned_bands(mesh, src, e, **bkw)
+            tuned[name] = {"band_width": w, **plan.autotune()}
+        else:
+            plan = P.Plan(mesh, src, **kw)
+            if name != "patches":
+                tuned[name] = plan.autotune()
         setup_s = time.time() - t
         rows[name] = time_plan(plan, args.steps, args.warmup)
         rows[name]["setup_s"] = setup_s
@@ -182,7 +194,10 @@ def main():
         dhat, pred_ms = net.predict(mesh, src)
         pred_wall_ms = (time.perf_counter() - t_pred) * 1e3
         t = time.time()
-        plan = P.Plan(mesh, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, **band_kw)
+        # Band width and launch shape by the library's autotuners.
+        plan, w = P.Plan.tuned_bands(mesh, src, dhat, schedule=L.SCHED_LEARNED_BAND, tol=1e-6)
+        tuned["band_learned"] = {"band_width": w, **plan.autotune()}
+        band_kw = dict(band_width=w, tol=1e-6)
         setup_s = time.time() - t
         rows["band_learned"] = time_plan(plan, args.steps, args.warmup)
         rows["band_learned"]["setup_s"] = setup_s
@@ -205,6 +220,7 @@ def main():
                 C = P.che(xyz, faces)
                 t0 = time.perf_counter()
                 plan = P.Plan(C, src, precision=L.PREC_F32, **kw)
+                plan.autotune()            # the launch shape tuned above, from the cache
                 stx = plan.run()
                 plan.result()
                 plan.close()
@@ -238,6 +254,7 @@ def main():
                 dhat, _ = net.predict(C, src)
                 t1 = time.perf_counter()
                 plan = P.Plan(C, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, device_setup=dev, **band_kw)
+                plan.autotune()            # the launch shape tuned above, from the cache
                 t2 = time.perf_counter()
                 stx = plan.run()
                 plan.result()
@@ -250,7 +267,7 @@ def main():
                         "plan_ms": round(parts[k][1], 2), "kernel_ms": round(parts[k][2], 4)}
             log(f"[bench] end to end, {tag}: {e2e[tag]}")
     # The 16-source field of the same mesh (one multi-source field).
-    src16 = G.draw_sources(np.random.default_rng(args.seed + 2000), len(xyz), 16)
+    src16 = W.c2(args.seed, 16, args.n)[2]
     multi = {}
     if not only or "src16" in only:
         for name, kw in (("plain_ref_f32", dict(precision=L.PREC_F32)),
@@ -340,7 +357,9 @@ def main():
         "config": {"workload": f"height field {args.n}x{args.n} ({mesh.n_vertices()} vertices), {args.sources} "
                                f"source(s), seed "
                                f"{args.seed}; value = {headline}", "sources": src.tolist(),
-                   "band_width": args.band_width, "local_iters": args.local_iters, "patch": args.patch},
+                   "band_width": tuned.get(headline, {}).get("band_width"), "local_iters": args.local_iters,
+                   "patch": args.patch},
+        "autotuned": tuned,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 5),
                      "traffic": round(traffic) if traffic else None,
@@ -348,13 +367,15 @@ def main():
                                       "stride per plan; the current tables hold fewer slot rows)") if traffic else None,
                      "algorithmic_bytes_per_launch": round(bytes_alg / launches),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else "gp::k_sweep<false, false, 8u>"},
+                     "kernel": f"gp::k_patch_sweep<{args.patch}u>" if patch_row else L.sweep_kernel_name(st["kernel"])},
         "speedup_vs_plain_at_bar": (round(rows["plain_uncapped_f32"]["kernel_ms"] / h["kernel_ms"], 3)
                                     if "plain_uncapped_f32" in rows else None),
         "speedup_vs_plain_ref": (round(rows["plain_ref_f32"]["kernel_ms"] / h["kernel_ms"], 3)
                                  if "plain_ref_f32" in rows else None),
         "rows": {n: {"kernel_ms": round(r["kernel_ms"], 4), "kernel_ms_min": round(r["kernel_ms_min"], 4),
-                     "sweeps": r["st"]["sweeps"], "relaxations": r["st"]["relaxations"],
+                     "sweeps": r["st"]["sweeps"], "check_sweeps": r["st"]["check_sweeps"],
+                     "precision": "f64" if r["st"]["precision"] == L.PREC_F64 else "f32",
+                     "relaxations": r["st"]["relaxations"],
                      "f64_updates": r["st"]["f64_updates"], "patch_visits": r["st"]["patch_visits"],
                      "host_setup_s": round(r["setup_s"], 3),
                      "deferred": r["st"]["deferred"], **errs.get(n, {}),
@@ -375,6 +396,11 @@ def main():
                                for n, r in multi.items()}}
     if cpu:
         out["cpu_baseline"] = cpu
+    # The batched (C3) and precision-stress (C5) configurations, each timed
+    # over 3 launches after 1 warm-up and checked against the FP64 oracle.
+    if not args.no_c3c5 and not only:
+        out["c3"] = batch(args, L, P, G, steps=3, warmup=1, emit=False)
+        out["c5"] = stress(args, L, P, G, steps=3, warmup=1, emit=False)
     print(json.dumps(out), flush=True)
 
 
@@ -478,77 +504,72 @@ def next_rows(args, L, P, G):
     print(json.dumps(out), flush=True)
 
 
-def stress(args, L, P, G):
-    """2048 x 2048 grid stretched 1:100 in x, jitter U(-0.4, 0.4) cells, 4
-    sources: FP32 with the FP64 fallback vs all-FP64 GPU vs the FP64 oracle
-    (the reference schedule; the oracle runs on the host cores).  The FP32
-    row reads the geometry table the plan picks (the FP64 one here: most
-    corners are flagged); f32_fallback_geo32 forces the FP32 table (same
-    bits, checked)."""
+def stress(args, L, P, G, steps=None, warmup=None, emit=True):
+    """C5: 2048 x 2048 grid stretched 1:100 in x, jitter U(-0.4, 0.4) cells, 4
+    sources, the reference schedule, against the FP64 oracle (it runs on the
+    host cores).  Rows: f32_request (an FP32 request; over a quarter of this
+    mesh's triangles are flagged, so it runs FP64 sweeps), f32_forced (FP32
+    sweeps forced with the FP32 geometry table, the FP64 update on flagged
+    triangles), f64."""
+    from gproshan_amd import workloads as W
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t = time.time()
-    xyz, faces = G.anisotropic_grid(2048, 100.0, 0.4, args.seed)
-    src = G.draw_sources(np.random.default_rng(args.seed + 1000), len(xyz), 4)
+    xyz, faces, src = W.c5(args.seed)
     mesh = P.che(xyz, faces)
     log(f"[stress] V={mesh.n_vertices()} sources={src.tolist()} ({time.time() - t:.1f}s)")
     rows, traces = {}, {}
-    for name, kw in (("f32_fallback", dict(precision=L.PREC_F32)),
-                     ("f32_fallback_geo32", dict(precision=L.PREC_F32, fp32_geometry=L.GEO_FP32)),
+    for name, kw in (("f32_request", dict(precision=L.PREC_F32)),
+                     ("f32_forced", dict(precision=L.PREC_F32, fp32_geometry=L.GEO_FP32)),
                      ("f64", dict(precision=L.PREC_F64))):
         plan = P.Plan(mesh, src, **kw)
-        rows[name] = time_plan(plan, args.steps, args.warmup)
+        rows[name] = time_plan(plan, steps, warmup)
         traces[name] = plan.trace()
         plan.close()
-        log(f"[stress] {name}: {rows[name]['kernel_ms']:.2f} ms, sweeps {rows[name]['st']['sweeps']}")
+        log(f"[stress] {name}: {rows[name]['kernel_ms']:.2f} ms, sweeps {rows[name]['st']['sweeps']}, "
+            f"precision {rows[name]['st']['precision']}")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     om = O.Mesh(xyz, faces)
     t0 = time.perf_counter()
     ref, nsw = om.ptp(src, O.GPU_COAL, n_threads=os.cpu_count() or 1)
     cpu_s = time.perf_counter() - t0
-    # Where the FP32 run's schedule first departs from the FP64 run's (a ring
-    # test flipped by rounding moves values by about PTP_TOL).
-    t32, t64 = traces["f32_fallback"][:, 1:7], traces["f64"][:, 1:7]
+    # Where the forced-FP32 run's schedule first departs from the FP64 run's
+    # (a ring test flipped by rounding moves values by about PTP_TOL).
+    t32, t64 = traces["f32_forced"][:, 1:7], traces["f64"][:, 1:7]
     n = min(len(t32), len(t64))
     diff = np.flatnonzero(np.any(t32[:n] != t64[:n], axis=1))
     first_diff = int(diff[0]) + 1 if len(diff) else (None if len(t32) == len(t64) else n + 1)
     out = {"metric": "C5 precision stress: max relative error vs the FP64 oracle", "unit": "relative error",
-           "f32_schedule_matches_f64": first_diff is None, "f32_first_differing_sweep": first_diff,
+           "f32_forced_schedule_matches_f64": first_diff is None, "f32_forced_first_differing_sweep": first_diff,
            "config": {"workload": "anisotropic grid 2048x2048 (4194304 vertices), stretch 1:100, jitter 0.4, "
                                   f"4 sources, seed {args.seed}", "sources": src.tolist()},
            "oracle": {"sweeps": nsw, "seconds": round(cpu_s, 2), "cores": os.cpu_count()},
-           "geometry_tables_identical": bool(np.array_equal(rows["f32_fallback"]["dist"].view(np.uint64),
-                                                            rows["f32_fallback_geo32"]["dist"].view(np.uint64))),
            "rows": {}}
     for name, r in rows.items():
         mx, mn = rel_err(r["dist"], ref)
         st = r["st"]
-        out["rows"][name] = {"kernel_ms": round(r["kernel_ms"], 3), "sweeps": st["sweeps"], "max_rel_err": mx,
-                             "mean_rel_err": mn, "f64_updates": st["f64_updates"], "relaxations": st["relaxations"]}
+        gbs = algorithmic_bytes(st, st["precision"] == L.PREC_F64) / (r["kernel_ms"] * 1e-3) / 1e9
+        out["rows"][name] = {"kernel_ms": round(r["kernel_ms"], 3), "GB_per_s": round(gbs, 1),
+                             "sweeps": st["sweeps"], "precision": "f64" if st["precision"] == L.PREC_F64 else "f32",
+                             "max_rel_err": mx, "mean_rel_err": mn, "meets_1e-4": mx <= ERR_BAR,
+                             "f64_updates": st["f64_updates"], "relaxations": st["relaxations"]}
         log(f"[stress] {name}: max {mx:.3e} mean {mn:.3e} vs FP64 oracle ({nsw} sweeps, {cpu_s:.1f}s on the host)")
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def batch(args, L, P, G):
-    """256 meshes (grid height fields and displaced icospheres), vertex counts
-    log-uniform in [2k, 200k], U{1..64} sources each; every (mesh, source)
-    pair is its own one-source field, all in one persistent launch."""
-    import math
-    rng = np.random.default_rng(args.seed)
+def batch(args, L, P, G, steps=None, warmup=None, emit=True):
+    """C3: 256 meshes (grid height fields and displaced icospheres), vertex
+    counts log-uniform in [2k, 200k], U{1..64} sources each; every (mesh,
+    source) pair is its own one-source field, all in one persistent launch."""
+    from gproshan_amd import workloads as W
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t = time.time()
-    meshes, raw, jobs = [], [], []
-    for k in range(256):
-        nv = math.exp(rng.uniform(math.log(2000), math.log(200000)))
-        if rng.random() < 0.75:
-            rows = max(16, int(round(math.sqrt(nv) * rng.uniform(0.8, 1.25))))
-            cols = max(16, int(round(nv / rows)))
-            xyz, f = G.height_field_rect(rows, cols, seed=int(rng.integers(1 << 31)))
-        else:
-            level = int(min(range(4, 8), key=lambda l: abs(math.log(10 * 4 ** l + 2) - math.log(nv))))
-            xyz, f = G.icosphere(level, 0.01, seed=int(rng.integers(1 << 31)))
-        meshes.append(P.che(xyz, f))
-        raw.append((xyz, f))
-        for s in G.draw_sources(rng, len(xyz), int(rng.integers(1, 65))):
-            jobs.append((k, int(s)))
+    raw, jobs = W.c3(args.seed)
+    meshes = [P.che(xyz, f) for xyz, f in raw]
     nvf = sum(meshes[m].n_vertices() for m, _ in jobs)
 
     def max_neighbours(f):
@@ -562,38 +583,46 @@ def batch(args, L, P, G):
     t = time.time()
     b = P.Batch(meshes, jobs, block=0)
     log(f"[batch] created and autotuned in {time.time() - t:.1f}s")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.run()
     ms = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         st = b.run()
         ms.append(st["kernel_ms"])
     kms = float(np.mean(ms))
     gbs = st["relaxations"] * B_RELAX_F32 / (kms * 1e-3) / 1e9
     log(f"[batch] {kms:.2f} ms per launch, block {st['block']}, grid {st['grid']}, sweeps {st['sweeps']}, "
         f"relaxations {st['relaxations']}, {gbs:.1f} GB/s algorithmic")
-    # Spot-check the smallest fields against the FP64 oracle.
+    # Check fields against the FP64 oracle: 24 drawn from the batch (seeded).
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    small = sorted(range(len(jobs)), key=lambda q: meshes[jobs[q][0]].n_vertices())[:8]
-    worst = 0.0
-    for q in small:
+    pick = np.random.default_rng(args.seed + 3).choice(len(jobs), 24, replace=False)
+    worst, mean_sum, t0 = 0.0, 0.0, time.time()
+    for q in pick:
         m, s = jobs[q]
         ref, _ = O.Mesh(*raw[m]).ptp(np.array([s], np.uint32), O.GPU_PLAIN)
-        d, _ = b.result(q)
-        worst = max(worst, rel_err(d, ref)[0])
-    log(f"[batch] max relative error of the 8 smallest fields vs the FP64 oracle: {worst:.2e}")
+        d, _ = b.result(int(q))
+        mx, mn = rel_err(d, ref)
+        worst = max(worst, mx)
+        mean_sum += mn
+    log(f"[batch] max relative error of 24 drawn fields vs the FP64 oracle: {worst:.2e} ({time.time() - t0:.1f}s)")
+    b.close()
     out = {"metric": "C3 batched PTP: ms per launch of all fields", "value": round(kms, 3), "unit": "ms",
-           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "dtype": "f32",
+           "higher_is_better": False, "n_gpus": 1, "steps": steps, "warmup": warmup, "dtype": "f32",
            "data": "synthetic",
            "config": {"workload": f"256 meshes 2k-200k vertices, {len(jobs)} one-source fields, seed {args.seed}",
                       "vertex_fields": nvf, "block": st["block"], "grid": st["grid"]},
+           "kernel_ms": round(kms, 3), "GB_per_s": round(gbs, 1), "sweeps": st["sweeps"],
+           "relaxations": st["relaxations"],
            "fields_per_s": round(len(jobs) / (kms * 1e-3), 1),
            "vertex_fields_per_s": round(nvf / (kms * 1e-3), 1),
            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None, "kernel": f"gp::k_batch<{st['block']}u, {ms_slots}u>"},
-           "spot_check_max_rel_err": worst}
-    print(json.dumps(out), flush=True)
+           "max_rel_err_24_fields": worst, "mean_rel_err_24_fields": mean_sum / len(pick),
+           "meets_1e-4": worst <= ERR_BAR}
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

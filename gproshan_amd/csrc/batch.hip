@@ -213,7 +213,9 @@ struct gp_batch {
 	unsigned long long total_out = 0;
 	double tol = GP_PTP_TOL;
 	hipStream_t stream = nullptr;
-	hipGraphExec_t graph = nullptr;
+	hipGraphExec_t graph = nullptr;     // null when graphs are disabled
+	gp::BArgs args{};                    // the persistent launch (without a graph)
+	const void *fn = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 
 	void free_scratch()
@@ -299,6 +301,11 @@ static int batch_configure(gp_batch *b, uint32_t block)
 	}
 	BArgs a{ b->d_meshes, b->d_jobs, b->njobs, b->max_nv, b->d_queue, b->d_level, b->d_sorted, b->d_limits, b->d_nlim,
 	         b->d_key, b->d_dist, b->d_out, b->tol, b->d_fcid, b->d_fmidx, b->d_finv, b->d_fgeo, fslots };
+	b->args = a;
+	b->fn = fn;
+	b->grid = grid;
+	b->block = block;
+	if (!graphs_enabled()) return GP_OK;
 	void *kargs[] = { &a };
 	hipGraph_t gr = nullptr;
 	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -320,7 +327,13 @@ static int batch_configure(gp_batch *b, uint32_t block)
 static int batch_launch(gp_batch *b, float *ms)
 {
 	GP_HIP(hipEventRecord(b->e0, b->stream));
-	GP_HIP(hipGraphLaunch(b->graph, b->stream));
+	if (b->graph) {
+		GP_HIP(hipGraphLaunch(b->graph, b->stream));
+	} else {
+		void *kargs[] = { &b->args };
+		k_batch_reset<<<1, 1, 0, b->stream>>>(b->d_queue);
+		GP_HIP(hipLaunchKernel(b->fn, dim3(b->grid), dim3(b->block), kargs, 0, b->stream));
+	}
 	GP_HIP(hipEventRecord(b->e1, b->stream));
 	GP_HIP(hipStreamSynchronize(b->stream));
 	GP_HIP(hipEventElapsedTime(ms, b->e0, b->e1));

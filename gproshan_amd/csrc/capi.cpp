@@ -43,6 +43,7 @@ void gp_ptp_opts_default(gp_ptp_opts *o)
 	o->launch_mode = GP_LAUNCH_GRAPH;
 	o->band_width = 0.0;
 	o->local_iters = 8;
+	o->band_check = 1;
 }
 
 const char *gp_last_error(void) { return gp::last_error(); }
@@ -158,6 +159,23 @@ int gp_compute_toplesets(const gp_che *mesh, const uint32_t *sources, uint32_t n
 	)
 }
 
+int gp_compute_toplesets_k(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t k,
+                           uint32_t *toplesets, uint32_t *sorted, uint32_t *limits, uint32_t cap_limits,
+                           uint32_t *n_limits)
+{
+	GP_GUARD(
+		if (!mesh || !toplesets || !sorted || !limits || !n_limits || (n_sources && !sources))
+			return gp::fail(GP_ERR_INVALID, "null argument");
+		std::vector<uint32_t> lim;
+		int rc = gp::toplesets_host(mesh->m, sources, n_sources, toplesets, sorted, lim, k);
+		if (rc) return rc;
+		*n_limits = (uint32_t) lim.size();
+		if (lim.size() > cap_limits) return gp::fail(GP_ERR_CAPACITY, "limits needs %zu entries", lim.size());
+		std::memcpy(limits, lim.data(), sizeof(uint32_t) * lim.size());
+		return GP_OK;
+	)
+}
+
 int gp_compute_toplesets_gpu(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t *toplesets,
                              uint32_t *sorted, uint32_t *limits, uint32_t cap_limits, uint32_t *n_limits)
 {
@@ -179,8 +197,11 @@ static int ptp_gpu_driver(gp_che *mesh, const uint32_t *sources, uint32_t ns, co
                           const uint32_t *sorted, double *dist, uint32_t *clusters, double *seconds, int coalesce)
 {
 	if (!mesh || !dist) return gp::fail(GP_ERR_INVALID, "null argument");
+	// The reference's default build computes in FP64 (include/config.h:7-8):
+	// the drivers named after its functions return its bits.
 	gp_ptp_opts o;
 	gp_ptp_opts_default(&o);
+	o.precision = GP_PREC_F64;
 	o.coalesce = coalesce;
 	o.clusters = clusters ? 1 : 0;
 	gp_plan *p = nullptr;
@@ -339,10 +360,10 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 		dhat = d.data();
 	}
 	// Bands of predicted distance take the place of the rings in the
-	// reference's uncapped window loop.
+	// reference's uncapped window loop; the plan keeps the band schedule, which
+	// adds the check sweeps over every vertex at the end of a run.
 	const double width = (o.band_width > 0 ? o.band_width : 1.0) * m.mean_edge;
 	gp_ptp_opts ob = o;
-	ob.schedule = GP_SCHED_RINGS;
 	ob.coalesce = 1;
 	ob.uncapped = 1;
 	if (o.device_setup && !o.clusters && dhat &&
@@ -396,6 +417,59 @@ int gp_plan_create_bands(gp_che *mesh, const uint32_t *sources, uint32_t n_sourc
 }
 
 int gp_plan_run(gp_plan *plan, gp_ptp_stats *stats) { GP_GUARD(return gp::plan_run(plan, stats);) }
+int gp_plan_autotune(gp_plan *plan, gp_tune *out) { GP_GUARD(return gp::plan_autotune(plan, out);) }
+
+int gp_plan_create_bands_tuned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                               const gp_ptp_opts *opts, const double *widths, uint32_t n_widths, gp_plan **out,
+                               double *width_out)
+{
+	GP_GUARD(
+		if (!mesh || !sources || !out || !dhat) return gp::fail(GP_ERR_INVALID, "null argument");
+		*out = nullptr;
+		gp_ptp_opts o;
+		if (opts) o = *opts; else gp_ptp_opts_default(&o);
+		if (o.schedule == GP_SCHED_RINGS || o.schedule == GP_SCHED_PATCHES)
+			return gp::fail(GP_ERR_INVALID, "gp_plan_create_bands_tuned needs a band schedule");
+		static const double defaults[] = { 0.45, 0.5, 0.6, 0.75, 1.0 };
+		// With the default candidates, a width chosen before for this mesh
+		// shape is taken without timing, and a new choice is remembered.
+		const bool use_cache = !widths || !n_widths;
+		if (use_cache) { widths = defaults; n_widths = 5; }
+		double cached = 0;
+		if (use_cache && gp::tune_width_lookup(mesh->m, o.schedule, o.precision, &cached)) {
+			widths = &cached;
+			n_widths = 1;
+		}
+		std::unique_ptr<gp_plan, void (*)(gp_plan *)> best(nullptr, gp_plan_destroy);
+		double best_ms = 1e300, best_w = 0;
+		for (uint32_t q = 0; q < n_widths; q++) {
+			if (!(widths[q] > 0)) return gp::fail(GP_ERR_INVALID, "band width %g must be positive", widths[q]);
+			o.band_width = widths[q];
+			gp_plan *p = nullptr;
+			int rc = plan_create_any(mesh, sources, n_sources, dhat, o, &p);
+			if (rc) return rc;
+			std::unique_ptr<gp_plan, void (*)(gp_plan *)> cand(p, gp_plan_destroy);
+			gp_tune t{};
+			if ((rc = gp::plan_autotune(p, &t))) return rc;
+			// A cached shape was not timed here: time the plan with it.
+			if (t.candidates == 0) {
+				gp_ptp_stats st{};
+				double ms = 0;
+				for (int k = 0; k < 3 && !rc; k++) {
+					rc = gp::plan_run(p, &st);
+					if (k) ms += st.kernel_ms / 2;
+				}
+				if (rc) return rc;
+				t.kernel_ms = ms;
+			}
+			if (t.kernel_ms < best_ms) { best_ms = t.kernel_ms; best_w = widths[q]; best = std::move(cand); }
+		}
+		if (use_cache && n_widths > 1) gp::tune_width_store(mesh->m, o.schedule, o.precision, best_w);
+		if (width_out) *width_out = best_w;
+		*out = best.release();
+		return GP_OK;
+	)
+}
 int gp_plan_clusters(gp_plan *plan, uint32_t *clusters) { GP_GUARD(return gp::plan_clusters(plan, clusters);) }
 int gp_plan_result(gp_plan *plan, double *dist) { GP_GUARD(return gp::plan_result(plan, dist);) }
 int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap, uint32_t *n)

@@ -178,6 +178,38 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 	return GP_OK;
 }
 
+bool prefer_f64(const Che &m)
+{
+	if (m.n_flag_tri < 0) {
+		long long nflag = 0, nslot = 0;
+		#pragma omp parallel for schedule(static) reduction(+ : nflag, nslot)
+		for (int64_t vv = 0; vv < (int64_t) m.nv; vv++) {
+			const uint32_t v = (uint32_t) vv;
+			const uint32_t s0 = m.fan_off[v], n = m.fan_off[v + 1] - s0;
+			nslot += (n && m.fan_closed[v]) ? n - 1 : n;
+			const double *pv = &m.gt[3 * (size_t) v];
+			for (uint32_t k = 0; k + 1 < n; k++) {
+				const double *pa = &m.gt[3 * (size_t) m.fan[s0 + k]];
+				const double *pb = &m.gt[3 * (size_t) m.fan[s0 + k + 1]];
+				const double X0[3] = { pa[0] - pv[0], pa[1] - pv[1], pa[2] - pv[2] };
+				const double X1[3] = { pb[0] - pv[0], pb[1] - pv[1], pb[2] - pv[2] };
+				const double q0 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+				const double q1 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
+				const double g = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
+				const double cx = X0[1] * X1[2] - X0[2] * X1[1];
+				const double cy = X0[2] * X1[0] - X0[0] * X1[2];
+				const double cz = X0[0] * X1[1] - X0[1] * X1[0];
+				const double det = cx * cx + cy * cy + cz * cz;
+				const double qq = q0 * q1;
+				if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) nflag++;
+			}
+		}
+		m.n_fan_slots = nslot;
+		m.n_flag_tri = nflag;
+	}
+	return 4 * m.n_flag_tri > m.n_fan_slots;
+}
+
 int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Layout &L, uint32_t tile, uint32_t W,
                  uint32_t uniform_max, bool wrap)
 {
@@ -378,7 +410,7 @@ int band_order(const Che &m, const uint32_t *src, uint32_t ns, const uint32_t *l
 }
 
 int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels,
-                   uint32_t *sorted, std::vector<uint32_t> &limits)
+                   uint32_t *sorted, std::vector<uint32_t> &limits, uint32_t k)
 {
 	limits.clear();
 	if (ns == 0) return GP_OK;                                   // src/che.cpp:548
@@ -396,7 +428,14 @@ int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *lev
 	uint32_t level = 0;
 	for (uint32_t i = 0; i < p; i++) {
 		const uint32_t v = sorted[i];
-		if (levels[v] > level) { level++; limits.push_back(i); }
+		if (levels[v] > level) {
+			// With a level cut k the walk stops at the first vertex of level
+			// k + 1, whose level-(k + 1) discoveries stay in sorted and in the
+			// last ring (src/che.cpp:572-575, 592).
+			level++;
+			if (level > k) break;
+			limits.push_back(i);
+		}
 		const uint32_t *L = m.fan.data() + m.fan_off[v];
 		const uint32_t n = m.link_size(v);
 		const uint32_t lv = levels[v] + 1;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "gp_internal.hpp"
 
 namespace gp {
@@ -30,6 +32,36 @@ struct DeviceMesh {
 
 // Uploads the fan tables and FP64 positions of m once (m.dev caches them).
 int upload_mesh(Che &m);
+
+// Replayed launch sequences (sweeps, BFS levels, patch launches, the batch)
+// run from captured hipGraphs unless the environment sets
+// GPROSHAN_AMD_GRAPHS=0: then the same kernels are launched one by one on the
+// stream, with the same results (a profiler that cannot follow graph replays
+// traces these).
+inline bool graphs_enabled()
+{
+	static const bool on = [] {
+		const char *e = std::getenv("GPROSHAN_AMD_GRAPHS");
+		return !(e && e[0] == '0');
+	}();
+	return on;
+}
+
+// Copies a state record of n 32-bit words from device memory into pinned
+// host memory (hipHostMalloc, mapped into the device's address space); the
+// host reads it after an event recorded behind this launch.  The replay loops
+// take their per-replay snapshots this way instead of with hipMemcpyAsync.
+static __global__ void k_snapshot_words(const uint32_t *src, uint32_t *dst, uint32_t n)
+{
+	for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) dst[q] = src[q];
+}
+
+template <typename S>
+inline void snapshot(const S *dev, S *host, hipStream_t stream)
+{
+	static_assert(sizeof(S) % 4 == 0, "state records are whole words");
+	k_snapshot_words<<<1, 64, 0, stream>>>((const uint32_t *) dev, (uint32_t *) host, (uint32_t) (sizeof(S) / 4));
+}
 
 
 }  // namespace gp

@@ -341,7 +341,8 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		if (e != hipSuccess) return fail(GP_ERR_HIP, "fps: hipGraphInstantiate failed: %s", hipGetErrorString(e));
 		return GP_OK;
 	};
-	if ((rc = capture(BIG, &r.g_big)) || (rc = capture(SMALL, &r.g_small))) return rc;
+	const bool graphs = graphs_enabled();
+	if (graphs && ((rc = capture(BIG, &r.g_big)) || (rc = capture(SMALL, &r.g_small)))) return rc;
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(2048, (nv + 255) / 256));
 	const uint32_t gg = std::max<uint32_t>(1, std::min<uint32_t>(4096, (npos + 255) / 256));
 	const auto t0 = std::chrono::steady_clock::now();
@@ -366,15 +367,15 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		else
 			k_gather_tables<false, true><<<gg, 256, 0, r.stream>>>(lv.sorted, r.inv, &r.st->nr, npos, S, lv.coff, mt, r.cid,
 			                                                        r.cgeo, nullptr, r.midx, nullptr);
+		snapshot(r.st, r.h_st, r.stream);
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
 		GP_HIP(hipStreamSynchronize(r.stream));
 		const uint32_t nl = r.h_st->nl;
 		if (nl < 2 || nl > nv + 2) return fail(GP_ERR_KERNEL, "fps: bad ring count %u", nl);
 		// Sweeps: graph replays until the device reports done (at most
 		// 2*|limits| sweeps plus the replay that records the last outcome),
-		// two in flight: each replay is followed by a copy of the state into
-		// its pinned snapshot and an event, and the host waits for the older
+		// two in flight: each replay is followed by a kernel writing the state
+		// into its pinned snapshot and an event, and the host waits for the older
 		// replay while the newer one runs (a replay queued behind the one that
 		// finishes meets done slots and returns at once).  j grows by one ring
 		// per sweep, so big graphs run while the last seen j, one replay
@@ -383,9 +384,17 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		uint32_t launched = 0, j_seen = 2;
 		auto replay = [&](uint32_t s) -> int {
 			const bool big = nl - 1 > j_seen + BIG + (launched ? BIG : 0);
-			GP_HIP(hipGraphLaunch(big ? r.g_big : r.g_small, r.stream));
+			if (graphs) {
+				GP_HIP(hipGraphLaunch(big ? r.g_big : r.g_small, r.stream));
+			} else {
+				for (uint32_t q = 0; q < (big ? BIG : SMALL); q++) {
+					if (ms8) k_fps_sweep<F64, 8><<<grid, FB, 0, r.stream>>>(a, q & 3u);
+					else k_fps_sweep<F64, MAXS><<<grid, FB, 0, r.stream>>>(a, q & 3u);
+				}
+			}
 			launched += big ? BIG : SMALL;
-			GP_HIP(hipMemcpyAsync(snap + s, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
+			snapshot(r.st, snap + s, r.stream);
+			GP_HIP(hipGetLastError());
 			GP_HIP(hipEventRecord(r.ev[s], r.stream));
 			return GP_OK;
 		};
@@ -403,8 +412,8 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 		k_fps_max<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, nv, r.st);
 		k_fps_arg<T><<<ig, 256, 0, r.stream>>>(r.d0, r.d1, r.inv, nv, r.st);
 		k_fps_append<T><<<1, 1, 0, r.stream>>>(r.job, r.samples, r.st);
+		snapshot(r.st, r.h_st, r.stream);
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(r.h_st, r.st, sizeof(FState), hipMemcpyDeviceToHost, r.stream));
 		GP_HIP(hipStreamSynchronize(r.stream));
 		if (r.h_st->stop) break;
 		count++;

@@ -39,6 +39,9 @@ struct Che {
 	double mean_edge = 1.0;           // mean length of the fan edges
 
 	DeviceMesh *dev = nullptr;        // lazily uploaded device tables
+	// Fan triangles taking the FP64 update (the flag test of build_layout) and
+	// all fan slots of the closed-fan-without-repeat form; -1 until counted.
+	mutable long long n_flag_tri = -1, n_fan_slots = -1;
 
 	uint32_t n_tri(uint32_t v) const {
 		const uint32_t c = fan_off[v + 1] - fan_off[v];
@@ -73,6 +76,13 @@ struct Edge64 { double x, y, z; };     // FP64 edge vector a - v of a fan entry
 // test flags, are within 2.4e-7); on a 1:100 anisotropic grid 89% of the
 // corners are flagged.
 constexpr double FLAG_SIN2_MIN = 0.01, FLAG_COS_MIN = -0.97;
+
+// FP32 solves of a mesh whose flagged triangles exceed a quarter of its fan
+// slots run FP64 sweeps instead (gp_ptp_opts.fp32_geometry == GP_GEO_AUTO):
+// there most triangle updates already take the FP64 path, and an FP32 ring
+// test can flip the schedule (a 1:100 stretched grid departs from the FP64
+// schedule and ends 1.4e-3 off).  Counted once per mesh.
+bool prefer_f64(const Che &m);
 
 // ELL layout of a solve.  Positions (ring order: vertex order[pos], or pos
 // when the mesh is numbered by rings) are grouped 64 to a chunk, one lane per
@@ -133,8 +143,9 @@ int read_mesh(const char *path, std::vector<double> &xyz, std::vector<uint32_t> 
 int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf, Che &m);
 
 // Host BFS rings (che::compute_toplesets).  limits is resized.
+// k: the reference's level cut (NIL: none).
 int toplesets_host(const Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels,
-                   uint32_t *sorted, std::vector<uint32_t> &limits);
+                   uint32_t *sorted, std::vector<uint32_t> &limits, uint32_t k = NIL);
 
 // Vertices renumbered by rings (ptp_coalescence, src/geodesics_ptp.cpp:15-38):
 // the mesh of the reached vertices in ring order, plus inv (old -> new).
@@ -160,6 +171,11 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 // BFS levels of the sources computed on the device (NIL where unreached).
 int levels_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
+int plan_autotune(gp_plan *p, gp_tune *out);
+// The band-width autotuner's remembered choice per mesh shape (vertex count
+// rounded down to a power of two, longest link), schedule and precision.
+bool tune_width_lookup(const Che &mesh, int schedule, int precision, double *width);
+void tune_width_store(const Che &mesh, int schedule, int precision, double width);
 int plan_result(gp_plan *p, double *dist);
 int plan_clusters(gp_plan *p, uint32_t *clusters);
 int ptp_iter_error(Che *mesh, const uint32_t *src, uint32_t ns, const double *exact, bool f64, uint32_t *iter,

@@ -336,6 +336,8 @@ int Levels::create(const DeviceMesh *mesh, hipStream_t s)
 	// Enough workgroups for a frontier of 64k vertices per pass.
 	// One wave per workgroup, as the sweeps.
 	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(1024, (nv + 63) / 64));
+	lv_grid = grid;
+	if (!graphs_enabled()) return GP_OK;
 	hipGraph_t g = nullptr;
 	GP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
 	for (uint32_t q = 0; q < LV; q++) k_lv_step<<<grid, 64, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
@@ -356,15 +358,22 @@ void Levels::seed(const uint32_t *d_src, uint32_t first, uint32_t ns)
 }
 
 // Replays until the state reports done, two in flight: each replay is
-// followed by a copy of the state into its pinned snapshot and an event; the
-// host waits for the older replay while the newer one runs (a replay queued
-// behind the one that finishes meets done slots and returns at once).
+// followed by a kernel that writes the state into its pinned snapshot and an
+// event; the host waits for the older replay while the newer one runs (a
+// replay queued behind the one that finishes meets done slots and returns at
+// once).
 int Levels::expand()
 {
 	GP_HIP(hipGetLastError());
 	auto replay = [&](uint32_t s) -> int {
-		GP_HIP(hipGraphLaunch(graph, stream));
-		GP_HIP(hipMemcpyAsync(h_ls + s, ls, sizeof(LevelState), hipMemcpyDeviceToHost, stream));
+		if (graph) {
+			GP_HIP(hipGraphLaunch(graph, stream));
+		} else {
+			for (uint32_t q = 0; q < LV; q++)
+				k_lv_step<<<lv_grid, 64, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
+		}
+		snapshot(ls, h_ls + s, stream);
+		GP_HIP(hipGetLastError());
 		GP_HIP(hipEventRecord(ev[s], stream));
 		return GP_OK;
 	};

@@ -33,7 +33,8 @@ struct Levels {
 	uint32_t *coff = nullptr;     // [nchunks() + 1] their exclusive prefix (chunk c at row coff[c])
 	LevelState *ls = nullptr, *h_ls = nullptr;   // h_ls: two pinned snapshots
 	hipEvent_t ev[2] = { nullptr, nullptr };
-	hipGraphExec_t graph = nullptr;
+	hipGraphExec_t graph = nullptr;   // null when graphs are disabled
+	uint32_t lv_grid = 1;
 	hipStream_t stream = nullptr;   // borrowed
 	const DeviceMesh *dm = nullptr;
 

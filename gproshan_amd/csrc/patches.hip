@@ -494,7 +494,7 @@ int patch_run(PatchPlan *pp, gp_ptp_stats *stats)
 	if (!pp) return fail(GP_ERR_INVALID, "plan is null");
 	const auto t0 = std::chrono::steady_clock::now();
 	const PArgs a = patch_args(pp);
-	const bool graph = pp->opts.launch_mode == GP_LAUNCH_GRAPH;
+	const bool graph = pp->opts.launch_mode == GP_LAUNCH_GRAPH && graphs_enabled();
 	if (graph && !pp->graph) {
 		hipGraph_t g = nullptr;
 		GP_HIP(hipStreamBeginCapture(pp->stream, hipStreamCaptureModeThreadLocal));
@@ -516,7 +516,8 @@ int patch_run(PatchPlan *pp, gp_ptp_stats *stats)
 		else enqueue_patch(pp, a, PATCH_GRAPH);
 		launched += PATCH_GRAPH;
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(pp->h_state, pp->d_state, sizeof(PatchState), hipMemcpyDeviceToHost, pp->stream));
+		snapshot(pp->d_state, pp->h_state, pp->stream);
+		GP_HIP(hipGetLastError());
 		GP_HIP(hipStreamSynchronize(pp->stream));
 		if (pp->h_state->done) break;
 		if (launched >= pp->max_launch)
@@ -538,6 +539,10 @@ int patch_run(PatchPlan *pp, gp_ptp_stats *stats)
 		stats->deferred = s.deferred;
 		stats->kernel_ms = ms;
 		stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		stats->precision = GP_PREC_F32;
+		stats->capped = 0;
+		stats->kernel = 0;
+		stats->check_sweeps = 0;
 	}
 	return GP_OK;
 }

@@ -31,37 +31,23 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
+#include <vector>
 
 #include "device.hpp"
 #include "levels.hpp"
 #include "sweep_dev.hpp"
+#include "sweep_kernel.hpp"
 #include "update_step.hpp"
 
 namespace gp {
-
-// Vertex-major slot tables of a mesh in its own numbering (slot k of vertex v
-// at fan_off[v] + k; a closed fan stored without its repeated first entry
-// leaves its last fan entry unused), the source that device-built plans and
-// farthest-point sampling gather their ring-ordered tables from.  Built on the
-// device once per mesh and precision.
-struct MasterLayout {
-	uint32_t *cid = nullptr;
-	float4 *cgeo = nullptr;
-	double4 *cgeo64 = nullptr;
-	Edge64 *cx64 = nullptr;
-	uint32_t max_slots = 0;
-	VertexTables view(const DeviceMesh *dm) const
-	{
-		return VertexTables{ dm->fan_off, dm->closed, cid, cgeo, cgeo64, cx64 };
-	}
-	~MasterLayout()
-	{
-		for (void *p : { (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
-	}
-};
 
 void device_mesh_free(DeviceMesh *d)
 {
@@ -114,213 +100,6 @@ int upload_mesh(Che &m)
 		return cleanup(fail(GP_ERR_HIP, "device mesh upload failed"));
 	m.dev = dm.release();
 	return GP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// Kernels
-
-// Schedule state in device memory.  Sweep kernel k of a run (k counted from
-// 0) reads the state of sweep k-1 from slot[(k-1) & 3] together with its
-// non-converged count bad[(k-1) & 3], advances it (i, j, d as the reference
-// loop does), publishes it in slot[k & 3] (workgroup 0), zeroes bad[(k+1) & 3]
-// for the next sweep, and relaxes its own window counting into bad[k & 3].
-// No workgroup reads a slot or counter while another workgroup of the same
-// launch writes it, so kernel boundaries are the only synchronisation.  The
-// captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
-//
-// A slot also carries its sweep's window in positions (ws, wc, we: start, end
-// of the convergence-tested ring, end) and, written at the end of the sweep,
-// the two windows the next sweep can have (ring i kept or retired: cs/cc[0|1],
-// end ce), so that the next sweep knows its window from the slot alone,
-// without a dependent read of limits.
-struct Slot {
-	uint32_t i, j, d, iter, done, fresh, ws, wc;
-	uint32_t we, cs0, cc0, cs1, cc1, ce, pad0, pad1;
-};
-struct PState {
-	Slot slot[4];
-	uint32_t bad[4];
-	uint32_t done, sweeps, final_d, pad;
-	unsigned long long relax, cond, f64;
-};
-
-struct KArgs {
-	const uint32_t *limits;
-	const uint32_t *coff;      // per chunk: first slot row
-	uint32_t sb;               // > 0: every chunk has sb slots (chunk c at row c * sb; coff unused)
-	const uint32_t *cid;
-	const float4 *cgeo;
-	const double4 *cgeo64;
-	const Edge64 *cx64;
-	const unsigned long long *fcum;  // FP64-update triangles of positions [0, pos)
-	const uint32_t *order;     // position -> vertex; null when the mesh is numbered by rings
-	void *dist0, *dist1;
-	PState *st;
-	uint32_t *trace;           // 8 words per sweep, or null
-	double tol;
-	float h, inv_h;
-	uint32_t nl, max_iter;
-	uint32_t k2;               // 1: K/2 limit (the drivers); 0: the uncapped iter_error loop
-	uint32_t *cl0, *cl1;       // cluster labels (double-buffered like the distances), or null
-};
-
-// Relaxes the window [start, end) (positions); returns the wave's count of
-// non-converged vertices among positions [start, cend).
-template <bool F64, bool CL, uint32_t MS, bool G64>
-__device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, uint32_t end, uint32_t cend,
-                                               const typename std::conditional<F64, double, float>::type *od,
-                                               typename std::conditional<F64, double, float>::type *nd)
-{
-	using T = typename std::conditional<F64, double, float>::type;
-	if (start >= end) return 0;
-	const uint32_t c0 = start >> 6, c1 = (end + 63u) >> 6;
-	const uint32_t lane = threadIdx.x & 63u;
-	// XCD-aware split: workgroups b with b % 8 == x run on XCD x and take the
-	// x-th contiguous eighth of the window's chunks.
-	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
-	const uint32_t x = blockIdx.x & 7u;
-	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
-	const uint32_t wpb = blockDim.x >> 6;
-	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
-	const uint32_t nw_x = nbx * wpb;
-	const uint32_t len = c1 - c0;
-	const uint32_t part = (len + ngrp - 1) / ngrp;
-	const uint32_t lo = c0 + x * part;
-	const uint32_t hi = min(c1, lo + part);
-	uint32_t bad = 0;
-	for (uint32_t c = lo + lw; c < hi; c += nw_x) {
-		const uint32_t pos = (c << 6) + lane;
-		const uint32_t cb = a.sb ? c * a.sb : a.coff[c], S = a.sb ? a.sb : a.coff[c + 1] - cb;
-		bool lbad = false;
-		if (pos >= start && pos < end) {
-			const uint32_t v = a.order ? a.order[pos] : pos;
-			const T old = od[v];
-			const size_t base = (size_t) cb * 64 + lane;
-			uint32_t bk = NO_TRI;
-			const T best = relax_lane<F64, 64, true, CL, MS, G64>(a, base, S, od, old, &bk);
-			nd[v] = best;
-			if constexpr (CL) {
-				// The label of the nearer of the winning triangle's two other
-				// vertices (old values; ties to the first fan entry), as the
-				// reference's GPU relax does (src/cuda/geodesics_ptp.cu:266-276).
-				const uint32_t *ocl = od == (const T *) a.dist0 ? a.cl0 : a.cl1;
-				uint32_t *ncl = od == (const T *) a.dist0 ? a.cl1 : a.cl0;
-				uint32_t c = ocl[v];
-				if (bk != NO_TRI) {
-					const uint32_t ia = a.cid[base + 64 * (size_t) bk] & SLOT_ID_MASK;
-					const uint32_t nb = bk + 1 < S ? a.cid[base + 64 * (size_t) (bk + 1)] : SLOT_EMPTY;
-					const uint32_t ib = ((nb & SLOT_EMPTY) ? a.cid[base] : nb) & SLOT_ID_MASK;
-					c = od[ib] < od[ia] ? ocl[ib] : ocl[ia];
-				}
-				ncl[v] = c;
-			}
-			if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
-		}
-		bad += (uint32_t) __popcll(__ballot(lbad));
-	}
-	return bad;
-}
-
-// One wave per workgroup: waves of a sweep finish and free their slots
-// independently, and the XCD split deals the window out in finer pieces
-// (measured on the 1M field against 4 waves per workgroup: 2.5 % faster on
-// the reference schedule, 4 % on the learned bands).
-constexpr uint32_t SWEEP_BLOCK = 64;
-
-// Sweep k (k4 = k & 3) of a run: advance the schedule, then relax its window.
-template <bool F64, bool CL, uint32_t MS, bool G64 = false>
-__global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
-{
-	using T = typename std::conditional<F64, double, float>::type;
-	PState *st = a.st;
-	const uint32_t kp = (k4 + 3u) & 3u;
-	const Slot prev = st->slot[kp];
-	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-	if (prev.done) {
-		// Later launches of the replay read slot k & 3 in turn: keep it done.
-		if (lead) st->slot[k4] = prev;
-		return;
-	}
-	Slot cur = prev;
-	cur.fresh = 0;
-	if (!prev.fresh) {
-		// The previous sweep's outcome: the reference loop tail.
-		const uint32_t nbad = st->bad[kp];
-		const uint32_t pi = a.k2 && prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
-		const bool adv = nbad == 0;
-		cur.i = pi + (adv ? 1u : 0u);
-		cur.j = prev.j + (prev.j < a.nl - 1 ? 1u : 0u);
-		cur.d = prev.d ^ 1u;
-		cur.iter = prev.iter + 1;
-		cur.done = !(cur.i < cur.j) || !(cur.iter < a.max_iter);
-		cur.ws = adv ? prev.cs1 : prev.cs0;
-		cur.wc = adv ? prev.cc1 : prev.cc0;
-		cur.we = prev.ce;
-		if (lead) {
-			const uint32_t l0 = prev.ws, lj = prev.we;
-			const uint32_t ncond = prev.wc - prev.ws;
-			if (a.trace) {
-				uint32_t *r = a.trace + 8 * (size_t) prev.iter;
-				r[0] = prev.iter + 1; r[1] = pi; r[2] = prev.j; r[3] = l0; r[4] = lj; r[5] = ncond;
-				r[6] = ncond - nbad; r[7] = 0;
-			}
-			st->relax += lj - l0;
-			st->cond += ncond;
-			if (a.fcum) st->f64 += a.fcum[lj] - a.fcum[l0];
-			st->sweeps = cur.iter;
-		}
-	}
-	const uint32_t i = a.k2 && cur.i < (cur.j >> 1) ? (cur.j >> 1) : cur.i;   // K/2 limit band size
-	if (prev.fresh && !cur.done) {
-		cur.ws = a.limits[i]; cur.wc = a.limits[i + 1]; cur.we = a.limits[cur.j];
-	}
-	// The next sweep's two possible windows (read here, stored at the end).
-	uint32_t nx[5] = { 0, 0, 0, 0, 0 };
-	if (lead && !cur.done) {
-		const uint32_t jn = cur.j + (cur.j < a.nl - 1 ? 1u : 0u);
-		for (uint32_t adv = 0; adv < 2; adv++) {
-			const uint32_t in = i + adv;
-			const uint32_t iw = a.k2 && in < (jn >> 1) ? (jn >> 1) : in;
-			if (in < jn) { nx[2 * adv] = a.limits[iw]; nx[2 * adv + 1] = a.limits[iw + 1]; }
-		}
-		nx[4] = a.limits[jn];
-	}
-	if (lead) {
-		st->slot[k4] = cur;
-		st->bad[(k4 + 1u) & 3u] = 0;
-		if (cur.done) { st->final_d = cur.d; st->done = 1; }
-	}
-	if (cur.done) return;
-	const uint32_t start = cur.ws, cend = cur.wc, end = cur.we;
-	const T *od = (const T *) (cur.d ? a.dist1 : a.dist0);
-	T *nd = (T *) (cur.d ? a.dist0 : a.dist1);
-	const uint32_t bad = sweep_body<F64, CL, MS, G64>(a, start, end, cend, od, nd);
-	if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(&st->bad[k4], bad);
-	if (lead) {
-		Slot *sl = &st->slot[k4];
-		sl->cs0 = nx[0]; sl->cc0 = nx[1]; sl->cs1 = nx[2]; sl->cc1 = nx[3]; sl->ce = nx[4];
-	}
-}
-
-template <typename T>
-__global__ void k_init(T *d0, T *d1, uint32_t n, PState *st, uint32_t done)
-{
-	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-	for (uint32_t v = t; v < n; v += gridDim.x * blockDim.x) { d0[v] = (T) INFINITY; d1[v] = (T) INFINITY; }
-	if (t == 0) {
-		uint32_t *sw = (uint32_t *) st;
-		for (uint32_t q = 0; q < sizeof(PState) / 4; q++) sw[q] = 0;
-		// The state "before" sweep 0 lives in slot 3 and is taken as is.
-		st->slot[3].i = 1; st->slot[3].j = 2; st->slot[3].fresh = 1; st->slot[3].done = done;
-		st->done = done;
-	}
-}
-
-template <typename T>
-__global__ void k_sources(T *d0, T *d1, const uint32_t *src, uint32_t ns)
-{
-	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < ns) { d0[src[t]] = (T) 0; d1[src[t]] = (T) 0; }
 }
 
 // Cluster labels: NIL everywhere, then source q gets q + 1
@@ -429,6 +208,7 @@ struct gp_plan {
 	uint32_t nchunks = 0, max_slots = 0, sb = 0;
 	bool ms8 = false;                        // every fan fits 8 slots: the 8-slot kernels
 	bool g64 = false;                        // FP32 sweeps read the FP64 geometry table (GP_GEO_FP64)
+	bool band = false;                       // band schedule: final check sweeps over every position
 	unsigned long long n_flagged = 0;
 	gp::PatchPlan *pp = nullptr;             // patch / band schedules
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
@@ -441,18 +221,22 @@ struct gp_plan {
 	uint32_t *d_cl0 = nullptr, *d_cl1 = nullptr;   // cluster labels (opts.clusters)
 	uint32_t *d_inv = nullptr;                // device-built plans: original vertex -> position (NIL: unreached)
 	double *d_out = nullptr;
+	double *d_orig = nullptr;                 // device-built plans: the result in the original numbering
 	gp::PState *d_state = nullptr, *h_state = nullptr;   // h_state: 2 pinned snapshots
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = { nullptr, nullptr };
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
+	uint32_t big = 64, small = 8;            // sweeps per captured graph (multiples of 4)
 	bool has_run = false;
+	bool capped = false;                     // the last run stopped at its sweep cap
+	uint32_t check_sweeps = 0;               // check sweeps of the last run (band plans)
 
 	~gp_plan()
 	{
 		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
 		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
 		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_cl0, (void *) d_cl1,
-		                 (void *) d_inv })
+		                 (void *) d_inv, (void *) d_orig })
 			(void) hipFree(p);
 		if (pp) gp::patch_destroy(pp);
 		if (g_big) (void) hipGraphExecDestroy(g_big);
@@ -525,7 +309,22 @@ static int plan_run_buffers(gp_plan *p, bool f64)
 	return GP_OK;
 }
 
-// Builds a plan from given rings (limits/sorted in the original numbering).
+// The precision a plan runs in: FP32 requests on meshes with more than a
+// quarter of their fan triangles flagged run FP64 sweeps (GP_GEO_AUTO).
+static bool plan_f64(const Che &mesh, const gp_ptp_opts &opts)
+{
+	if (opts.precision == GP_PREC_F64) return true;
+	return opts.fp32_geometry == GP_GEO_AUTO && prefer_f64(mesh);
+}
+
+static bool ring_like(int schedule)
+{
+	return schedule == GP_SCHED_RINGS || schedule == GP_SCHED_LEARNED_BAND || schedule == GP_SCHED_EUCLID_BAND ||
+	       schedule == GP_SCHED_RING_BAND;
+}
+
+// Builds a plan from given rings (limits/sorted in the original numbering;
+// for the band schedules, the bands in the order to sweep them).
 int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_t *limits, uint32_t nl,
                       const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out)
 {
@@ -542,9 +341,14 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 		if (sorted[q] >= mesh->nv) return fail(GP_ERR_INVALID, "plan: sorted[%u] out of range", q);
 	for (uint32_t q = 0; q < ns; q++)
 		if (src[q] >= mesh->nv) return fail(GP_ERR_INVALID, "plan: source %u out of range", src[q]);
+	if (opts.precision != GP_PREC_F32 && opts.precision != GP_PREC_F64)
+		return fail(GP_ERR_INVALID, "plan: unknown precision %d", opts.precision);
+	if (!ring_like(opts.schedule)) return fail(GP_ERR_INVALID, "plan_create_rings: ring or band schedule only");
 	auto p = std::make_unique<gp_plan>();
 	p->mesh = mesh;
 	p->opts = opts;
+	p->opts.precision = plan_f64(*mesh, opts) ? GP_PREC_F64 : GP_PREC_F32;
+	p->band = opts.schedule != GP_SCHED_RINGS && opts.band_check;
 	p->nv_orig = mesh->nv;
 	p->nl = nl;
 	p->ns = ns;
@@ -582,8 +386,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (rc) return rc;
 	p->dm = use->dev;
 	p->n = use->nv;
-	const bool f64 = opts.precision == GP_PREC_F64;
-	if (opts.schedule != GP_SCHED_RINGS) return fail(GP_ERR_INVALID, "plan_create_rings: ring schedule only");
+	const bool f64 = p->opts.precision == GP_PREC_F64;
 	Layout L;
 	rc = build_layout(*use, opts.coalesce ? nullptr : sorted, nr, f64, L, 0, 64, 0, true);
 	if (rc) return rc;
@@ -591,8 +394,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	p->max_slots = L.max_slots;
 	p->sb = L.uniform;
 	p->n_flagged = L.n_flagged;
-	p->g64 = !f64 && (opts.fp32_geometry == GP_GEO_FP64 ||
-	                  (opts.fp32_geometry == GP_GEO_AUTO && 4 * L.n_flagged > L.n_slots));
+	p->g64 = !f64 && opts.fp32_geometry == GP_GEO_FP64;
 	// The drivers cap the loop at 2*|limits| sweeps (src/geodesics_ptp.cpp:66-67);
 	// the uncapped loop (src/cuda/test_geodesics_ptp_coalescence.cu:223-248)
 	// has no cap, so a generous one stands in as a guard.
@@ -731,8 +533,10 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	*out = nullptr;
 	if (!mesh || !src) return fail(GP_ERR_INVALID, "plan: null argument");
 	if (ns == 0) return fail(GP_ERR_INVALID, "plan: no sources");
-	if (opts.schedule != GP_SCHED_RINGS || !opts.coalesce || opts.clusters)
-		return fail(GP_ERR_INVALID, "plan_create_device: ring schedule, coalesced, without labels only");
+	if (!ring_like(opts.schedule) || !opts.coalesce || opts.clusters)
+		return fail(GP_ERR_INVALID, "plan_create_device: ring or band schedule, coalesced, without labels only");
+	if (opts.precision != GP_PREC_F32 && opts.precision != GP_PREC_F64)
+		return fail(GP_ERR_INVALID, "plan: unknown precision %d", opts.precision);
 	{
 		std::vector<uint8_t> seen(mesh->nv, 0);
 		for (uint32_t q = 0; q < ns; q++) {
@@ -743,7 +547,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	}
 	int rc = upload_mesh(*mesh);
 	if (rc) return rc;
-	const bool f64 = opts.precision == GP_PREC_F64;
+	const bool f64 = plan_f64(*mesh, opts);
 	DeviceMesh *dm = mesh->dev;
 	VertexTables mt{};
 	uint32_t S = 0;
@@ -752,6 +556,8 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	auto p = std::make_unique<gp_plan>();
 	p->mesh = mesh;
 	p->opts = opts;
+	p->opts.precision = f64 ? GP_PREC_F64 : GP_PREC_F32;
+	p->band = opts.schedule != GP_SCHED_RINGS && opts.band_check;
 	p->nv_orig = mesh->nv;
 	p->ns = ns;
 	p->dm = dm;
@@ -812,6 +618,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	const size_t lanes = (size_t) rows * 64;
 	if (hipMalloc(&p->d_limits, 4ull * nl) != hipSuccess || hipMalloc(&p->d_src, 4ull * ns) != hipSuccess ||
 	    hipMalloc(&p->d_inv, 4ull * mesh->nv) != hipSuccess ||
+	    hipMalloc(&p->d_orig, sizeof(double) * mesh->nv) != hipSuccess ||
 	    hipMalloc(&p->d_coff, 4ull * (p->nchunks + 1)) != hipSuccess ||
 	    hipMalloc(&p->d_cid, 4 * std::max<size_t>(lanes, 1)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
@@ -852,15 +659,16 @@ static KArgs make_args(const gp_plan *p)
 	a.st = p->d_state; a.trace = p->d_trace;
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
 	a.nl = p->nl; a.max_iter = p->max_iter; a.k2 = p->opts.uncapped ? 0u : 1u;
+	a.npos = p->n;
 	a.cl0 = p->d_cl0; a.cl1 = p->d_cl1;
 	return a;
 }
 
-// Enqueues sweeps k0 .. k0+count-1 (their k & 3 slots).
-static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32_t count)
+// Launches the plan's sweep kernel variant with argument k4 (a sweep slot,
+// or 4 + d for a check sweep from buffer d).
+static void launch_sweep(const gp_plan *p, const KArgs &a, uint32_t k4)
 {
-	for (uint32_t q = 0; q < count; q++) {
-		const uint32_t k4 = (k0 + q) & 3u;
+	{
 		const bool cl = a.cl0 != nullptr;
 		if (p->opts.precision == GP_PREC_F64) {
 			if (cl && p->ms8) k_sweep<true, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
@@ -879,6 +687,19 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 			else k_sweep<false, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
 		}
 	}
+}
+
+// Enqueues sweeps k0 .. k0+count-1 (their k & 3 slots).
+static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32_t count)
+{
+	for (uint32_t q = 0; q < count; q++) launch_sweep(p, a, (k0 + q) & 3u);
+}
+
+// Sweep kernel variant bits of gp_ptp_stats.kernel.
+static uint32_t kernel_bits(const gp_plan *p)
+{
+	return (p->opts.precision == GP_PREC_F64 ? 1u : 0u) | (p->d_cl0 ? 2u : 0u) | (p->ms8 ? 4u : 0u) |
+	       (p->g64 && p->opts.precision != GP_PREC_F64 ? 8u : 0u);
 }
 
 static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *out)
@@ -902,13 +723,14 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	const uint32_t done0 = p->nl < 3 ? 1u : 0u;
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));
 	const KArgs a = make_args(p);
-	constexpr uint32_t BIG = 64, SMALL = 8;
-	const bool graph = p->opts.launch_mode == GP_LAUNCH_GRAPH;
+	const uint32_t BIG = p->big, SMALL = p->small;
+	const bool graph = p->opts.launch_mode == GP_LAUNCH_GRAPH && graphs_enabled();
 	if (graph && !p->g_big) {
 		int rc = capture(p, a, BIG, &p->g_big);
 		if (!rc) rc = capture(p, a, SMALL, &p->g_small);
 		if (rc) return rc;
 	}
+	p->has_run = false;
 	if (f64) {
 		k_init<double><<<ig, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->n, p->d_state, done0);
 		k_sources<double><<<(p->ns + 255) / 256, 256, 0, p->stream>>>((double *) p->d_dist0, (double *) p->d_dist1, p->d_src, p->ns);
@@ -923,14 +745,15 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	GP_HIP(hipGetLastError());
 	GP_HIP(hipEventRecord(p->e0, p->stream));
 	// Replays until the device reports done, two replays in flight: after
-	// each replay the state is copied to its pinned snapshot and an event is
-	// recorded; the host waits for the older replay's event while the newer
-	// one runs, so the GPU does not idle on the poll.  The device time ends
-	// at the event of the replay whose state says done (a replay queued
-	// behind it only meets done slots and returns at once).  One sweep more
-	// than the sweep count runs (it records the last sweep's outcome).  j
-	// grows by one ring per sweep until |limits|-1, so at least nl-1-j sweeps
-	// remain: big graphs while that bound allows, small ones near the end.
+	// each replay a one-wave kernel writes the state into the replay's pinned
+	// snapshot and an event is recorded; the host waits for the older
+	// replay's event while the newer one runs, so the GPU does not idle on
+	// the poll.  The device time ends at the event of the replay whose state
+	// says done (a replay queued behind it only meets done slots and returns
+	// at once).  One sweep more than the sweep count runs (it records the
+	// last sweep's outcome).  j grows by one ring per sweep until |limits|-1,
+	// so at least nl-1-j sweeps remain: big graphs while that bound allows,
+	// small ones near the end.
 	uint32_t launched = 0, j_seen = 2;
 	const uint32_t cap = p->max_iter;
 	auto replay = [&](uint32_t s) -> int {
@@ -939,8 +762,8 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 		if (graph) GP_HIP(hipGraphLaunch(big ? p->g_big : p->g_small, p->stream));
 		else enqueue_sweeps(p, a, launched, cnt);
 		launched += cnt;
+		snapshot(p->d_state, p->h_state + s, p->stream);
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(p->h_state + s, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipEventRecord(p->ev[s], p->stream));
 		return GP_OK;
 	};
@@ -965,25 +788,66 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	GP_HIP(hipStreamSynchronize(p->stream));
 	if (!done) return fail(GP_ERR_KERNEL, "ptp: schedule did not finish in %u sweeps", launched);
 	if (done0) {
-		GP_HIP(hipEventRecord(p->e1, p->stream));
-		GP_HIP(hipMemcpyAsync(p->h_state + cur, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
-		GP_HIP(hipStreamSynchronize(p->stream));
+		snapshot(p->d_state, p->h_state + cur, p->stream);
+		GP_HIP(hipGetLastError());
 	}
+	GP_HIP(hipEventRecord(p->e1, p->stream));
+	GP_HIP(hipStreamSynchronize(p->stream));
 	if (cur != 0) std::memcpy(p->h_state, p->h_state + cur, sizeof(PState));
+	// The loop stopped at its cap when the last sweep's counter reached it
+	// with ring i still below j.
+	{
+		const Slot &last = p->h_state->slot[3];
+		p->capped = !done0 && last.iter >= p->max_iter && last.i < last.j;
+	}
+	// Band schedules: check sweeps over every position until one of them
+	// moves no vertex by tol or more (a prediction that put a vertex in a
+	// band ahead of its upwind neighbours cannot leave it unconverged).
+	p->check_sweeps = 0;
+	uint32_t fd = p->h_state->final_d;
+	if (p->band) {
+		const uint32_t check_cap = 4 * p->nl + 64;
+		for (;;) {
+			k_fill_u32<<<1, 64, 0, p->stream>>>(&p->d_state->check_bad, 1, 0);
+			launch_sweep(p, a, 4u + fd);
+			snapshot(p->d_state, p->h_state + 1, p->stream);
+			GP_HIP(hipGetLastError());
+			GP_HIP(hipStreamSynchronize(p->stream));
+			fd ^= 1u;
+			p->check_sweeps++;
+			if (p->h_state[1].check_bad == 0) break;
+			if (p->check_sweeps >= check_cap) {
+				p->h_state->final_d = fd;
+				return fail(GP_ERR_NOT_CONVERGED, "ptp: %u vertices still move after %u check sweeps",
+				            p->h_state[1].check_bad, p->check_sweeps);
+			}
+		}
+		GP_HIP(hipEventRecord(p->e1, p->stream));
+		GP_HIP(hipStreamSynchronize(p->stream));
+		p->h_state->final_d = fd;
+	}
 	float ms = 0;
-	GP_HIP(hipEventElapsedTime(&ms, p->e0, done0 ? p->e1 : p->ev[cur]));
+	GP_HIP(hipEventElapsedTime(&ms, p->e0, p->band || done0 ? p->e1 : p->ev[cur]));
 	p->has_run = true;
 	if (stats) {
 		stats->sweeps = p->h_state->sweeps;
 		stats->n_limits = p->nl;
-		stats->relaxations = p->h_state->relax;
-		stats->cond_checks = p->h_state->cond;
+		stats->relaxations = p->h_state->relax + (unsigned long long) p->check_sweeps * p->n;
+		stats->cond_checks = p->h_state->cond + (unsigned long long) p->check_sweeps * p->n;
 		stats->f64_updates = p->h_state->f64;
 		stats->patch_visits = 0;
 		stats->deferred = 0;
 		stats->kernel_ms = ms;
 		stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+		stats->precision = p->opts.precision;
+		stats->capped = p->capped ? 1 : 0;
+		stats->kernel = kernel_bits(p);
+		stats->check_sweeps = p->check_sweeps;
 	}
+	// The uncapped loop has no cap of its own: reaching the guard means the
+	// field did not converge (band plans were settled by the check sweeps).
+	if (p->capped && p->opts.uncapped && !p->band)
+		return fail(GP_ERR_NOT_CONVERGED, "ptp: the uncapped loop reached its guard of %u sweeps", p->max_iter);
 	return GP_OK;
 }
 
@@ -1002,15 +866,11 @@ int plan_result(gp_plan *p, double *dist)
 		k_to_f64<float><<<g, 256, 0, p->stream>>>((const float *) buf, p->d_out, p->n);
 	GP_HIP(hipGetLastError());
 	if (p->d_inv) {
-		double *d_orig = nullptr;
-		if (hipMalloc(&d_orig, sizeof(double) * p->nv_orig) != hipSuccess)
-			return fail(GP_ERR_NOMEM, "plan_result: allocation failed");
 		const uint32_t go = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->nv_orig + 255) / 256));
-		k_from_positions<<<go, 256, 0, p->stream>>>(p->d_out, p->d_inv, p->nv_orig, d_orig);
-		const hipError_t e1 = hipMemcpyAsync(dist, d_orig, sizeof(double) * p->nv_orig, hipMemcpyDeviceToHost, p->stream);
-		const hipError_t e2 = hipStreamSynchronize(p->stream);
-		(void) hipFree(d_orig);
-		if (e1 != hipSuccess || e2 != hipSuccess) return fail(GP_ERR_HIP, "plan_result: copy failed");
+		k_from_positions<<<go, 256, 0, p->stream>>>(p->d_out, p->d_inv, p->nv_orig, p->d_orig);
+		GP_HIP(hipGetLastError());
+		GP_HIP(hipMemcpyAsync(dist, p->d_orig, sizeof(double) * p->nv_orig, hipMemcpyDeviceToHost, p->stream));
+		GP_HIP(hipStreamSynchronize(p->stream));
 	} else if (p->coal) {
 		std::vector<double> tmp(p->n);
 		GP_HIP(hipMemcpyAsync(tmp.data(), p->d_out, sizeof(double) * p->n, hipMemcpyDeviceToHost, p->stream));
@@ -1068,6 +928,183 @@ int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const doubl
 	p->mesh = mesh;
 	p->opts = opts;
 	*out = p.release();
+	return GP_OK;
+}
+
+// Autotuner of a plan's launch shape: the sweep kernel's workgroups (a
+// quarter to all of those that fit on the device at once) and the sweeps per
+// captured graph replay, timed on the plan itself (one warm-up run, then the
+// mean device time of two) and remembered per plan shape, so that a later plan
+// of the same shape takes the choice without timing.
+namespace {
+struct TuneKey {
+	uint32_t nv_log2, max_slots, f64, band, coalesce, uncapped;
+	bool operator<(const TuneKey &o) const
+	{
+		return std::tie(nv_log2, max_slots, f64, band, coalesce, uncapped) <
+		       std::tie(o.nv_log2, o.max_slots, o.f64, o.band, o.coalesce, o.uncapped);
+	}
+};
+struct WidthKey {
+	uint32_t nv_log2, max_slots, schedule, f64;
+	bool operator<(const WidthKey &o) const
+	{
+		return std::tie(nv_log2, max_slots, schedule, f64) < std::tie(o.nv_log2, o.max_slots, o.schedule, o.f64);
+	}
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, gp_tune> g_tune_cache;
+std::map<WidthKey, double> g_width_cache;
+bool g_tune_loaded = false;
+
+// The choices persist across processes when GPROSHAN_AMD_TUNE_FILE names a
+// file: it is read on first use and every new choice is appended, one line
+// each ("shape <key> <grid> <replay> <ms>", "width <key> <width>").
+const char *tune_file() { return std::getenv("GPROSHAN_AMD_TUNE_FILE"); }
+
+void tune_load_locked()
+{
+	if (g_tune_loaded) return;
+	g_tune_loaded = true;
+	const char *path = tune_file();
+	if (!path || !*path) return;
+	FILE *f = std::fopen(path, "r");
+	if (!f) return;
+	char kind[16];
+	while (std::fscanf(f, "%15s", kind) == 1) {
+		if (!std::strcmp(kind, "shape")) {
+			TuneKey k{};
+			gp_tune t{};
+			if (std::fscanf(f, "%u %u %u %u %u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.f64, &k.band, &k.coalesce,
+			                &k.uncapped, &t.grid, &t.replay, &t.kernel_ms) != 9)
+				break;
+			if (t.grid && t.replay && t.replay % 4 == 0) g_tune_cache[k] = t;
+		} else if (!std::strcmp(kind, "width")) {
+			WidthKey k{};
+			double w = 0;
+			if (std::fscanf(f, "%u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.schedule, &k.f64, &w) != 5) break;
+			if (w > 0) g_width_cache[k] = w;
+		} else {
+			break;
+		}
+	}
+	std::fclose(f);
+}
+
+void tune_append(const char *line)
+{
+	const char *path = tune_file();
+	if (!path || !*path) return;
+	if (FILE *f = std::fopen(path, "a")) {
+		std::fputs(line, f);
+		std::fclose(f);
+	}
+}
+
+uint32_t log2_floor(uint32_t n)
+{
+	uint32_t lg = 0;
+	while ((2ull << lg) <= n) lg++;
+	return lg;
+}
+
+uint32_t mesh_max_slots(const Che &m)
+{
+	uint32_t s = 0;
+	for (uint32_t v = 0; v < m.nv; v++) s = std::max(s, m.link_size(v));
+	return s;
+}
+}  // namespace
+
+bool tune_width_lookup(const Che &mesh, int schedule, int precision, double *width)
+{
+	const WidthKey k{ log2_floor(mesh.nv), mesh_max_slots(mesh), (uint32_t) schedule,
+	                  precision == GP_PREC_F64 ? 1u : 0u };
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	tune_load_locked();
+	auto it = g_width_cache.find(k);
+	if (it == g_width_cache.end()) return false;
+	*width = it->second;
+	return true;
+}
+
+void tune_width_store(const Che &mesh, int schedule, int precision, double width)
+{
+	const WidthKey k{ log2_floor(mesh.nv), mesh_max_slots(mesh), (uint32_t) schedule,
+	                  precision == GP_PREC_F64 ? 1u : 0u };
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	tune_load_locked();
+	g_width_cache[k] = width;
+	char line[160];
+	std::snprintf(line, sizeof(line), "width %u %u %u %u %.17g\n", k.nv_log2, k.max_slots, k.schedule, k.f64, width);
+	tune_append(line);
+}
+
+static void plan_drop_graphs(gp_plan *p)
+{
+	if (p->g_big) (void) hipGraphExecDestroy(p->g_big);
+	if (p->g_small) (void) hipGraphExecDestroy(p->g_small);
+	p->g_big = p->g_small = nullptr;
+}
+
+int plan_autotune(gp_plan *p, gp_tune *out)
+{
+	if (!p) return fail(GP_ERR_INVALID, "autotune: plan is null");
+	if (p->pp) return fail(GP_ERR_INVALID, "autotune: ring and band plans only");
+	const TuneKey key{ log2_floor(p->n), p->max_slots, p->opts.precision == GP_PREC_F64 ? 1u : 0u, p->band ? 1u : 0u,
+	                   (uint32_t) (p->opts.coalesce != 0), (uint32_t) (p->opts.uncapped != 0) };
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		tune_load_locked();
+		auto it = g_tune_cache.find(key);
+		if (it != g_tune_cache.end()) {
+			plan_drop_graphs(p);
+			p->grid = it->second.grid;
+			p->big = it->second.replay;
+			if (out) { *out = it->second; out->candidates = 0; }
+			return GP_OK;
+		}
+	}
+	// The grid sweep_grid picked is the largest candidate (all resident
+	// workgroups, or fewer when the plan has fewer chunks).
+	const uint32_t gmax = p->grid;
+	std::vector<uint32_t> grids;
+	for (uint32_t q = 1; q <= 4; q++) grids.push_back(std::max<uint32_t>(8, (gmax * q / 4 + 7) & ~7u));
+	grids.erase(std::unique(grids.begin(), grids.end()), grids.end());
+	const uint32_t replays[] = { 16, 32, 64, 128 };
+	gp_tune best{};
+	best.kernel_ms = 1e300;
+	uint32_t tried = 0;
+	for (uint32_t g : grids) {
+		for (uint32_t r : replays) {
+			plan_drop_graphs(p);
+			p->grid = g;
+			p->big = r;
+			gp_ptp_stats st{};
+			int rc = plan_run(p, &st);
+			double ms = 0;
+			for (int k = 0; k < 2 && !rc; k++) {
+				rc = plan_run(p, &st);
+				ms += st.kernel_ms / 2;
+			}
+			if (rc) return rc;
+			tried++;
+			if (ms < best.kernel_ms) { best.kernel_ms = ms; best.grid = g; best.replay = r; }
+		}
+	}
+	plan_drop_graphs(p);
+	p->grid = best.grid;
+	p->big = best.replay;
+	best.candidates = tried;
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		g_tune_cache[key] = best;
+		char line[200];
+		std::snprintf(line, sizeof(line), "shape %u %u %u %u %u %u %u %u %.6f\n", key.nv_log2, key.max_slots, key.f64,
+		              key.band, key.coalesce, key.uncapped, best.grid, best.replay, best.kernel_ms);
+		tune_append(line);
+	}
+	if (out) *out = best;
 	return GP_OK;
 }
 
@@ -1131,8 +1168,8 @@ int ptp_iter_error(Che *mesh, const uint32_t *src, uint32_t ns, const double *ex
 	uint32_t ne = 0, nb = 0;
 	for (uint32_t k = 0; !done0 && k <= p->max_iter; k++) {
 		enqueue_sweeps(p.get(), a, k, 1);
+		snapshot(p->d_state, p->h_state, p->stream);
 		GP_HIP(hipGetLastError());
-		GP_HIP(hipMemcpyAsync(p->h_state, p->d_state, sizeof(PState), hipMemcpyDeviceToHost, p->stream));
 		GP_HIP(hipStreamSynchronize(p->stream));
 		if (p->h_state->done) break;
 		// Sweep k + 1 just ran with this state (the uncapped loop has no K/2).

@@ -234,6 +234,27 @@ struct VertexTables {
 	const Edge64 *cx64;
 };
 
+// Vertex-major slot tables of a mesh in its own numbering (slot k of vertex v
+// at fan_off[v] + k; a closed fan stored without its repeated first entry
+// leaves its last fan entry unused), the source that device-built plans and
+// farthest-point sampling gather their ring-ordered tables from.  Built on the
+// device once per mesh and precision.
+struct MasterLayout {
+	uint32_t *cid = nullptr;
+	float4 *cgeo = nullptr;
+	double4 *cgeo64 = nullptr;
+	Edge64 *cx64 = nullptr;
+	uint32_t max_slots = 0;
+	VertexTables view(const DeviceMesh *dm) const
+	{
+		return VertexTables{ dm->fan_off, dm->closed, cid, cgeo, cgeo64, cx64 };
+	}
+	~MasterLayout()
+	{
+		for (void *p : { (void *) cid, (void *) cgeo, (void *) cgeo64, (void *) cx64 }) (void) hipFree(p);
+	}
+};
+
 // The mesh's vertex-major tables for the precision (built once per mesh) and
 // its longest fan's slot count.
 int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots);

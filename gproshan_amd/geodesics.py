@@ -121,14 +121,21 @@ class che:
         check(lib().gp_che_mean_edge(self._h, C.byref(h)))
         return h.value
 
-    def compute_toplesets(self, sources, gpu: bool = False):
-        """Returns (toplesets u32[V], sorted u32[V], limits u32[R+1])."""
+    def compute_toplesets(self, sources, gpu: bool = False, k: int = NIL):
+        """Returns (toplesets u32[V], sorted u32[V], limits u32[R+1]); k is the
+        reference's level cut (host builder only)."""
         src = _u32(sources)
         top = np.zeros(self._nv, np.uint32)
         srt = np.zeros(self._nv, np.uint32)
         cap = self._nv + 2
         lim = np.zeros(cap, np.uint32)
         nl = C.c_uint32()
+        if k != NIL:
+            if gpu:
+                raise GPError(L.ERR_INVALID, "the level cut k is computed by the host builder")
+            check(lib().gp_compute_toplesets_k(self._h, ptr(src), len(src), int(k), ptr(top), ptr(srt), ptr(lim), cap,
+                                               C.byref(nl)))
+            return top, srt, lim[:nl.value].copy()
         fn = lib().gp_compute_toplesets_gpu if gpu else lib().gp_compute_toplesets
         check(fn(self._h, ptr(src), len(src), ptr(top), ptr(srt), ptr(lim), cap, C.byref(nl)))
         return top, srt, lim[:nl.value].copy()
@@ -205,13 +212,14 @@ def normalize_ptp(dist):
 
 def ptp(mesh, sources, *, precision=L.PREC_F32, result_buffer=L.RESULT_LAST_WRITTEN, schedule=L.SCHED_RINGS,
         coalesce=True, tol=PTP_TOL, max_sweeps=0, grid=0, launch_mode=L.LAUNCH_GRAPH, uncapped=False,
-        device_setup=False):
+        device_setup=False, fp32_geometry=L.GEO_AUTO):
     """One PTP solve with explicit options; returns (dist, sorted, stats dict).
     sorted is the reference's BFS order, built on the host while the solve runs."""
     src = _u32(sources)
     o = L.default_opts(precision=precision, result_buffer=result_buffer, schedule=schedule,
                        coalesce=int(coalesce), tol=tol, max_sweeps=max_sweeps, grid=grid,
-                       launch_mode=launch_mode, uncapped=int(uncapped), device_setup=int(device_setup))
+                       launch_mode=launch_mode, uncapped=int(uncapped), device_setup=int(device_setup),
+                       fp32_geometry=int(fp32_geometry))
     dist = np.zeros(mesh.n_vertices())
     srt = np.zeros(mesh.n_vertices(), np.uint32)
     st = L.PtpStats()
@@ -225,7 +233,8 @@ class Plan:
     def __init__(self, mesh, sources, dhat=None, **kw):
         self.mesh = mesh
         self._src = _u32(sources)
-        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup", "fp32_geometry") else v) for k, v in kw.items()})
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup", "fp32_geometry",
+                                                  "band_check") else v) for k, v in kw.items()})
         h = C.c_void_p()
         if dhat is None:
             check(lib().gp_plan_create(mesh.handle, ptr(self._src), len(self._src), C.byref(o), C.byref(h)))
@@ -236,6 +245,32 @@ class Plan:
             check(lib().gp_plan_create_bands(mesh.handle, ptr(self._src), len(self._src), ptr(d), C.byref(o),
                                              C.byref(h)))
         self._h = h
+
+    @classmethod
+    def tuned_bands(cls, mesh, sources, dhat, widths=None, **kw):
+        """Band plan of dhat at the fastest of the candidate widths
+        (gp_plan_create_bands_tuned); returns (plan, width)."""
+        obj = cls.__new__(cls)
+        obj.mesh = mesh
+        obj._src = _u32(sources)
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup", "fp32_geometry",
+                                                  "band_check") else v) for k, v in kw.items()})
+        d = np.ascontiguousarray(dhat, dtype=np.float64)
+        if d.shape != (mesh.n_vertices(),):
+            raise GPError(L.ERR_INVALID, "dhat must hold one value per vertex")
+        w = None if widths is None else np.ascontiguousarray(widths, dtype=np.float64)
+        h, wout = C.c_void_p(), C.c_double()
+        check(lib().gp_plan_create_bands_tuned(mesh.handle, ptr(obj._src), len(obj._src), ptr(d), C.byref(o),
+                                               None if w is None else ptr(w), 0 if w is None else len(w),
+                                               C.byref(h), C.byref(wout)))
+        obj._h = h
+        return obj, wout.value
+
+    def autotune(self):
+        """gp_plan_autotune: the plan's launch shape (grid, replay) timed and kept."""
+        t = L.Tune()
+        check(lib().gp_plan_autotune(self._h, C.byref(t)))
+        return t.as_dict()
 
     def run(self):
         st = L.PtpStats()
@@ -312,12 +347,14 @@ class geodesics:
 
     The option numbering is the reference's CUDA build (GPROSHAN_CUDA defined,
     include/geodesics.h:21-28).  FM, HEAT_FLOW and HEAT_FLOW_GPU are other
-    algorithms, outside this library: requesting them raises.
+    algorithms, outside this library: requesting them raises.  PTP_GPU computes
+    in FP64 by default, as the reference's default build (include/config.h:7-8);
+    precision=PREC_F32 selects the FP32 bulk update with the FP64 fallback.
     """
     FM, PTP_GPU, HEAT_FLOW_GPU, PTP_CPU, HEAT_FLOW = range(5)
 
     def __init__(self, mesh, sources, opt=FM, e_dist=None, cluster=False, n_iter=0, radio=math.inf, *,
-                 precision=L.PREC_F32, schedule=L.SCHED_RINGS):
+                 precision=L.PREC_F64, schedule=L.SCHED_RINGS):
         n = mesh.n_vertices()
         if n <= 0:
             raise GPError(L.ERR_INVALID, "mesh has no vertices")

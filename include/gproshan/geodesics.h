@@ -93,12 +93,11 @@ class che {
 	void compute_toplesets(index_t *&toplesets, index_t *&sorted, std::vector<index_t> &limits,
 	                       const std::vector<index_t> &sources, const index_t &k = NIL) const
 	{
-		if (k != NIL) throw ptp_error(GP_ERR_INVALID, "compute_toplesets: the level cut k is not supported");
 		if (sources.empty()) return;
 		std::vector<uint32_t> lim(nv_ + 2);
 		uint32_t nl = 0;
-		check(gp_compute_toplesets(h_, sources.data(), (uint32_t) sources.size(), toplesets, sorted, lim.data(),
-		                           (uint32_t) lim.size(), &nl));
+		check(gp_compute_toplesets_k(h_, sources.data(), (uint32_t) sources.size(), k, toplesets, sorted, lim.data(),
+		                             (uint32_t) lim.size(), &nl));
 		limits.insert(limits.end(), lim.begin(), lim.begin() + nl);
 	}
 
@@ -207,9 +206,13 @@ class geodesics {
 
 	index_t *clusters = nullptr;
 
+	// precision (no reference counterpart): the GPU arithmetic of PTP_GPU.
+	// The default, GP_PREC_F64, is the reference's default build
+	// (include/config.h:7-8) and returns its bits; GP_PREC_F32 is the FP32
+	// bulk update with the FP64 fallback (within 1e-4 of it).
 	geodesics(che *mesh, const std::vector<index_t> &sources, const option_t &opt = FM,
 	          distance_t *const &e_dist = nullptr, const bool &cluster = false, const size_t &n_iter = 0,
-	          const distance_t &radio = INFINITY)
+	          const distance_t &radio = INFINITY, const int &precision = GP_PREC_F64)
 		: n_vertices_(mesh->n_vertices())
 	{
 		(void) n_iter; (void) radio;                      // ignored by PTP in the reference too
@@ -247,6 +250,7 @@ class geodesics {
 				// for fans past the register-held slots).
 				gp_ptp_opts o;
 				gp_ptp_opts_default(&o);
+				o.precision = precision;
 				o.coalesce = 1;
 				o.device_setup = 1;
 				check(gp_ptp(mesh->handle(), sources.data(), (uint32_t) sources.size(), &o, dist_, sorted_index_,

@@ -34,14 +34,21 @@ enum gp_status {
 	GP_ERR_HIP = -4,              /* a HIP runtime call failed                             */
 	GP_ERR_NO_DEVICE = -5,        /* no HIP device, or the device is not gfx950            */
 	GP_ERR_CAPACITY = -6,         /* output buffer too small / size over the supported max */
-	GP_ERR_KERNEL = -7            /* a kernel reported an internal error (bounded wait hit) */
+	GP_ERR_KERNEL = -7,           /* a kernel reported an internal error (bounded wait hit) */
+	GP_ERR_NOT_CONVERGED = -8     /* an uncapped or band loop stopped at its sweep guard
+	                                 before every vertex met the tolerance              */
 };
 
 /* Arithmetic precision of the GPU relaxation. */
 enum gp_precision {
 	GP_PREC_F32 = 0,  /* FP32 bulk update; FP64 update on obtuse/near-degenerate triangles;
-	                     FP64 convergence test.  Default.                                   */
-	GP_PREC_F64 = 1   /* FP64 throughout, the reference's default build.                    */
+	                     FP64 convergence test.  The default of gp_ptp_opts_default.  With
+	                     fp32_geometry == GP_GEO_AUTO, a mesh with more than a quarter of
+	                     its fan triangles flagged runs FP64 sweeps instead (reported in
+	                     gp_ptp_stats.precision).                                          */
+	GP_PREC_F64 = 1   /* FP64 throughout, the reference's default build (include/config.h:7-8).
+	                     The reference-named drivers (gp_parallel_toplesets_propagation_*_gpu)
+	                     and the C++ front end default to it.                              */
 };
 
 /* Which double buffer the call returns.  The reference's CPU drivers return
@@ -87,7 +94,8 @@ enum gp_launch_mode {
  * others after the fan loop); GP_GEO_FP64 reads the 32-byte FP64 geometry of
  * every slot and runs each triangle once, which is faster when many corners
  * are flagged.  GP_GEO_AUTO picks GP_GEO_FP64 for host-built plans with more
- * than a quarter of their triangles flagged. */
+ * than a quarter of their triangles flagged; since such meshes run FP64 sweeps
+ * under GP_GEO_AUTO (gp_precision), GP_GEO_FP64 is only used when asked for. */
 enum gp_fp32_geometry {
 	GP_GEO_AUTO = 0,
 	GP_GEO_FP32 = 1,
@@ -127,6 +135,11 @@ typedef struct gp_ptp_opts {
 	                       sweep kernels' register-held slots or a band index
 	                       would pass the vertex count                        */
 	int fp32_geometry;  /* gp_fp32_geometry (precision F32, ring schedules)     */
+	int band_check;     /* band schedules: 1 (default) = end every run with check
+	                       sweeps over all reached vertices until none moves by
+	                       tol (a vertex frozen in a band before its upwind
+	                       neighbours settled is relaxed again); 0 = the bare
+	                       band loop, as the uncapped loop over the same bands */
 } gp_ptp_opts;
 
 typedef struct gp_ptp_stats {
@@ -140,10 +153,21 @@ typedef struct gp_ptp_stats {
 	uint64_t deferred;        /* band schedules: patch entries carried over           */
 	double kernel_ms;         /* device time of the sweep loop (hipEvent)             */
 	double total_ms;          /* host wall time of the whole call                     */
+	int32_t precision;        /* gp_precision the sweeps ran in                       */
+	int32_t capped;           /* 1: the loop stopped at its sweep cap (the reference
+	                             schedule's 2*|limits|, or the guard of the uncapped and
+	                             band loops, which then return GP_ERR_NOT_CONVERGED)  */
+	uint32_t kernel;          /* sweep kernel variant: bit 0 FP64, bit 1 cluster labels,
+	                             bit 2 eight register-held slots, bit 3 FP64 geometry
+	                             table (FP32 sweeps)                                  */
+	uint32_t check_sweeps;    /* band schedules: full sweeps of the final convergence
+	                             check over every reached vertex (1 when the bands
+	                             already met tol)                                     */
 } gp_ptp_stats;
 
 /* Fill opts with the defaults (FP32, last written, rings, coalesce, PTP_TOL;
- * band_width 0 (schedule default), patches: block 256, local_iters 8). */
+ * band_width 0 (schedule default), patches: block 256, local_iters 8,
+ * band_check 1). */
 void gp_ptp_opts_default(gp_ptp_opts *opts);
 
 /* Text of the last error on this thread ("" if none). */
@@ -199,6 +223,15 @@ int gp_che_mean_edge(const gp_che *mesh, double *h);
 int gp_compute_toplesets(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
                          uint32_t *toplesets, uint32_t *sorted, uint32_t *limits,
                          uint32_t cap_limits, uint32_t *n_limits);
+
+/* che::compute_toplesets with its level cut k (include/che.h:107 default
+ * argument; src/che.cpp:572-575,592): the walk stops at the first vertex of
+ * level k + 1, so sorted ends with the level-(k + 1) vertices discovered from
+ * level k and the last ring [limits[n-2], limits[n-1]) holds levels k and
+ * k + 1; toplesets is NIL beyond level k + 1.  k = GP_NIL: no cut. */
+int gp_compute_toplesets_k(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources, uint32_t k,
+                           uint32_t *toplesets, uint32_t *sorted, uint32_t *limits,
+                           uint32_t cap_limits, uint32_t *n_limits);
 
 /* The same result computed on the GPU (frontier expansion + discovery-key
  * ordering); bit-identical to gp_compute_toplesets. */
@@ -272,6 +305,43 @@ int gp_plan_trace(gp_plan *plan, uint32_t *trace_out, uint32_t cap_sweeps, uint3
  * from the same buffer as gp_plan_result. */
 int gp_plan_clusters(gp_plan *plan, uint32_t *clusters);
 void gp_plan_destroy(gp_plan *plan);
+
+/* ---- autotuner (no reference counterpart) ------------------------------ */
+
+/* Launch shape of a ring or band plan's sweeps. */
+typedef struct gp_tune {
+	uint32_t grid;        /* workgroups of the sweep kernel (one wave each)            */
+	uint32_t replay;      /* sweeps per replay of the long captured graph              */
+	double kernel_ms;     /* device time per field of the chosen shape                 */
+	uint32_t candidates;  /* shapes timed (0: the choice came from the cache)          */
+	uint32_t pad;
+} gp_tune;
+
+/* Times the plan's sweep loop over candidate grids (a quarter, half, three
+ * quarters and all of the workgroups that fit on the device at once) and
+ * replay lengths (16, 32, 64, 128 sweeps): one warm-up run and the mean
+ * device time of two runs each.  The fastest shape is kept in the plan and
+ * remembered for plans of the same shape (vertex count rounded down to a power
+ * of two, longest fan, precision, band or ring schedule, coalescing, uncapped
+ * loop), which then take it without timing.  The plan's last result is that of
+ * the last timed run.  Patch plans: GP_ERR_INVALID. */
+int gp_plan_autotune(gp_plan *plan, gp_tune *out);
+
+/* Band-width autotuner of the band schedules: builds the band plan of dhat
+ * at each candidate width (mean edge lengths; widths == NULL: 0.45, 0.5, 0.6,
+ * 0.75, 1.0), autotunes its launch shape (gp_plan_autotune), and returns the
+ * plan with the smallest device time per field; *width_out (may be NULL)
+ * receives its width.  Every run ends with the check sweeps (band_check), so
+ * every candidate meets the same convergence test.  With the default
+ * candidates the choice is remembered per mesh shape (vertex count rounded
+ * down to a power of two, longest link), schedule and precision, and later
+ * calls build that width's plan without timing.  Both autotuners keep their
+ * choices for the process; when GPROSHAN_AMD_TUNE_FILE names a file, they are
+ * also read from it and appended to it, so they carry over to later
+ * processes. */
+int gp_plan_create_bands_tuned(gp_che *mesh, const uint32_t *sources, uint32_t n_sources, const double *dhat,
+                               const gp_ptp_opts *opts, const double *widths, uint32_t n_widths,
+                               gp_plan **out, double *width_out);
 
 /* ---- learned band predictor (no reference counterpart) ------------------ */
 

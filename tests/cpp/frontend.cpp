@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using namespace gproshan;
@@ -47,12 +48,27 @@ int main(int argc, char **argv)
 	}
 
 	if (gpu) {
-		std::vector<distance_t> ext(mesh.n_vertices());
+		// PTP_GPU in its default FP64 and with GP_PREC_F32, written to
+		// <prefix>.f64 / <prefix>.f32 (raw doubles) for the comparison with the
+		// oracle in tests/test_frontend_cpp.py.
+		std::vector<distance_t> ext(mesh.n_vertices()), ext32(mesh.n_vertices());
 		geodesics g(&mesh, src, geodesics::PTP_GPU, ext.data());
+		geodesics g32(&mesh, src, geodesics::PTP_GPU, ext32.data(), false, 0, INFINITY, GP_PREC_F32);
+		if (argc > 2) {
+			for (int q = 0; q < 2; q++) {
+				const std::string path = std::string(argv[2]) + (q ? ".f32" : ".f64");
+				FILE *f = std::fopen(path.c_str(), "wb");
+				ok &= f != nullptr;
+				if (f) {
+					ok &= std::fwrite((q ? ext32 : ext).data(), sizeof(distance_t), ext.size(), f) == ext.size();
+					std::fclose(f);
+				}
+			}
+		}
 		double d = 0;
-		for (index_t v = 0; v < mesh.n_vertices(); v++) d = std::fmax(d, std::fabs(ext[v] - cpu[v]) / std::fmax(cpu[v], 1e-30));
-		std::printf("ptp_gpu vs ptp_cpu max rel diff %.3e\n", d);
-		ok &= d < 2e-3;   // last-written (GPU) vs last-read (CPU) buffer, FP32 vs FP64
+		for (index_t v = 0; v < mesh.n_vertices(); v++) d = std::fmax(d, std::fabs(ext32[v] - ext[v]) / std::fmax(ext[v], 1e-30));
+		std::printf("ptp_gpu f32 vs f64 max rel diff %.3e\n", d);
+		ok &= d <= 1e-4;
 
 		// Cluster labels of two sources at opposite corners: each corner's
 		// own neighbourhood carries its label (1-based source index).

@@ -77,7 +77,7 @@ def test_device_setup_f32_close(case):
     mx, _ = rel_err(dev, host)
     same = len(tr_h) == len(tr_d) and np.array_equal(tr_h[:, 1:7], tr_d[:, 1:7])
     print(f"[device f32] {name} max {mx:.2e} same schedule {same}", flush=True)
-    assert mx <= (1e-6 if same else 1e-3)
+    assert mx <= (1e-6 if same else 1e-4)
     assert np.array_equal(np.isinf(host), np.isinf(dev))
 
 

@@ -62,6 +62,34 @@ def test_host_toplesets_match_oracle(pair):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("k", [0, 1, 2, 5])
+def test_host_toplesets_level_cut_match_oracle(pair, k):
+    M, C, src = pair
+    for a, b in zip(C.compute_toplesets(src, k=k), M.toplesets(src, k=k)):
+        assert np.array_equal(a, b)
+
+
+def test_toplesets_level_cut_by_hand():
+    # A strip of four unit squares, vertices 0-4 at y = 0 and 5-9 at y = 1,
+    # each square split along its (i, i + 6) diagonal.  From source 0:
+    # level 1 = {1, 6, 5}, level 2 = {2, 7}, level 3 = {3, 8}, ...  With the
+    # cut k = 1 the walk stops at the first level-2 vertex, after the level-1
+    # vertices have discovered {2, 7}: those stay in sorted, at level 2, and in
+    # the last ring (src/che.cpp:572-575, 592).
+    xyz = np.array([[x, y, 0.0] for y in (0, 1) for x in range(5)])
+    f = []
+    for i in range(4):
+        f += [[i, i + 1, i + 6], [i, i + 6, i + 5]]
+    C = P.che(xyz, np.array(f, np.uint32))
+    lev, srt, lim = C.compute_toplesets([0], k=1)
+    assert lim.tolist() == [0, 1, 6]
+    assert sorted(srt[1:4].tolist()) == [1, 5, 6] and sorted(srt[4:6].tolist()) == [2, 7]
+    assert (srt[6:] == L.NIL).all()
+    assert lev[[0, 1, 5, 6, 2, 7]].tolist() == [0, 1, 1, 1, 2, 2] and (lev[[3, 4, 8, 9]] == L.NIL).all()
+    lev0, srt0, lim0 = C.compute_toplesets([0])
+    assert lim0.tolist() == [0, 1, 4, 6, 8, 10]
+
+
 def test_host_ptp_cpu_bit_exact(pair, oracle):
     M, C, src = pair
     ts = C.compute_toplesets(src)

@@ -58,3 +58,63 @@ def test_features_on_a_plane(oracle):
     interior = [v for v in range(81) if 0 < v % 9 < 8 and 0 < v // 9 < 8]
     assert np.allclose(F[interior, 2], 0, atol=1e-12)   # flat: no curvature
     assert np.allclose(F[:, 8], lev / lev.max())
+
+
+# Hand-derived features (source 0, mean edge h = 0.5).  Tetrahedron
+# (tests/test_oracle_che.py): every vertex has three link neighbours; the
+# area-weighted fan normals are -(1,1,1)/sqrt(3) at the origin corner and the
+# axis directions at vertices 1, 2, 3; the links are [3, 2, 1] at 0 and
+# [2, 3, 0] at 1.  Open 2 x 2 square: links [1, 3, 2] at 0, [3, 0] at 1,
+# [0, 3] at 2 and [2, 0, 1] at 3, normals +z.
+S3, S2 = np.sqrt(3.0), np.sqrt(2.0)
+
+
+def _tetra_expected(h):
+    lbar1 = (1 + 2 * S2) / 3                      # |1-2| = |1-3| = sqrt 2, |1-0| = 1
+    f0 = [0.0, -2 / S3, 0.5, 0.0, 0.0, 1 / h, 1.0, 0.0, 1.0]
+    row0 = [0.0] + f0                             # e = 0: u = 0
+    row1 = [np.log1p(1 / h), 1.0, -2 / lbar1, 0.5, h / (1 + h), -1 / (S3 * (1 + h)), lbar1 / h, -1 / S3, 1.0,
+            1 / S2]
+    return np.array([row0, row1, row1, row1])
+
+
+def _square_expected(h):
+    lb03 = (2 + S2) / 3
+    return np.array([
+        [0.0, 0.0, 0.0, 0.5, 0.0, 0.0, lb03 / h, 1.0, 0.0, 1 / S2],
+        [np.log1p(1 / h), 0.0, 0.0, 2 / 6, h / (1 + h), 0.0, 1 / h, 1.0, 1.0, 1.0],
+        [np.log1p(1 / h), 0.0, 0.0, 2 / 6, h / (1 + h), 0.0, 1 / h, 1.0, 1.0, 1.0],
+        [np.log1p(S2 / h), 0.0, 0.0, 0.5, h / (S2 + h), 0.0, lb03 / h, 1.0, 1.0, 1 / S2],
+    ])
+
+
+@pytest.mark.parametrize("which", ["tetra", "square"])
+def test_features_hand_values(oracle, which):
+    from test_oracle_che import GRID2_F, GRID2_XYZ, TETRA_F, TETRA_XYZ
+    xyz, f = (TETRA_XYZ, TETRA_F) if which == "tetra" else (GRID2_XYZ, GRID2_F)
+    xyz = np.array(xyz, np.float64)
+    m = oracle.Mesh(xyz, np.array(f, np.uint32))
+    lev, _, _ = m.toplesets([0])
+    h = 0.5
+    F, e = Mo.features(m, xyz, [0], lev, h)
+    want = _tetra_expected(h) if which == "tetra" else _square_expected(h)
+    assert np.allclose(F, want, rtol=0, atol=1e-12), (F, want)
+
+
+def test_curvature_on_a_sphere(oracle):
+    # On a sphere of radius r a link vertex q at chord length l from p lies
+    # l^2 / (2 r) below the tangent plane ((q - p) . p / r = -l^2 / (2 r)
+    # since |q| = |p| = r), so at a vertex whose fan normal is p / r -- an
+    # original icosahedron vertex, by its five-fold symmetry -- with five
+    # equidistant neighbours, f2 = 2 (-l^2 / (2 r)) / l = -l / r.
+    r = 2.0
+    xyz, f = G.icosphere(3, 0.0, 0, radius=r)
+    m = oracle.Mesh(xyz, f)
+    lev, _, _ = m.toplesets([0])
+    F, _ = Mo.features(m, xyz, [0], lev, 0.1)
+    for v in range(12):
+        L = [int(m.vt[h]) for h in m.link(v)]
+        d = np.linalg.norm(xyz[L] - xyz[v], axis=1)
+        assert len(L) == 5 and np.allclose(d, d[0], rtol=1e-12)
+        assert abs(F[v, 2] - (-d[0] / r)) <= 1e-12
+        assert abs(F[v, 9] - 1.0) <= 1e-12 and abs(F[v, 3] - 5 / 6) <= 1e-15
