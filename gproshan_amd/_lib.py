@@ -30,6 +30,7 @@ EXPORTS = [
     "gp_che_create", "gp_che_destroy", "gp_che_read", "gp_che_positions", "gp_che_sizes", "gp_che_tables", "gp_che_link", "gp_che_mean_edge",
     "gp_compute_toplesets", "gp_compute_toplesets_k", "gp_compute_toplesets_gpu",
     "gp_parallel_toplesets_propagation_gpu", "gp_parallel_toplesets_propagation_coalescence_gpu",
+    "gp_parallel_toplesets_propagation_gpu_noalloc",
     "gp_parallel_toplesets_propagation_cpu", "gp_parallel_toplesets_propagation_coalescence_cpu",
     "gp_normalize_ptp", "gp_ptp",
     "gp_plan_create", "gp_plan_create_bands", "gp_plan_run", "gp_plan_result", "gp_plan_trace", "gp_plan_destroy",
@@ -123,6 +124,7 @@ def lib() -> C.CDLL:
     drv = [P, P, u32, P, u32, P, P, P, C.POINTER(C.c_double)]
     L.gp_parallel_toplesets_propagation_gpu.argtypes = drv
     L.gp_parallel_toplesets_propagation_coalescence_gpu.argtypes = drv
+    L.gp_parallel_toplesets_propagation_gpu_noalloc.argtypes = [P, P, u32, P, u32, P, P, P]
     L.gp_parallel_toplesets_propagation_cpu.argtypes = [P, P, u32, P, u32, P, P]
     L.gp_parallel_toplesets_propagation_coalescence_cpu.argtypes = [P, P, u32, P, u32, P, P]
     L.gp_normalize_ptp.argtypes = [P, u32]

@@ -23,7 +23,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgproshan_ptp.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-HIP_SOURCES = ["ptp_gpu.hip", "patches.hip", "rings.hip", "mlp.hip", "batch.hip", "fps.hip", "levels.hip"]
+HIP_SOURCES = ["ptp_gpu.hip", "patches.hip", "rings.hip", "mlp.hip", "batch.hip", "fps.hip", "levels.hip",
+               "noalloc.hip"]
 CPP_SOURCES = ["che.cpp", "ptp_cpu.cpp", "capi.cpp", "mesh_io.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",

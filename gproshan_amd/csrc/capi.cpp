@@ -232,6 +232,16 @@ int gp_parallel_toplesets_propagation_coalescence_gpu(gp_che *mesh, const uint32
 	GP_GUARD(return ptp_gpu_driver(mesh, sources, n_sources, limits, n_limits, sorted, dist, clusters, seconds, 1);)
 }
 
+int gp_parallel_toplesets_propagation_gpu_noalloc(gp_che *mesh, const uint32_t *d_sources, uint32_t n_sources,
+                                                  const uint32_t *d_limits, uint32_t n_limits,
+                                                  const uint32_t *d_sorted, double *d_dist, void *stream)
+{
+	GP_GUARD(
+		if (!mesh) return gp::fail(GP_ERR_INVALID, "null argument");
+		return gp::ptp_gpu_noalloc(mesh->m, d_sources, n_sources, d_limits, n_limits, d_sorted, d_dist, stream);
+	)
+}
+
 int gp_parallel_toplesets_propagation_cpu(const gp_che *mesh, const uint32_t *sources, uint32_t n_sources,
                                           const uint32_t *limits, uint32_t n_limits, const uint32_t *sorted,
                                           double *dist)

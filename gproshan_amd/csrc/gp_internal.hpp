@@ -171,6 +171,9 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 // BFS levels of the sources computed on the device (NIL where unreached).
 int levels_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
+// noalloc.hip: the allocation-free FP64 driver (stream: a hipStream_t).
+int ptp_gpu_noalloc(Che &mesh, const uint32_t *d_src, uint32_t ns, const uint32_t *d_limits, uint32_t nl,
+                    const uint32_t *d_sorted, double *d_dist, void *stream);
 int plan_autotune(gp_plan *p, gp_tune *out);
 // The band-width autotuner's remembered choice per mesh shape (vertex count
 // rounded down to a power of two, longest link), schedule and precision.

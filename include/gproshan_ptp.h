@@ -261,6 +261,28 @@ int gp_parallel_toplesets_propagation_coalescence_gpu(gp_che *mesh, const uint32
                                                       double *dist, uint32_t *clusters,
                                                       double *seconds);
 
+/* No counterpart in the reference: the allocation-free form of
+ * gp_parallel_toplesets_propagation_gpu, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  Same result (the FP64 window schedule over sorted[], K/2,
+ * PTP_TOL, 2*|limits| cap, last-written buffer; bit-identical), from device
+ * inputs: d_sources[n_sources], d_limits[n_limits] and d_sorted[n_vertices] as
+ * gp_compute_toplesets writes them; the field goes to d_dist[n_vertices]
+ * (device, INF where unreached).  Every kernel is enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream); the call allocates, frees and
+ * synchronises nothing, so the caller waits on the stream.  The mesh's FP64
+ * device tables must exist: the first FP64 solve with device_setup on the
+ * mesh (gp_ptp or gp_plan_create with precision GP_PREC_F64, device_setup 1)
+ * builds them; else GP_ERR_INVALID.  At most GP_NOALLOC_MAX_VERTICES vertices
+ * (GP_ERR_CAPACITY beyond).  Its scratch memory (schedule state and two FP64
+ * distance buffers, 16 MiB + 256 B) is one device array per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race. */
+#define GP_NOALLOC_MAX_VERTICES 1048576u
+int gp_parallel_toplesets_propagation_gpu_noalloc(gp_che *mesh, const uint32_t *d_sources, uint32_t n_sources,
+                                                  const uint32_t *d_limits, uint32_t n_limits,
+                                                  const uint32_t *d_sorted, double *d_dist, void *stream);
+
 /* parallel_toplesets_propagation_cpu (include/geodesics_ptp.h:40;
  * src/geodesics_ptp.cpp:122-199): FP64 on the host (OpenMP), last-read buffer. */
 int gp_parallel_toplesets_propagation_cpu(const gp_che *mesh, const uint32_t *sources,
