@@ -52,6 +52,30 @@ __device__ __forceinline__ double tri_f64(const Edge64 &xa, const Edge64 &xb, do
 // bits with one 32-byte read per slot and no second pass.
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
 
+// FP32 relaxation of a lane whose fan is closed and stored without its
+// repeat, with SC real slots and no flagged triangle: triangle k joins slots
+// k and (k + 1) mod SC.  These are the values and the order of the general
+// register path of relax_lane for such a lane, with each slot's edge length
+// computed once and no per-slot selects.
+template <uint32_t SC, uint32_t MS, typename A>
+__device__ __forceinline__ float relax_closed(const A &a, const uint32_t (&id)[MS], const float4 (&g)[MS],
+                                              const float *od, float best)
+{
+	float t[SC], s[SC];
+#pragma unroll
+	for (uint32_t k = 0; k < SC; k++) t[k] = od[id[k] & SLOT_ID_MASK];
+#pragma unroll
+	for (uint32_t k = 0; k < SC; k++) s[k] = stable_sqrt(g[k].x);
+#pragma unroll
+	for (uint32_t k = 0; k < SC; k++) {
+		const uint32_t kn = k + 1 < SC ? k + 1 : 0;
+		const float p = update_stable_s<float>(s[k], s[kn], g[k].x, g[kn].x, g[k].y, g[k].z, g[k].w, t[k], t[kn], a.h,
+		                                       a.inv_h);
+		if (p < best) best = p;
+	}
+	return best;
+}
+
 template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, uint32_t MS = MAXS, bool G64 = false,
           typename A>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
@@ -122,6 +146,26 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
 		}
 		const bool wrap = (id[0] & SLOT_WRAP) != 0u;
+		// FP32 without labels, chunked tables: when every active lane holds a
+		// closed fan without empty slots or flagged triangles (the interior of
+		// a mesh), the closed-fan path for the chunk's slot count.
+		if constexpr (!F64 && !CL && ST == 64) {
+			bool plain = wrap;
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++)
+				if (k < S) plain = plain && !(id[k] & (SLOT_EMPTY | F64_FLAG));
+			if (__ballot(!plain) == 0) {
+				switch (S) {
+				case 3: return relax_closed<3>(a, id, g, od, best);
+				case 4: return relax_closed<4>(a, id, g, od, best);
+				case 5: return relax_closed<(MS >= 5 ? 5 : MS)>(a, id, g, od, best);
+				case 6: return relax_closed<(MS >= 6 ? 6 : MS)>(a, id, g, od, best);
+				case 7: return relax_closed<(MS >= 7 ? 7 : MS)>(a, id, g, od, best);
+				case 8: return relax_closed<(MS >= 8 ? 8 : MS)>(a, id, g, od, best);
+				default: break;
+				}
+			}
+		}
 		// Slot k + 1 is real (else the triangle of slot k closes onto slot 0).
 		auto next_real = [&](uint32_t k) {
 			return k + 1 < MS && k + 1 < S && !(id[(k + 1) % MS] & SLOT_EMPTY);

@@ -116,12 +116,15 @@ GP_HD double stable_rcp(double x) { return 1.0 / x; }
 // infinite, the discriminant is negative or the front does not arrive from
 // inside the wedge (the planar terms computed from an infinite input are
 // discarded by the same test).
+//
+// s0 = sqrt(q0) and s1 = sqrt(q1) (edge lengths in units of h) are taken as
+// arguments, so that a caller holding a whole fan computes each slot's once.
 template <typename T>
-GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
+GP_HD T update_stable_s(T s0, T s1, T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 {
 	const T inf = (T) INFINITY;
-	const T dp0 = t0 + stable_sqrt(q0) * h;
-	const T dp1 = t1 + stable_sqrt(q1) * h;
+	const T dp0 = t0 + s0 * h;
+	const T dp1 = t1 + s1 * h;
 	const T edge = (dp1 < dp0) ? dp1 : dp0;
 	const T m = t0 < t1 ? t0 : t1;
 	const T u0 = (t0 - m) * inv_h;
@@ -134,6 +137,12 @@ GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 	const T c1 = q0 * (u1 - pp) - g * (u0 - pp);
 	const bool planar = (t0 != inf && t1 != inf) && !(disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0);
 	return planar ? m + pp * h : edge;
+}
+
+template <typename T>
+GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
+{
+	return update_stable_s<T>(stable_sqrt(q0), stable_sqrt(q1), q0, q1, g, det, S, t0, t1, h, inv_h);
 }
 
 }  // namespace gp
