@@ -79,10 +79,10 @@ def sweep_kernel_name(bits: int) -> str:
 
 class Tune(C.Structure):
     _fields_ = [("grid", C.c_uint32), ("replay", C.c_uint32), ("kernel_ms", C.c_double),
-                ("candidates", C.c_uint32), ("pad", C.c_uint32)]
+                ("candidates", C.c_uint32), ("xcd_block", C.c_uint32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class BatchStats(C.Structure):

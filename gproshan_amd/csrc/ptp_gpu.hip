@@ -227,6 +227,7 @@ struct gp_plan {
 	hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = { nullptr, nullptr };
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
 	uint32_t big = 64, small = 8;            // sweeps per captured graph (multiples of 4)
+	uint32_t xblk = 0;                       // log2 of the chunk blocks dealt to XCDs (0: eighths)
 	bool has_run = false;
 	bool capped = false;                     // the last run stopped at its sweep cap
 	uint32_t check_sweeps = 0;               // check sweeps of the last run (band plans)
@@ -660,6 +661,7 @@ static KArgs make_args(const gp_plan *p)
 	a.tol = p->opts.tol; a.h = p->dm->h; a.inv_h = p->dm->inv_h;
 	a.nl = p->nl; a.max_iter = p->max_iter; a.k2 = p->opts.uncapped ? 0u : 1u;
 	a.npos = p->n;
+	a.xblk = p->xblk;
 	a.cl0 = p->d_cl0; a.cl1 = p->d_cl1;
 	return a;
 }
@@ -975,10 +977,10 @@ void tune_load_locked()
 		if (!std::strcmp(kind, "shape")) {
 			TuneKey k{};
 			gp_tune t{};
-			if (std::fscanf(f, "%u %u %u %u %u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.f64, &k.band, &k.coalesce,
-			                &k.uncapped, &t.grid, &t.replay, &t.kernel_ms) != 9)
+			if (std::fscanf(f, "%u %u %u %u %u %u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.f64, &k.band,
+			                &k.coalesce, &k.uncapped, &t.grid, &t.replay, &t.xcd_block, &t.kernel_ms) != 10)
 				break;
-			if (t.grid && t.replay && t.replay % 4 == 0) g_tune_cache[k] = t;
+			if (t.grid && t.replay && t.replay % 4 == 0 && t.xcd_block < 16) g_tune_cache[k] = t;
 		} else if (!std::strcmp(kind, "width")) {
 			WidthKey k{};
 			double w = 0;
@@ -1061,6 +1063,7 @@ int plan_autotune(gp_plan *p, gp_tune *out)
 			plan_drop_graphs(p);
 			p->grid = it->second.grid;
 			p->big = it->second.replay;
+			p->xblk = it->second.xcd_block;
 			if (out) { *out = it->second; out->candidates = 0; }
 			return GP_OK;
 		}
@@ -1075,33 +1078,43 @@ int plan_autotune(gp_plan *p, gp_tune *out)
 	gp_tune best{};
 	best.kernel_ms = 1e300;
 	uint32_t tried = 0;
-	for (uint32_t g : grids) {
-		for (uint32_t r : replays) {
-			plan_drop_graphs(p);
-			p->grid = g;
-			p->big = r;
-			gp_ptp_stats st{};
-			int rc = plan_run(p, &st);
-			double ms = 0;
-			for (int k = 0; k < 2 && !rc; k++) {
-				rc = plan_run(p, &st);
-				ms += st.kernel_ms / 2;
-			}
-			if (rc) return rc;
-			tried++;
-			if (ms < best.kernel_ms) { best.kernel_ms = ms; best.grid = g; best.replay = r; }
+	auto time_shape = [&](uint32_t g, uint32_t r, uint32_t xb) -> int {
+		plan_drop_graphs(p);
+		p->grid = g;
+		p->big = r;
+		p->xblk = xb;
+		gp_ptp_stats st{};
+		int rc = plan_run(p, &st);
+		double ms = 0;
+		for (int k = 0; k < 2 && !rc; k++) {
+			rc = plan_run(p, &st);
+			ms += st.kernel_ms / 2;
 		}
-	}
+		if (rc) return rc;
+		tried++;
+		if (ms < best.kernel_ms) { best.kernel_ms = ms; best.grid = g; best.replay = r; best.xcd_block = xb; }
+		return GP_OK;
+	};
+	for (uint32_t g : grids)
+		for (uint32_t r : replays)
+			if (int rc = time_shape(g, r, 0)) return rc;
+	// Then the XCD deal of the chosen shape: contiguous eighths of the window,
+	// or blocks of 16, 64 or 256 chunks dealt round-robin to the XCDs.
+	const uint32_t g_best = best.grid, r_best = best.replay;
+	for (uint32_t xb : { 4u, 6u, 8u })
+		if (int rc = time_shape(g_best, r_best, xb)) return rc;
 	plan_drop_graphs(p);
 	p->grid = best.grid;
 	p->big = best.replay;
+	p->xblk = best.xcd_block;
 	best.candidates = tried;
 	{
 		std::lock_guard<std::mutex> lock(g_tune_mu);
 		g_tune_cache[key] = best;
 		char line[200];
-		std::snprintf(line, sizeof(line), "shape %u %u %u %u %u %u %u %u %.6f\n", key.nv_log2, key.max_slots, key.f64,
-		              key.band, key.coalesce, key.uncapped, best.grid, best.replay, best.kernel_ms);
+		std::snprintf(line, sizeof(line), "shape %u %u %u %u %u %u %u %u %u %.6f\n", key.nv_log2, key.max_slots,
+		              key.f64, key.band, key.coalesce, key.uncapped, best.grid, best.replay, best.xcd_block,
+		              best.kernel_ms);
 		tune_append(line);
 	}
 	if (out) *out = best;

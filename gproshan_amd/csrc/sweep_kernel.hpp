@@ -58,6 +58,7 @@ struct KArgs {
 	float h, inv_h;
 	uint32_t nl, max_iter;
 	uint32_t npos;             // positions of the plan (check sweeps cover [0, npos))
+	uint32_t xblk;             // log2 of the chunk blocks dealt to XCDs (0: contiguous eighths)
 	uint32_t k2;               // 1: K/2 limit (the drivers); 0: the uncapped iter_error loop
 	uint32_t *cl0, *cl1;       // cluster labels (double-buffered like the distances), or null
 	const uint8_t *closed;     // vertex-major sweeps: closed-fan flags of the mesh
@@ -80,21 +81,19 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 	if (start >= end) return 0;
 	const uint32_t c0 = start >> 6, c1 = (end + 63u) >> 6;
 	const uint32_t lane = threadIdx.x & 63u;
-	// XCD-aware split: workgroups b with b % 8 == x run on XCD x and take the
-	// x-th contiguous eighth of the window's chunks.
-	const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
+	// XCD-aware split: workgroups b with b % 8 == x run on XCD x.  xblk == 0:
+	// XCD x takes the x-th contiguous eighth of the window's chunks.  xblk > 0:
+	// chunks are dealt to XCDs in blocks of 2^xblk, block q to XCD q % 8, so a
+	// chunk is relaxed on the same XCD in every sweep (its tables and its
+	// neighbours' values stay in that XCD's L2).
 	const uint32_t x = blockIdx.x & 7u;
 	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
 	const uint32_t wpb = blockDim.x >> 6;
 	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
 	const uint32_t nw_x = nbx * wpb;
-	const uint32_t len = c1 - c0;
-	const uint32_t part = (len + ngrp - 1) / ngrp;
-	const uint32_t lo = c0 + x * part;
-	const uint32_t hi = min(c1, lo + part);
 	uint32_t bad = 0;
 	static_assert(!(VM && CL), "vertex-major sweeps carry no labels");
-	for (uint32_t c = lo + lw; c < hi; c += nw_x) {
+	auto chunk = [&](uint32_t c) {
 		const uint32_t pos = (c << 6) + lane;
 		bool lbad = false;
 		if constexpr (VM) {
@@ -109,7 +108,7 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 				if (CHK && best == old) lbad = false;
 			}
 			bad += (uint32_t) __popcll(__ballot(lbad));
-			continue;
+			return;
 		}
 		const uint32_t cb = a.sb ? c * a.sb : a.coff[c], S = a.sb ? a.sb : a.coff[c + 1] - cb;
 		if (pos >= start && pos < end) {
@@ -125,19 +124,37 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 				// reference's GPU relax does (src/cuda/geodesics_ptp.cu:266-276).
 				const uint32_t *ocl = od == (const T *) a.dist0 ? a.cl0 : a.cl1;
 				uint32_t *ncl = od == (const T *) a.dist0 ? a.cl1 : a.cl0;
-				uint32_t c = ocl[v];
+				uint32_t cl = ocl[v];
 				if (bk != NO_TRI) {
 					const uint32_t ia = a.cid[base + 64 * (size_t) bk] & SLOT_ID_MASK;
 					const uint32_t nb = bk + 1 < S ? a.cid[base + 64 * (size_t) (bk + 1)] : SLOT_EMPTY;
 					const uint32_t ib = ((nb & SLOT_EMPTY) ? a.cid[base] : nb) & SLOT_ID_MASK;
-					c = od[ib] < od[ia] ? ocl[ib] : ocl[ia];
+					cl = od[ib] < od[ia] ? ocl[ib] : ocl[ia];
 				}
-				ncl[v] = c;
+				ncl[v] = cl;
 			}
 			if (pos < cend) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);
 			if (CHK && best == old) lbad = false;
 		}
 		bad += (uint32_t) __popcll(__ballot(lbad));
+	};
+	if (a.xblk && gridDim.x >= 8u) {
+		const uint32_t L = a.xblk, B = 1u << L;
+		const uint32_t q0 = c0 >> L, q1 = (c1 - 1) >> L;
+		const uint32_t qx = q0 + ((x + 8u - (q0 & 7u)) & 7u);     // XCD x's first block
+		for (uint32_t k = lw;; k += nw_x) {
+			const uint32_t q = qx + 8u * (k >> L);
+			if (q > q1) break;
+			const uint32_t c = (q << L) + (k & (B - 1u));
+			if (c >= c0 && c < c1) chunk(c);
+		}
+	} else {
+		const uint32_t ngrp = gridDim.x < 8u ? gridDim.x : 8u;
+		const uint32_t len = c1 - c0;
+		const uint32_t part = (len + ngrp - 1) / ngrp;
+		const uint32_t lo = c0 + x * part;
+		const uint32_t hi = min(c1, lo + part);
+		for (uint32_t c = lo + lw; c < hi; c += nw_x) chunk(c);
 	}
 	return bad;
 }

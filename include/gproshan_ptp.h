@@ -336,13 +336,16 @@ typedef struct gp_tune {
 	uint32_t replay;      /* sweeps per replay of the long captured graph              */
 	double kernel_ms;     /* device time per field of the chosen shape                 */
 	uint32_t candidates;  /* shapes timed (0: the choice came from the cache)          */
-	uint32_t pad;
+	uint32_t xcd_block;   /* log2 of the chunk blocks dealt round-robin to the eight
+	                         XCDs (a chunk then stays on one XCD's L2 every sweep);
+	                         0: each XCD takes a contiguous eighth of the window   */
 } gp_tune;
 
 /* Times the plan's sweep loop over candidate grids (a quarter, half, three
  * quarters and all of the workgroups that fit on the device at once) and
- * replay lengths (16, 32, 64, 128 sweeps): one warm-up run and the mean
- * device time of two runs each.  The fastest shape is kept in the plan and
+ * replay lengths (16, 32, 64, 128 sweeps), then the XCD deal of the best
+ * (contiguous eighths, or blocks of 16, 64, 256 chunks): one warm-up run and
+ * the mean device time of two runs each.  The fastest shape is kept in the plan and
  * remembered for plans of the same shape (vertex count rounded down to a power
  * of two, longest fan, precision, band or ring schedule, coalescing, uncapped
  * loop), which then take it without timing.  The plan's last result is that of
