@@ -44,15 +44,15 @@ B_RELAX_F64 = 92
 PEAK_HBM_GBS = 8000.0
 ERR_BAR = 1e-4
 # HBM bytes per dispatch of the headline kernel on the default workload, from
-# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, 1,184
-# dispatches of one band_learned run) scaled by what the same counters read for
-# a 256 MiB device-to-device copy (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
-# profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources).
-# Measured on an earlier build of the kernel (one slot stride per plan, four
-# waves per workgroup); the current tables hold fewer slot rows.
-MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((13037.35 * 2.0 + 766.30 * 1.0) * 1024,
+# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch)
+# scaled by what the same counters read for a 256 MiB device-to-device copy
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
+# profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources):
+# (bytes per dispatch, how they were measured).
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((13049.30 * 2.0 + 766.30 * 1.0) * 1024,
                                                 "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per dispatch, "
-                                                "6,640 dispatches of band_learned (contiguous XCD eighths)")}
+                                                "mean of 6,640 dispatches of band_learned (0.5-edge bands, contiguous "
+                                                "XCD eighths), kernels launched one by one under the profiler")}
 
 
 def log(*a):
