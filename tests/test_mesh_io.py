@@ -1,7 +1,8 @@
 """Mesh file readers (gp_che_read) with the reference's face conventions
 (src/che_off.cpp:28-79, src/che_obj.cpp:24-79, src/che_ply.cpp:25-156):
 small files written here, the faces each reader must produce derived by hand,
-and the CPU PTP of a read mesh equal to the PTP of the same arrays."""
+the CPU PTP of a read mesh equal to the PTP of the same arrays, and (GPU)
+PTP_GPU of an OFF and a binary PLY file against the FP64 oracle."""
 import struct
 
 import numpy as np
@@ -98,6 +99,49 @@ def test_read_mesh_runs_ptp_like_the_arrays(tmp_path, oracle):
     ts = ref.compute_toplesets([10])
     assert np.array_equal(P.parallel_toplesets_propagation_cpu(C, [10], ts),
                           P.parallel_toplesets_propagation_cpu(ref, [10], ts))
+
+
+def _write_off(path, xyz, f3):
+    with open(path, "w") as fh:
+        fh.write(f"OFF\n{len(xyz)} {len(f3)} 0\n")
+        np.savetxt(fh, xyz, fmt="%.17g")
+        np.savetxt(fh, np.hstack([np.full((len(f3), 1), 3), f3]), fmt="%d")
+
+
+def _write_ply_binary(path, xyz, f3):
+    with open(path, "wb") as fh:
+        fh.write((f"ply\nformat binary_little_endian 1.0\nelement vertex {len(xyz)}\nproperty double x\n"
+                  f"property double y\nproperty double z\nelement face {len(f3)}\n"
+                  f"property list uchar int vertex_indices\nend_header\n").encode())
+        xyz.astype("<f8").tofile(fh)
+        rec = np.zeros(len(f3), dtype=[("n", "u1"), ("v", "<i4", 3)])
+        rec["n"] = 3
+        rec["v"] = f3
+        rec.tofile(fh)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", ["off", "ply"])
+def test_read_mesh_ptp_gpu_against_oracle(tmp_path, oracle, fmt):
+    # A 128^2 height field written as an OFF text file (17 significant digits)
+    # and as a binary little-endian PLY file, read back by gp_che_read, then
+    # PTP_GPU of the front end (FP64 by default) against the FP64 oracle run on
+    # the arrays that were written: bit-identical, as the reader reproduces
+    # positions and faces exactly.
+    xyz, f = G.height_field(128, 41)
+    f3 = f.reshape(-1, 3)
+    src = [4321, 9876]
+    path = tmp_path / f"hf.{fmt}"
+    (_write_off if fmt == "off" else _write_ply_binary)(path, xyz, f3)
+    C = P.che.read(path)
+    assert np.array_equal(C.gt(), xyz) and np.array_equal(C.tables()[0].reshape(-1, 3), f3)
+    g = P.geodesics(C, src, P.geodesics.PTP_GPU)
+    ref, _ = oracle.Mesh(xyz, f).ptp(src, oracle.GPU_PLAIN)
+    print(f"\n[read {fmt}] V={len(xyz)} identical={np.array_equal(g.dist, ref)}", flush=True)
+    assert np.array_equal(g.dist.view(np.uint64), ref.view(np.uint64))
+    g32 = P.geodesics(C, src, P.geodesics.PTP_GPU, precision=L.PREC_F32)
+    m = ref > 0
+    assert np.max(np.abs(g32.dist[m] - ref[m]) / ref[m]) <= 1e-4
 
 
 def test_unknown_extension_and_missing_file(tmp_path):
