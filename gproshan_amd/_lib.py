@@ -21,7 +21,7 @@ OK, ERR_INVALID, ERR_NOMEM, ERR_DUPLICATE_SOURCE, ERR_HIP, ERR_NO_DEVICE, ERR_CA
 PREC_F32, PREC_F64 = 0, 1
 RESULT_LAST_WRITTEN, RESULT_LAST_READ = 0, 1
 SCHED_RINGS, SCHED_LEARNED_BAND, SCHED_EUCLID_BAND, SCHED_RING_BAND, SCHED_PATCHES = 0, 1, 2, 3, 4
-LAUNCH_GRAPH, LAUNCH_STREAM = 0, 1
+LAUNCH_GRAPH, LAUNCH_STREAM, LAUNCH_FLOW = 0, 1, 2
 GEO_AUTO, GEO_FP32, GEO_FP64 = 0, 1, 2
 
 # Every symbol include/gproshan_ptp.h declares.
@@ -73,6 +73,8 @@ class PtpStats(C.Structure):
 def sweep_kernel_name(bits: int) -> str:
     f64, cl, ms8, g64 = bool(bits & 1), bool(bits & 2), bool(bits & 4), bool(bits & 8)
     b = lambda x: "true" if x else "false"
+    if bits & 16:
+        return f"gp::k_flow<{'8u' if ms8 else '10u'}, {b(g64)}>"
     tail = ", true" if g64 else ""
     return f"gp::k_sweep<{b(f64)}, {b(cl)}, {'8u' if ms8 else '10u'}{tail}>"
 

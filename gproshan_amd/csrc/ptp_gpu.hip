@@ -44,6 +44,7 @@
 #include "device.hpp"
 #include "levels.hpp"
 #include "sweep_dev.hpp"
+#include "flow_kernel.hpp"
 #include "sweep_kernel.hpp"
 #include "update_step.hpp"
 
@@ -231,9 +232,20 @@ struct gp_plan {
 	bool has_run = false;
 	bool capped = false;                     // the last run stopped at its sweep cap
 	uint32_t check_sweeps = 0;               // check sweeps of the last run (band plans)
+	// Dataflow launch (GP_LAUNCH_FLOW): eligibility (-1: not yet checked) and buffers.
+	int flow_ok = -1;
+	bool flow_last = false;                  // the last run was one dataflow launch
+	uint32_t flow_grid = 0;
+	unsigned long long *d_pr0 = nullptr, *d_pr1 = nullptr, *d_facc = nullptr;
+	uint32_t *d_cflag = nullptr, *d_hist = nullptr;
+	gp::FlowState *d_fs = nullptr, *h_fs = nullptr;
 
 	~gp_plan()
 	{
+		for (void *q : { (void *) d_pr0, (void *) d_pr1, (void *) d_facc, (void *) d_cflag, (void *) d_hist,
+		                 (void *) d_fs })
+			(void) hipFree(q);
+		if (h_fs) (void) hipHostFree(h_fs);
 		for (void *p : { (void *) d_limits, (void *) d_order, (void *) d_src, (void *) d_trace, (void *) d_coff,
 		                 (void *) d_cid, (void *) d_cgeo, (void *) d_cgeo64, (void *) d_cx64, d_dist0, d_dist1,
 		                 (void *) d_out, (void *) d_state, (void *) d_fcum, (void *) d_cl0, (void *) d_cl1,
@@ -701,7 +713,7 @@ static void enqueue_sweeps(const gp_plan *p, const KArgs &a, uint32_t k0, uint32
 static uint32_t kernel_bits(const gp_plan *p)
 {
 	return (p->opts.precision == GP_PREC_F64 ? 1u : 0u) | (p->d_cl0 ? 2u : 0u) | (p->ms8 ? 4u : 0u) |
-	       (p->g64 && p->opts.precision != GP_PREC_F64 ? 8u : 0u);
+	       (p->g64 && p->opts.precision != GP_PREC_F64 ? 8u : 0u) | (p->flow_last ? 16u : 0u);
 }
 
 static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *out)
@@ -716,6 +728,83 @@ static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *o
 	return GP_OK;
 }
 
+// The dataflow kernel variant of a plan.
+static const void *flow_kernel(const gp_plan *p)
+{
+	if (p->g64) return p->ms8 ? (const void *) k_flow<8, true> : (const void *) k_flow<MAXS, true>;
+	return p->ms8 ? (const void *) k_flow<8, false> : (const void *) k_flow<MAXS, false>;
+}
+
+// Whether the plan can run as one dataflow launch: FP32 without labels on the
+// ring-numbered chunked layout, every fan in registers, every ring nonempty
+// (a tested ring then has a chunk to publish the next state), at most
+// FLOW_MAX_OWN chunks per resident wave.  Allocates the buffers on success.
+static int flow_prepare(gp_plan *p, bool &ok)
+{
+	ok = false;
+	if (p->flow_ok >= 0) {
+		ok = p->flow_ok != 0;
+		return GP_OK;
+	}
+	p->flow_ok = 0;
+	if (p->opts.precision == GP_PREC_F64 || p->d_cl0 || p->d_order || p->pp || p->max_slots > MAXS || p->nl < 3)
+		return GP_OK;
+	std::vector<uint32_t> lim(p->nl);
+	GP_HIP(hipMemcpy(lim.data(), p->d_limits, 4ull * p->nl, hipMemcpyDeviceToHost));
+	for (uint32_t q = 1; q < p->nl; q++)
+		if (lim[q] <= lim[q - 1]) return GP_OK;
+	if (lim[p->nl - 1] != p->nr || p->max_iter >= FLOW_IMASK) return GP_OK;
+	int dev = 0, ncu = 0, occ = 0;
+	GP_HIP(hipGetDevice(&dev));
+	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	GP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, flow_kernel(p), 64, 0));
+	const uint32_t waves = (uint32_t) ncu * (uint32_t) std::max(occ, 0);
+	const uint32_t nch = (p->nr + 63) / 64;
+	if (waves == 0 || nch > FLOW_MAX_OWN * (size_t) waves) return GP_OK;
+	p->flow_grid = waves;
+	const size_t nh = (size_t) p->max_iter + 2;
+	if (hipMalloc(&p->d_pr0, 8ull * nch * 64) != hipSuccess || hipMalloc(&p->d_pr1, 8ull * nch * 64) != hipSuccess ||
+	    hipMalloc(&p->d_facc, 8 * nh) != hipSuccess || hipMalloc(&p->d_cflag, 4ull * nch) != hipSuccess ||
+	    hipMalloc(&p->d_hist, 4 * nh) != hipSuccess || hipMalloc(&p->d_fs, sizeof(FlowState)) != hipSuccess ||
+	    hipHostMalloc(&p->h_fs, sizeof(FlowState)) != hipSuccess)
+		return fail(GP_ERR_NOMEM, "plan: dataflow buffer allocation failed");
+	p->flow_ok = 1;
+	ok = true;
+	return GP_OK;
+}
+
+// One run of the schedule as a single cooperative launch (after k_init and
+// k_sources): pairs from the initial distances, the dataflow sweep, both
+// distance buffers back from the pairs, and the state snapshot; the event
+// ev[0] ends the timed region.
+static int flow_run(gp_plan *p, const KArgs &a)
+{
+	const uint32_t nch = (p->nr + 63) / 64;
+	const uint32_t nh = p->max_iter + 2;
+	const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(2048, (p->n + 255) / 256));
+	k_flow_init<<<g, 256, 0, p->stream>>>((const float *) p->d_dist0, p->d_pr0, p->d_pr1, p->nr, p->d_cflag, nch,
+	                                       p->d_hist, p->d_facc, nh, p->d_fs);
+	GP_HIP(hipGetLastError());
+	FlowArgs fa{};
+	fa.a = a;
+	fa.pr0 = p->d_pr0; fa.pr1 = p->d_pr1; fa.cflag = p->d_cflag; fa.hist = p->d_hist; fa.acc = p->d_facc;
+	fa.fs = p->d_fs;
+	fa.nchunks = nch;
+	fa.nr = p->nr;
+	fa.budget = 2000000000ull;   // 20 s at the 100 MHz wall clock
+	void *args[] = { &fa };
+	GP_HIP(hipLaunchCooperativeKernel(flow_kernel(p), dim3(p->flow_grid), dim3(64), args, 0, p->stream));
+	k_flow_unpack<<<g, 256, 0, p->stream>>>(p->d_pr0, p->d_pr1, (float *) p->d_dist0, (float *) p->d_dist1, p->nr);
+	snapshot(p->d_state, p->h_state, p->stream);
+	snapshot(p->d_fs, p->h_fs, p->stream);
+	GP_HIP(hipGetLastError());
+	GP_HIP(hipEventRecord(p->ev[0], p->stream));
+	GP_HIP(hipStreamSynchronize(p->stream));
+	if (p->h_fs->abort) return fail(GP_ERR_KERNEL, "ptp: the dataflow sweep passed its deadline");
+	if (!p->h_state->done) return fail(GP_ERR_KERNEL, "ptp: the dataflow sweep ended before the schedule");
+	return GP_OK;
+}
+
 int plan_run(gp_plan *p, gp_ptp_stats *stats)
 {
 	if (!p) return fail(GP_ERR_INVALID, "plan is null");
@@ -726,7 +815,13 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (p->n + 255) / 256));
 	const KArgs a = make_args(p);
 	const uint32_t BIG = p->big, SMALL = p->small;
-	const bool graph = p->opts.launch_mode == GP_LAUNCH_GRAPH && graphs_enabled();
+	bool flow = false;
+	if (p->opts.launch_mode == GP_LAUNCH_FLOW) {
+		int rc = flow_prepare(p, flow);
+		if (rc) return rc;
+	}
+	p->flow_last = flow;
+	const bool graph = !flow && p->opts.launch_mode == GP_LAUNCH_GRAPH && graphs_enabled();
 	if (graph && !p->g_big) {
 		int rc = capture(p, a, BIG, &p->g_big);
 		if (!rc) rc = capture(p, a, SMALL, &p->g_small);
@@ -771,7 +866,11 @@ int plan_run(gp_plan *p, gp_ptp_stats *stats)
 	};
 	uint32_t cur = 0;
 	bool done = done0 != 0;
-	if (!done) {
+	if (!done && flow) {
+		int rc = flow_run(p, a);
+		if (rc) return rc;
+		done = true;
+	} else if (!done) {
 		int rc = replay(0);
 		if (rc) return rc;
 		for (;;) {

@@ -52,18 +52,25 @@ __device__ __forceinline__ double tri_f64(const Edge64 &xa, const Edge64 &xb, do
 // bits with one 32-byte read per slot and no second pass.
 constexpr uint32_t NO_TRI = 0xFFFFFFFFu;
 
+// Distances of slots 0 .. S-1 of a lane (INF past S).  OD is a distance
+// array here; the dataflow sweep passes an accessor with its own overload that
+// waits until each neighbour's value of the previous sweep is published.
+template <uint32_t MS, typename T>
+__device__ __forceinline__ void gather_slots(const T *od, const uint32_t (&id)[MS], uint32_t S, T (&t)[MS])
+{
+#pragma unroll
+	for (uint32_t k = 0; k < MS; k++) t[k] = (k < S) ? od[id[k] & SLOT_ID_MASK] : (T) INFINITY;
+}
+
 // FP32 relaxation of a lane whose fan is closed and stored without its
 // repeat, with SC real slots and no flagged triangle: triangle k joins slots
 // k and (k + 1) mod SC.  These are the values and the order of the general
 // register path of relax_lane for such a lane, with each slot's edge length
 // computed once and no per-slot selects.
 template <uint32_t SC, uint32_t MS, typename A>
-__device__ __forceinline__ float relax_closed(const A &a, const uint32_t (&id)[MS], const float4 (&g)[MS],
-                                              const float *od, float best)
+__device__ __forceinline__ float relax_closed(const A &a, const float4 (&g)[MS], const float (&t)[MS], float best)
 {
-	float t[SC], s[SC];
-#pragma unroll
-	for (uint32_t k = 0; k < SC; k++) t[k] = od[id[k] & SLOT_ID_MASK];
+	float s[SC];
 #pragma unroll
 	for (uint32_t k = 0; k < SC; k++) s[k] = stable_sqrt(g[k].x);
 #pragma unroll
@@ -77,10 +84,10 @@ __device__ __forceinline__ float relax_closed(const A &a, const uint32_t (&id)[M
 }
 
 template <bool F64, uint32_t ST = 64, bool PF = true, bool CL = false, uint32_t MS = MAXS, bool G64 = false,
-          typename A>
+          typename A, typename OD>
 __device__ __forceinline__ typename std::conditional<F64, double, float>::type
-relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<F64, double, float>::type *od,
-           typename std::conditional<F64, double, float>::type best, uint32_t *bk_out = nullptr)
+relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::conditional<F64, double, float>::type best,
+           uint32_t *bk_out = nullptr)
 {
 	using T = typename std::conditional<F64, double, float>::type;
 	using G = typename std::conditional<F64, Edge64, float4>::type;
@@ -111,9 +118,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 				if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = a.cgeo64[base + ST * k]; }
 			}
 			const bool wrap = (id[0] & SLOT_WRAP) != 0u;
-#pragma unroll
-			for (uint32_t k = 0; k < MS; k++)
-				t[k] = (k < S) ? od[id[k] & SLOT_ID_MASK] : (T) INFINITY;
+			gather_slots(od, id, S, t);
 #pragma unroll
 			for (uint32_t k = 0; k < MS; k++) {
 				if (k < S) {
@@ -154,15 +159,15 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 #pragma unroll
 			for (uint32_t k = 0; k < MS; k++)
 				if (k < S) plain = plain && !(id[k] & (SLOT_EMPTY | F64_FLAG));
-			if (__ballot(!plain) == 0) {
+			if (__ballot(!plain) == 0 && S >= 3 && S <= 8 && S <= MS) {
+				gather_slots(od, id, S, t);
 				switch (S) {
-				case 3: return relax_closed<3>(a, id, g, od, best);
-				case 4: return relax_closed<4>(a, id, g, od, best);
-				case 5: return relax_closed<(MS >= 5 ? 5 : MS)>(a, id, g, od, best);
-				case 6: return relax_closed<(MS >= 6 ? 6 : MS)>(a, id, g, od, best);
-				case 7: return relax_closed<(MS >= 7 ? 7 : MS)>(a, id, g, od, best);
-				case 8: return relax_closed<(MS >= 8 ? 8 : MS)>(a, id, g, od, best);
-				default: break;
+				case 3: return relax_closed<3>(a, g, t, best);
+				case 4: return relax_closed<4>(a, g, t, best);
+				case 5: return relax_closed<(MS >= 5 ? 5 : MS)>(a, g, t, best);
+				case 6: return relax_closed<(MS >= 6 ? 6 : MS)>(a, g, t, best);
+				case 7: return relax_closed<(MS >= 7 ? 7 : MS)>(a, g, t, best);
+				default: return relax_closed<(MS >= 8 ? 8 : MS)>(a, g, t, best);
 				}
 			}
 		}
@@ -187,9 +192,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const typename std::conditional<
 				q10 = a.cgeo64[base + ST * (size_t) kn0].x;
 			}
 		}
-#pragma unroll
-		for (uint32_t k = 0; k < MS; k++)
-			t[k] = (k < S) ? od[id[k] & SLOT_ID_MASK] : (T) INFINITY;
+		gather_slots(od, id, S, t);
 		T fa = (T) INFINITY, fb = (T) INFINITY;
 #pragma unroll
 		for (uint32_t k = 0; k < MS; k++) {

@@ -80,10 +80,16 @@ enum gp_schedule {
 	GP_SCHED_PATCHES = 4       /* patch solver (optionally gated by a caller's dhat)                */
 };
 
-/* How the sweeps are launched. */
+/* How the sweeps are launched.  GP_LAUNCH_FLOW runs the whole schedule of an
+ * FP32 ring or band plan (no cluster labels, coalesce = 1) as one persistent
+ * cooperative launch in which each 64-vertex chunk starts sweep k as soon as
+ * the neighbour values of sweep k-1 it reads are published: the same windows,
+ * ring tests and arithmetic, hence the same bits and sweep counts; other
+ * plans run GP_LAUNCH_GRAPH (gp_ptp_stats.kernel tells which ran). */
 enum gp_launch_mode {
 	GP_LAUNCH_GRAPH = 0,   /* sweep + schedule kernels replayed from captured hipGraphs (default) */
-	GP_LAUNCH_STREAM = 1   /* the same kernels launched one by one on the stream                 */
+	GP_LAUNCH_STREAM = 1,  /* the same kernels launched one by one on the stream                 */
+	GP_LAUNCH_FLOW = 2     /* one dataflow launch for the whole schedule                          */
 };
 
 /* Geometry table the FP32 ring sweeps read.  Both forms give the same bits:
@@ -159,7 +165,7 @@ typedef struct gp_ptp_stats {
 	                             band loops, which then return GP_ERR_NOT_CONVERGED)  */
 	uint32_t kernel;          /* sweep kernel variant: bit 0 FP64, bit 1 cluster labels,
 	                             bit 2 eight register-held slots, bit 3 FP64 geometry
-	                             table (FP32 sweeps)                                  */
+	                             table (FP32 sweeps), bit 4 the dataflow launch ran  */
 	uint32_t check_sweeps;    /* band schedules: full sweeps of the final convergence
 	                             check over every reached vertex (1 when the bands
 	                             already met tol)                                     */
