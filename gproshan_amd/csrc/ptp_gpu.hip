@@ -791,7 +791,14 @@ static int flow_run(gp_plan *p, const KArgs &a)
 	fa.fs = p->d_fs;
 	fa.nchunks = nch;
 	fa.nr = p->nr;
-	fa.budget = 2000000000ull;   // 20 s at the 100 MHz wall clock
+	// Wall-clock budget of the run (100 MHz ticks): 20 s, or
+	// GPROSHAN_AMD_FLOW_DEADLINE_US microseconds (a short one makes the
+	// bounded waits give up, which is how the failure path is tested).
+	fa.budget = 2000000000ull;
+	if (const char *e = std::getenv("GPROSHAN_AMD_FLOW_DEADLINE_US")) {
+		const unsigned long long us = std::strtoull(e, nullptr, 10);
+		if (us) fa.budget = std::min<unsigned long long>(us * 100ull, fa.budget);
+	}
 	void *args[] = { &fa };
 	GP_HIP(hipLaunchCooperativeKernel(flow_kernel(p), dim3(p->flow_grid), dim3(64), args, 0, p->stream));
 	k_flow_unpack<<<g, 256, 0, p->stream>>>(p->d_pr0, p->d_pr1, (float *) p->d_dist0, (float *) p->d_dist1, p->nr);

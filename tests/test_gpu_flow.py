@@ -133,3 +133,26 @@ def test_flow_full_size(rows):
         kw.update(uncapped=True, tol=1e-6, schedule=L.SCHED_EUCLID_BAND, band_width=0.5)
     print(f"\n[full size] {rows}", flush=True)
     _check(mesh, src, reps=2, **kw)
+
+
+def test_flow_deadline_reports_error(monkeypatch):
+    """A deadline shorter than the run makes the bounded waits give up: every
+    wave leaves the kernel, the run returns GP_ERR_KERNEL, and the same plan
+    runs normally (bit-identical to the graph path) once the deadline is back."""
+    xyz, f = G.height_field(256, 3)
+    mesh = P.che(xyz, f)
+    src = np.array([20000], np.uint32)
+    kw = dict(precision=L.PREC_F32, fp32_geometry=L.GEO_FP32, uncapped=True, tol=1e-6)
+    (sg, dg, tg), = _run(mesh, src, None, L.LAUNCH_GRAPH, **kw)
+    plan = P.Plan(mesh, src, None, launch_mode=L.LAUNCH_FLOW, **kw)
+    monkeypatch.setenv("GPROSHAN_AMD_FLOW_DEADLINE_US", "2")
+    with pytest.raises(P.GPError) as e:
+        plan.run()
+    assert e.value.status == L.ERR_KERNEL, e.value
+    print(f"  short deadline: {e.value}", flush=True)
+    monkeypatch.delenv("GPROSHAN_AMD_FLOW_DEADLINE_US")
+    st = plan.run()
+    assert st["kernel"] & FLOW_BIT
+    assert np.array_equal(plan.result(), dg, equal_nan=True)
+    assert st["sweeps"] == sg["sweeps"]
+    plan.close()
