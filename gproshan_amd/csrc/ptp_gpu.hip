@@ -1086,12 +1086,16 @@ void tune_load_locked()
 			if (std::fscanf(f, "%u %u %u %u %u %u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.f64, &k.band,
 			                &k.coalesce, &k.uncapped, &t.grid, &t.replay, &t.xcd_block, &t.kernel_ms) != 10)
 				break;
-			if (t.grid && t.replay && t.replay % 4 == 0 && t.xcd_block < 16) g_tune_cache[k] = t;
+			// Only shapes the tuner itself can choose (a damaged file must not
+			// make a plan capture an oversized graph).
+			if (t.grid && t.grid <= (1u << 20) && t.replay >= 4 && t.replay <= 1024 && t.replay % 4 == 0 &&
+			    t.xcd_block < 16)
+				g_tune_cache[k] = t;
 		} else if (!std::strcmp(kind, "width")) {
 			WidthKey k{};
 			double w = 0;
 			if (std::fscanf(f, "%u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.schedule, &k.f64, &w) != 5) break;
-			if (w > 0) g_width_cache[k] = w;
+			if (w > 0 && w <= 64) g_width_cache[k] = w;
 		} else {
 			break;
 		}

@@ -54,3 +54,46 @@ def test_band_width_tuner(oracle):
     plan.close()
     assert w in widths
     assert mx <= 1e-4
+
+
+_CHILD = r"""
+import json, sys
+sys.path.insert(0, sys.argv[1])
+from gproshan_amd import _lib as L, geodesics as P, meshgen as G
+xyz, f = G.height_field(96, 23)
+plan = P.Plan(P.che(xyz, f), [4000], precision=L.PREC_F32, uncapped=True, tol=1e-6)
+t = plan.autotune()
+plan.close()
+print(json.dumps(t))
+"""
+
+
+def _tune_in_child(tmp_path, tune_file):
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, GPROSHAN_AMD_TUNE_FILE=str(tune_file))
+    r = subprocess.run([sys.executable, "-c", _CHILD, root], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_tune_file_persists_across_processes(tmp_path):
+    """GPROSHAN_AMD_TUNE_FILE: the first process times the candidates and
+    appends its choice; a second process reads it and times nothing; lines a
+    damaged file holds (unknown kind, out-of-range shape) are not taken."""
+    tf = tmp_path / "tune.txt"
+    first = _tune_in_child(tmp_path, tf)
+    print(f"\n[tune file] first process: {first}; file: {tf.read_text().strip()}", flush=True)
+    assert first["candidates"] > 0
+    second = _tune_in_child(tmp_path, tf)
+    assert second["candidates"] == 0
+    assert (second["grid"], second["replay"], second["xcd_block"]) == (first["grid"], first["replay"], first["xcd_block"])
+    # A file whose only entry for this shape is out of range: tuned again.
+    key = tf.read_text().split()[1:7]
+    bad = tmp_path / "bad.txt"
+    bad.write_text("shape " + " ".join(key) + " 4096 1000000 0 1.0\n")
+    third = _tune_in_child(tmp_path, bad)
+    assert third["candidates"] > 0 and third["replay"] in (16, 32, 64, 128)
