@@ -201,7 +201,8 @@ bool prefer_f64(const Che &m)
 				const double cz = X0[0] * X1[1] - X0[1] * X1[0];
 				const double det = cx * cx + cy * cy + cz * cz;
 				const double qq = q0 * q1;
-				if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) nflag++;
+				if (q0 * q1 - g * g == 0 || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq))
+					nflag++;
 			}
 		}
 		m.n_fan_slots = nslot;
@@ -290,7 +291,18 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 					gd.z = det * ih2 * ih2;
 					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
 					const double qq = q0 * q1;
-					if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) { id |= F64_FLAG; nflag++; }
+					// The reference's determinant q00 q11 - q01 q10 (its
+					// operations, in its order) is exactly 0 for a zero-length
+					// edge or exactly collinear edges: its update is NaN there
+					// and the triangle never wins.  det = NaN in the table gives
+					// the FP64 fallback the same outcome (the edge paths still
+					// apply when an input is INF, as in the reference).
+					const bool ref_degenerate = q0 * q1 - g * g == 0;
+					if (ref_degenerate) gd.z = std::nan("");
+					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) {
+						id |= F64_FLAG;
+						nflag++;
+					}
 				}
 				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w };
 				L.cgeo64[at] = gd;

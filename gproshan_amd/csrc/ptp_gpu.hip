@@ -482,7 +482,12 @@ __global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, co
 					gd.z = det * ih2 * ih2;
 					gd.w = (E[0] * E[0] + E[1] * E[1] + E[2] * E[2]) * ih2;
 					const double qq = q0 * q1;
-					if (!(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq)) id |= F64_FLAG;
+					// det = NaN where the reference's own determinant is 0
+					// (its update is NaN there; see build_layout).
+					const bool ref_degenerate = q0 * q1 - g * g == 0;
+					if (ref_degenerate) gd.z = __longlong_as_double(0x7FF8000000000000ll);
+					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq))
+						id |= F64_FLAG;
 				}
 				cgeo[at] = make_float4((float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w);
 				cgeo64[at] = gd;

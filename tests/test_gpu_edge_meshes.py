@@ -1,0 +1,152 @@
+"""GPU parity on meshes the benchmark workloads do not contain.
+
+  * two components (a height field and an icosphere): the component without a
+    source stays INF, and with a source in each both fronts run in one window;
+  * a height field with holes cut out (internal boundaries, open fans inside);
+  * 300 sources (ring 0 spans several 64-vertex chunks);
+  * collapsed vertices (zero-length edges): the reference's update divides by
+    its determinant q00 q11 - q01 q10, exactly 0 there, and returns NaN, which
+    loses every comparison (src/geodesics_ptp.cpp:201-262); the FP32 path's
+    tables mark such corners (det = NaN, FP64 fallback) so that they lose too
+    -- before they did, the FP32 field of this mesh was 6.2e-2 off;
+  * tiny meshes: one triangle, a quad, and a mesh whose every vertex is a
+    source (no sweep runs).
+
+Each mesh: GPU toplesets bit-identical to the oracle's; the reference schedule
+in FP64 bit-identical to the oracle's restatement (coalesced and sorted[]
+drivers, same sweep count), with host- and device-built rings alike; the
+uncapped loop in FP64 likewise; FP32 within 1e-4 of the FP64 oracle; the
+dataflow launch bit-identical to the graph path.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from gproshan_amd import _lib as L
+from gproshan_amd import geodesics as P
+from gproshan_amd import meshgen as G
+
+pytestmark = pytest.mark.gpu
+
+
+def two_components():
+    a, fa = G.height_field(64, 31)
+    b, fb = G.icosphere(3, 0.01, 2)
+    b = b + np.array([5.0, 0.0, 0.0])
+    return np.vstack([a, b]), np.vstack([fa, fb + len(a)]).astype(np.uint32)
+
+
+def holes():
+    n = 96
+    xyz, f = G.height_field(n, 32)
+    # Drop the faces whose first vertex lies in one of three square windows.
+    r, c = f[:, 0] // n, f[:, 0] % n
+    cut = np.zeros(len(f), bool)
+    for r0, c0, s in ((20, 20, 8), (50, 60, 12), (70, 15, 6)):
+        cut |= (r >= r0) & (r < r0 + s) & (c >= c0) & (c < c0 + s)
+    keep = f[~cut]
+    used = np.unique(keep)
+    remap = np.full(len(xyz), -1, np.int64)
+    remap[used] = np.arange(len(used))
+    return xyz[used], remap[keep].astype(np.uint32)
+
+
+def collapsed():
+    n = 64
+    xyz, f = G.height_field(n, 33)
+    xyz = xyz.copy()
+    for v in (n * 10 + 10, n * 30 + 41, n * 50 + 7):
+        xyz[v] = xyz[v + 1]          # the vertex moves onto its right neighbour
+    return xyz, f
+
+
+def one_triangle():
+    return np.array([[0, 0, 0], [1, 0, 0], [0.2, 0.9, 0.1]], float), np.array([[0, 1, 2]], np.uint32)
+
+
+def quad():
+    return (np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0]], float),
+            np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+
+
+CASES = [
+    ("two_components_one_source", two_components, lambda nv: [100]),
+    ("two_components_two_sources", two_components, lambda nv: [100, nv - 5]),
+    ("holes", holes, lambda nv: [nv // 2]),
+    ("sources300", lambda: G.height_field(128, 34), lambda nv: G.draw_sources(np.random.default_rng(34), nv, 300)),
+    ("collapsed", collapsed, lambda nv: [n for n in (64 * 32 + 32,)]),
+    ("one_triangle", one_triangle, lambda nv: [0]),
+    ("quad", quad, lambda nv: [1]),
+    ("all_sources", lambda: G.height_field(8, 35), lambda nv: list(range(nv))),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request, oracle):
+    name, gen, pick = request.param
+    xyz, f = gen()
+    src = np.array(sorted(set(int(s) for s in pick(len(xyz)))), np.uint32)
+    print(f"\n[case] {name} V={len(xyz)} F={len(f)} sources={len(src)}", flush=True)
+    return name, xyz, f, src, oracle.Mesh(xyz, f), P.che(xyz, f)
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a, np.float64), np.asarray(b, np.float64), equal_nan=True)
+
+
+def test_toplesets(case):
+    name, xyz, f, src, M, C = case
+    ref = M.toplesets(src)
+    got = C.compute_toplesets(src, gpu=True)
+    for a, b, what in zip(got, ref, ("toplesets", "sorted", "limits")):
+        assert np.array_equal(a, b), what
+
+
+@pytest.mark.parametrize("coal", [True, False], ids=["coalesce", "plain"])
+def test_f64_reference_schedule(case, oracle, coal):
+    name, xyz, f, src, M, C = case
+    ref, nsw = M.ptp(src, oracle.GPU_COAL if coal else oracle.GPU_PLAIN)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F64, coalesce=coal)
+    print(f"  f64 coal={coal}: sweeps {st['sweeps']}/{nsw}", flush=True)
+    assert st["sweeps"] == nsw
+    assert _same(d, ref)
+
+
+def test_f64_device_built(case, oracle):
+    name, xyz, f, src, M, C = case
+    ref, nsw = M.ptp(src, oracle.GPU_COAL)
+    plan = P.Plan(C, src, precision=L.PREC_F64, device_setup=True)
+    st = plan.run()
+    assert st["sweeps"] == nsw
+    assert _same(plan.result(), ref)
+    plan.close()
+
+
+def test_f64_uncapped(case, oracle):
+    name, xyz, f, src, M, C = case
+    ref, nsw = M.ptp(src, oracle.GPU_COAL | oracle.UNCAPPED, tol=1e-5)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F64, tol=1e-5, uncapped=True)
+    assert st["sweeps"] == nsw
+    assert _same(d, ref)
+
+
+def test_f32_within_bar(case, oracle):
+    name, xyz, f, src, M, C = case
+    ref, _ = M.ptp(src, oracle.GPU_COAL)
+    d, _, st = P.ptp(C, src, precision=L.PREC_F32, fp32_geometry=L.GEO_FP32)
+    mx, mean = rel_err(d, ref)
+    print(f"  f32: max {mx:.2e} mean {mean:.2e} f64 updates {st['f64_updates']}", flush=True)
+    assert mx <= 1e-4
+
+
+def test_flow_matches_graph(case):
+    name, xyz, f, src, M, C = case
+    out = {}
+    for mode in (L.LAUNCH_GRAPH, L.LAUNCH_FLOW):
+        plan = P.Plan(C, src, precision=L.PREC_F32, fp32_geometry=L.GEO_FP32, uncapped=True, tol=1e-6,
+                      launch_mode=mode)
+        st = plan.run()
+        out[mode] = (st["sweeps"], plan.result())
+        plan.close()
+    assert out[L.LAUNCH_GRAPH][0] == out[L.LAUNCH_FLOW][0]
+    assert _same(out[L.LAUNCH_GRAPH][1], out[L.LAUNCH_FLOW][1])
