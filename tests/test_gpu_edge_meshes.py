@@ -150,3 +150,29 @@ def test_flow_matches_graph(case):
         plan.close()
     assert out[L.LAUNCH_GRAPH][0] == out[L.LAUNCH_FLOW][0]
     assert _same(out[L.LAUNCH_GRAPH][1], out[L.LAUNCH_FLOW][1])
+
+
+def test_batch_on_edge_meshes(oracle):
+    """The batched solver on the holed, collapsed and two-component meshes in
+    one launch: every field within 1e-4 of the FP64 oracle (sorted[] driver,
+    variant GPU_PLAIN) and equal to the single-field FP32 solve."""
+    gens = [("holes", holes), ("collapsed", collapsed), ("two_components", two_components)]
+    meshes, orc, jobs = [], [], []
+    for k, (name, gen) in enumerate(gens):
+        xyz, f = gen()
+        meshes.append(P.che(xyz, f))
+        orc.append(oracle.Mesh(xyz, f))
+        for s in (0, len(xyz) // 3, len(xyz) - 1):
+            jobs.append((k, s))
+    B = P.Batch(meshes, jobs, block=1024)
+    B.run()
+    for q, (k, s) in enumerate(jobs):
+        d, sw = B.result(q)
+        ref, nsw = orc[k].ptp(np.array([s], np.uint32), oracle.GPU_PLAIN)
+        mx, _ = rel_err(d, ref)
+        single, _, st = P.ptp(meshes[k], [s], precision=L.PREC_F32, fp32_geometry=L.GEO_FP32, coalesce=False)
+        print(f"  batch {gens[k][0]} source {s}: sweeps {sw}/{nsw} max {mx:.2e}", flush=True)
+        assert mx <= 1e-4
+        assert sw == st["sweeps"]
+        assert _same(d, single)
+    B.close()

@@ -18,9 +18,26 @@ def _data(n=4096, seed=0):
     return F, t
 
 
-def test_features_match_oracle(oracle):
-    xyz, f = G.height_field(33, 4)
-    src = [100, 700]
+def _holed(n=40, seed=9):
+    xyz, f = G.height_field(n, seed)
+    r, c = f[:, 0] // n, f[:, 0] % n
+    keep = f[~((r >= 10) & (r < 16) & (c >= 12) & (c < 20))]
+    used = np.unique(keep)
+    remap = np.full(len(xyz), -1, np.int64)
+    remap[used] = np.arange(len(used))
+    return xyz[used], remap[keep].astype(np.uint32)
+
+
+@pytest.mark.parametrize("mesh", ["hf33", "holed40", "ico3"])
+def test_features_match_oracle(oracle, mesh):
+    """All ten per-vertex features: on an open height field, on one with an
+    internal hole (border vertices inside the mesh) and on a closed sphere."""
+    if mesh == "hf33":
+        (xyz, f), src = G.height_field(33, 4), [100, 700]
+    elif mesh == "holed40":
+        (xyz, f), src = _holed(), [5, 900]
+    else:
+        (xyz, f), src = G.icosphere(3, 0.01, 4), [0, 300]
     M = oracle.Mesh(xyz, f)
     C = P.che(xyz, f)
     lev = M.toplesets(src)[0]
