@@ -349,7 +349,13 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 	if (o.schedule != GP_SCHED_RINGS && o.schedule != GP_SCHED_LEARNED_BAND && o.schedule != GP_SCHED_EUCLID_BAND &&
 	    o.schedule != GP_SCHED_RING_BAND)
 		return gp::fail(GP_ERR_INVALID, "unknown schedule %d", o.schedule);
-	if (o.device_setup && o.schedule == GP_SCHED_RINGS && o.coalesce && !o.clusters) {
+	// Device-built plans read the mesh's own fans.  The reference's coalescence
+	// rebuilds the half-edge mesh from the reached faces in ring order
+	// (src/geodesics_ptp.cpp:15-38), which on a non-manifold mesh can change a
+	// vertex's star (a bowtie vertex whose other wing is unreached becomes an
+	// ordinary one); those meshes take the host build, which rebuilds it too.
+	const bool dev_ok = m.manifold;
+	if (dev_ok && o.device_setup && o.schedule == GP_SCHED_RINGS && o.coalesce && !o.clusters) {
 		const int rd = gp::plan_create_device(&mesh->m, sources, n_sources, o, out);
 		if (rd != GP_ERR_CAPACITY) return rd;
 		gp::clear_error();   // a fan too long for the device build: build on the host
@@ -376,7 +382,7 @@ static int plan_create_any(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 	gp_ptp_opts ob = o;
 	ob.coalesce = 1;
 	ob.uncapped = 1;
-	if (o.device_setup && !o.clusters && dhat &&
+	if (dev_ok && o.device_setup && !o.clusters && dhat &&
 	    (o.schedule == GP_SCHED_LEARNED_BAND || o.schedule == GP_SCHED_EUCLID_BAND)) {
 		const int rd = gp::plan_create_device(&mesh->m, sources, n_sources, ob, out, dhat, width);
 		if (rd != GP_ERR_CAPACITY) return rd;

@@ -138,8 +138,10 @@ typedef struct gp_ptp_opts {
 	                       order inside a ring is not the host's, which changes
 	                       no value, and f64_updates is not counted); falls
 	                       back to the host build when a fan is longer than the
-	                       sweep kernels' register-held slots or a band index
-	                       would pass the vertex count                        */
+	                       sweep kernels' register-held slots, a band index
+	                       would pass the vertex count or the mesh is not
+	                       manifold (the reference's coalescence rebuilds its
+	                       half-edge mesh, which can change a star there)      */
 	int fp32_geometry;  /* gp_fp32_geometry (precision F32, ring schedules)     */
 	int band_check;     /* band schedules: 1 (default) = end every run with check
 	                       sweeps over all reached vertices until none moves by

@@ -10,7 +10,10 @@
     tables mark such corners (det = NaN, FP64 fallback) so that they lose too
     -- before they did, the FP32 field of this mesh was 6.2e-2 off;
   * tiny meshes: one triangle, a quad, and a mesh whose every vertex is a
-    source (no sweep runs).
+    source (no sweep runs);
+  * non-manifold input (a bowtie vertex, an edge with three triangles), where
+    the half-edge tables follow the reference's (NIL twins and EVT) and so does
+    what propagates.
 
 Each mesh: GPU toplesets bit-identical to the oracle's; the reference schedule
 in FP64 bit-identical to the oracle's restatement (coalesced and sorted[]
@@ -64,6 +67,19 @@ def one_triangle():
     return np.array([[0, 0, 0], [1, 0, 0], [0.2, 0.9, 0.1]], float), np.array([[0, 1, 2]], np.uint32)
 
 
+def bowtie():
+    """Two triangles sharing only vertex 0 (non-manifold: the reference's
+    che leaves EVT[0] = NIL and propagates nothing through vertex 0)."""
+    return (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0]], float),
+            np.array([[0, 1, 2], [0, 3, 4]], np.uint32))
+
+
+def book3():
+    """Three triangles on one edge (a non-manifold edge)."""
+    return (np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0], [0.5, -1, 0], [0.5, 0, 1]], float),
+            np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4]], np.uint32))
+
+
 def quad():
     return (np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0]], float),
             np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
@@ -78,6 +94,8 @@ CASES = [
     ("one_triangle", one_triangle, lambda nv: [0]),
     ("quad", quad, lambda nv: [1]),
     ("all_sources", lambda: G.height_field(8, 35), lambda nv: list(range(nv))),
+    ("bowtie", bowtie, lambda nv: [1]),
+    ("book3", book3, lambda nv: [2]),
 ]
 
 
