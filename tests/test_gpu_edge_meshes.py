@@ -194,3 +194,17 @@ def test_batch_on_edge_meshes(oracle):
         assert sw == st["sweeps"]
         assert _same(d, single)
     B.close()
+
+
+@pytest.mark.parametrize("coal", [True, False], ids=["coalesce", "plain"])
+def test_cluster_labels_f64(case, oracle, coal):
+    """Cluster labels (src/cuda/geodesics_ptp.cu:257-282) in FP64: labels and
+    distances identical to the oracle's, NIL where unreached."""
+    name, xyz, f, src, M, C = case
+    dref, cref, nsw = M.ptp_clusters(src, oracle.GPU_COAL if coal else oracle.GPU_PLAIN)
+    plan = P.Plan(C, src, precision=L.PREC_F64, coalesce=coal, clusters=True)
+    st = plan.run()
+    assert st["sweeps"] == nsw
+    assert _same(plan.result(), dref)
+    assert np.array_equal(plan.clusters(), cref)
+    plan.close()
