@@ -11,6 +11,8 @@
     -- before they did, the FP32 field of this mesh was 6.2e-2 off;
   * tiny meshes: one triangle, a quad, and a mesh whose every vertex is a
     source (no sweep runs);
+  * coordinates far from the origin (3e5 offset) and at a micro scale (1e-6):
+    the FP32 path works on FP64 differences in units of the mean edge;
   * non-manifold input (a bowtie vertex, an edge with three triangles), where
     the half-edge tables follow the reference's (NIL twins and EVT) and so does
     what propagates.
@@ -94,6 +96,9 @@ CASES = [
     ("one_triangle", one_triangle, lambda nv: [0]),
     ("quad", quad, lambda nv: [1]),
     ("all_sources", lambda: G.height_field(8, 35), lambda nv: list(range(nv))),
+    ("far_offset", lambda: (G.height_field(96, 36)[0] + np.array([3.0e5, -2.0e5, 1.0e5]), G.height_field(96, 36)[1]),
+     lambda nv: [4000]),
+    ("micro_scale", lambda: (G.height_field(96, 37)[0] * 1e-6, G.height_field(96, 37)[1]), lambda nv: [5000]),
     ("bowtie", bowtie, lambda nv: [1]),
     ("book3", book3, lambda nv: [2]),
 ]
