@@ -307,18 +307,13 @@ static int batch_configure(gp_batch *b, uint32_t block)
 	b->block = block;
 	if (!graphs_enabled()) return GP_OK;
 	void *kargs[] = { &a };
-	hipGraph_t gr = nullptr;
-	GP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-	k_batch_reset<<<1, 1, 0, b->stream>>>(b->d_queue);
-	const hipError_t le = hipLaunchKernel(fn, dim3(grid), dim3(block), kargs, 0, b->stream);
-	const hipError_t ce = hipStreamEndCapture(b->stream, &gr);
-	if (le != hipSuccess || ce != hipSuccess) {
-		if (gr) (void) hipGraphDestroy(gr);
-		return fail(GP_ERR_HIP, "batch: capture failed: %s", hipGetErrorString(le != hipSuccess ? le : ce));
-	}
-	const hipError_t e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
-	(void) hipGraphDestroy(gr);
-	if (e != hipSuccess) return fail(GP_ERR_HIP, "batch: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	uint32_t *queue = b->d_queue;
+	void *rargs[] = { &queue };
+	GraphChain gc;
+	gc.add((const void *) k_batch_reset, dim3(1), dim3(1), rargs);
+	gc.add(fn, dim3(grid), dim3(block), kargs);
+	const hipError_t e = gc.instantiate(&b->graph);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
 	b->grid = grid;
 	b->block = block;
 	return GP_OK;

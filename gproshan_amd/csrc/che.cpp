@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -180,6 +181,9 @@ int build_che(const double *xyz, uint32_t nv, const uint32_t *faces, uint32_t nf
 
 bool prefer_f64(const Che &m)
 {
+	// The counts are cached in the mesh on first use (several threads may ask).
+	static std::mutex mu;
+	std::lock_guard<std::mutex> lock(mu);
 	if (m.n_flag_tri < 0) {
 		long long nflag = 0, nslot = 0;
 		#pragma omp parallel for schedule(static) reduction(+ : nflag, nslot)

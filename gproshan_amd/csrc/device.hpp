@@ -47,6 +47,43 @@ inline bool graphs_enabled()
 	return on;
 }
 
+// A chain of kernel launches built as a graph with the explicit node API.
+// Stream capture is not used: while one thread captures, HIP fails another
+// thread's legacy-stream calls (hipMemcpy and the like) and that error also
+// spoils the capture, so plans built from several threads would break each
+// other.  A node copies its arguments when it is added.
+struct GraphChain {
+	hipGraph_t g = nullptr;
+	hipGraphNode_t last = nullptr;
+	hipError_t err = hipSuccess;
+	GraphChain() { err = hipGraphCreate(&g, 0); }
+	~GraphChain()
+	{
+		if (g) (void) hipGraphDestroy(g);
+	}
+	GraphChain(const GraphChain &) = delete;
+	GraphChain &operator=(const GraphChain &) = delete;
+	void add(const void *fn, dim3 grid, dim3 block, void **args)
+	{
+		if (err != hipSuccess) return;
+		hipKernelNodeParams kp{};
+		kp.func = const_cast<void *>(fn);
+		kp.gridDim = grid;
+		kp.blockDim = block;
+		kp.sharedMemBytes = 0;
+		kp.kernelParams = args;
+		kp.extra = nullptr;
+		hipGraphNode_t n = nullptr;
+		err = hipGraphAddKernelNode(&n, g, last ? &last : nullptr, last ? 1 : 0, &kp);
+		if (err == hipSuccess) last = n;
+	}
+	hipError_t instantiate(hipGraphExec_t *out)
+	{
+		if (err != hipSuccess) return err;
+		return hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+	}
+};
+
 // Copies a state record of n 32-bit words from device memory into pinned
 // host memory (hipHostMalloc, mapped into the device's address space); the
 // host reads it after an event recorded behind this launch.  The replay loops

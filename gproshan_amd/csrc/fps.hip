@@ -329,16 +329,16 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	                                                               nchunks));
 	constexpr uint32_t BIG = 64, SMALL = 8;
 	auto capture = [&](uint32_t count, hipGraphExec_t *out) -> int {
-		hipGraph_t g = nullptr;
-		GP_HIP(hipStreamBeginCapture(r.stream, hipStreamCaptureModeThreadLocal));
+		FArgs fa = a;
+		const uint32_t k4s[4] = { 0, 1, 2, 3 };
+		const void *fn = ms8 ? (const void *) k_fps_sweep<F64, 8> : (const void *) k_fps_sweep<F64, MAXS>;
+		GraphChain gc;
 		for (uint32_t q = 0; q < count; q++) {
-			if (ms8) k_fps_sweep<F64, 8><<<grid, FB, 0, r.stream>>>(a, q & 3u);
-			else k_fps_sweep<F64, MAXS><<<grid, FB, 0, r.stream>>>(a, q & 3u);
+			void *args[] = { &fa, (void *) &k4s[q & 3u] };
+			gc.add(fn, dim3(grid), dim3(FB), args);
 		}
-		GP_HIP(hipStreamEndCapture(r.stream, &g));
-		const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-		(void) hipGraphDestroy(g);
-		if (e != hipSuccess) return fail(GP_ERR_HIP, "fps: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+		const hipError_t e = gc.instantiate(out);
+		if (e != hipSuccess) return fail(GP_ERR_HIP, "fps graph: %s", hipGetErrorString(e));
 		return GP_OK;
 	};
 	const bool graphs = graphs_enabled();

@@ -338,13 +338,16 @@ int Levels::create(const DeviceMesh *mesh, hipStream_t s)
 	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(1024, (nv + 63) / 64));
 	lv_grid = grid;
 	if (!graphs_enabled()) return GP_OK;
-	hipGraph_t g = nullptr;
-	GP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-	for (uint32_t q = 0; q < LV; q++) k_lv_step<<<grid, 64, 0, stream>>>(dm->fan_off, dm->fan, level, queue, ls, q & 3u);
-	GP_HIP(hipStreamEndCapture(stream, &g));
-	const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-	(void) hipGraphDestroy(g);
-	if (e != hipSuccess) return fail(GP_ERR_HIP, "levels: hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	const uint32_t *fo = dm->fan_off, *fa = dm->fan;
+	const uint32_t k4s[4] = { 0, 1, 2, 3 };
+	GraphChain gc;
+	for (uint32_t q = 0; q < LV; q++) {
+		void *args[] = { (void *) &fo, (void *) &fa, (void *) &level, (void *) &queue, (void *) &ls,
+		                 (void *) &k4s[q & 3u] };
+		gc.add((const void *) k_lv_step, dim3(grid), dim3(64), args);
+	}
+	const hipError_t e = gc.instantiate(&graph);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "levels graph: %s", hipGetErrorString(e));
 	return GP_OK;
 }
 

@@ -496,13 +496,16 @@ int patch_run(PatchPlan *pp, gp_ptp_stats *stats)
 	const PArgs a = patch_args(pp);
 	const bool graph = pp->opts.launch_mode == GP_LAUNCH_GRAPH && graphs_enabled();
 	if (graph && !pp->graph) {
-		hipGraph_t g = nullptr;
-		GP_HIP(hipStreamBeginCapture(pp->stream, hipStreamCaptureModeThreadLocal));
-		enqueue_patch(pp, a, PATCH_GRAPH);
-		GP_HIP(hipStreamEndCapture(pp->stream, &g));
-		const hipError_t e = hipGraphInstantiate(&pp->graph, g, nullptr, nullptr, 0);
-		(void) hipGraphDestroy(g);
-		if (e != hipSuccess) return fail(GP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+		PArgs pa = a;
+		const uint32_t k4s[4] = { 0, 1, 2, 3 };
+		const void *fn = pp->P == 256 ? (const void *) k_patch_sweep<256> : (const void *) k_patch_sweep<512>;
+		GraphChain gc;
+		for (uint32_t q = 0; q < PATCH_GRAPH; q++) {
+			void *args[] = { &pa, (void *) &k4s[q & 3u] };
+			gc.add(fn, dim3(pp->grid), dim3(pp->P), args);
+		}
+		const hipError_t e = gc.instantiate(&pp->graph);
+		if (e != hipSuccess) return fail(GP_ERR_HIP, "patch graph: %s", hipGetErrorString(e));
 	}
 	const uint32_t ig = std::max<uint32_t>(1, std::min<uint32_t>(1024, (pp->n + 255) / 256));
 	k_patch_init<<<ig, 256, 0, pp->stream>>>(pp->d_dist, pp->n, pp->d_mark, 4 * pp->npatch, pp->d_defer, pp->npatch,

@@ -78,9 +78,15 @@ int device_check(char *name, size_t cap)
 	return GP_OK;
 }
 
+// A mesh's device state (upload, vertex-major tables) is built on first use
+// and kept; plans of one mesh may be created from several host threads at
+// once, so that first use is serialised.
+static std::recursive_mutex g_mesh_mu;
+
 // Uploads the fan tables (ring builder) and FP64 positions once per mesh.
 int upload_mesh(Che &m)
 {
+	std::lock_guard<std::recursive_mutex> lock(g_mesh_mu);
 	if (m.dev) return GP_OK;
 	int rc = device_check(nullptr, 0);
 	if (rc) return rc;
@@ -502,6 +508,7 @@ __global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, co
 // precision, kept with the device mesh) and the longest fan's slot count.
 int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 {
+	std::lock_guard<std::recursive_mutex> lock(g_mesh_mu);
 	int rc = upload_mesh(mesh);
 	if (rc) return rc;
 	DeviceMesh *dm = mesh.dev;
@@ -683,29 +690,28 @@ static KArgs make_args(const gp_plan *p)
 	return a;
 }
 
+// The plan's sweep kernel variant.
+static const void *sweep_kernel(const gp_plan *p, bool cl)
+{
+	if (p->opts.precision == GP_PREC_F64) {
+		if (cl) return p->ms8 ? (const void *) k_sweep<true, true, 8> : (const void *) k_sweep<true, true, MAXS>;
+		return p->ms8 ? (const void *) k_sweep<true, false, 8> : (const void *) k_sweep<true, false, MAXS>;
+	}
+	if (p->g64) {
+		if (cl) return p->ms8 ? (const void *) k_sweep<false, true, 8, true> : (const void *) k_sweep<false, true, MAXS, true>;
+		return p->ms8 ? (const void *) k_sweep<false, false, 8, true> : (const void *) k_sweep<false, false, MAXS, true>;
+	}
+	if (cl) return p->ms8 ? (const void *) k_sweep<false, true, 8> : (const void *) k_sweep<false, true, MAXS>;
+	return p->ms8 ? (const void *) k_sweep<false, false, 8> : (const void *) k_sweep<false, false, MAXS>;
+}
+
 // Launches the plan's sweep kernel variant with argument k4 (a sweep slot,
 // or 4 + d for a check sweep from buffer d).
 static void launch_sweep(const gp_plan *p, const KArgs &a, uint32_t k4)
 {
-	{
-		const bool cl = a.cl0 != nullptr;
-		if (p->opts.precision == GP_PREC_F64) {
-			if (cl && p->ms8) k_sweep<true, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else if (cl) k_sweep<true, true, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else if (p->ms8) k_sweep<true, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else k_sweep<true, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-		} else {
-			if (p->g64) {
-				if (cl && p->ms8) k_sweep<false, true, 8, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-				else if (cl) k_sweep<false, true, MAXS, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-				else if (p->ms8) k_sweep<false, false, 8, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-				else k_sweep<false, false, MAXS, true><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			} else if (cl && p->ms8) k_sweep<false, true, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else if (cl) k_sweep<false, true, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else if (p->ms8) k_sweep<false, false, 8><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-			else k_sweep<false, false, MAXS><<<p->grid, p->block, 0, p->stream>>>(a, k4);
-		}
-	}
+	KArgs ka = a;
+	void *args[] = { &ka, &k4 };
+	(void) hipLaunchKernel(sweep_kernel(p, a.cl0 != nullptr), dim3(p->grid), dim3(p->block), args, 0, p->stream);
 }
 
 // Enqueues sweeps k0 .. k0+count-1 (their k & 3 slots).
@@ -723,13 +729,16 @@ static uint32_t kernel_bits(const gp_plan *p)
 
 static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *out)
 {
-	hipGraph_t g = nullptr;
-	GP_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-	enqueue_sweeps(p, a, 0, count);
-	GP_HIP(hipStreamEndCapture(p->stream, &g));
-	const hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-	(void) hipGraphDestroy(g);
-	if (e != hipSuccess) return fail(GP_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+	KArgs ka = a;
+	const uint32_t k4s[4] = { 0, 1, 2, 3 };
+	const void *fn = sweep_kernel(p, a.cl0 != nullptr);
+	GraphChain gc;
+	for (uint32_t q = 0; q < count; q++) {
+		void *args[] = { &ka, (void *) &k4s[q & 3u] };
+		gc.add(fn, dim3(p->grid), dim3(p->block), args);
+	}
+	const hipError_t e = gc.instantiate(out);
+	if (e != hipSuccess) return fail(GP_ERR_HIP, "sweep graph: %s", hipGetErrorString(e));
 	return GP_OK;
 }
 
