@@ -1,5 +1,5 @@
 // Batched PTP: many (mesh, source) fields of uneven size in one persistent
-// launch (captured in a hipGraph with its queue reset).
+// launch (a hipGraph of its queue reset and the kernel).
 //
 // Each 1024-thread workgroup repeatedly takes the next job from an atomic
 // queue (jobs sorted by decreasing cost V^1.5, so the grid drains evenly), and

@@ -34,7 +34,7 @@ struct DeviceMesh {
 int upload_mesh(Che &m);
 
 // Replayed launch sequences (sweeps, BFS levels, patch launches, the batch)
-// run from captured hipGraphs unless the environment sets
+// run from hipGraphs unless the environment sets
 // GPROSHAN_AMD_GRAPHS=0: then the same kernels are launched one by one on the
 // stream, with the same results (a profiler that cannot follow graph replays
 // traces these).

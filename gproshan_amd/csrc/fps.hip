@@ -12,7 +12,7 @@
 // Rings: the BFS levels from the sample set are kept from round to round; a
 // new sample only lowers the levels of the vertices nearer to it, so each
 // round runs a frontier BFS from the new sample alone (one launch per level,
-// all workgroups, replayed from captured graphs) that stops where the old
+// all workgroups, replayed from graphs) that stops where the old
 // levels are already as low, then a counting sort by level rebuilds sorted[]
 // and limits.  The order inside a ring differs from the reference's discovery
 // order; a sweep relaxes a whole window of rings from the previous buffer, so

@@ -1,7 +1,7 @@
 // Multi-source BFS levels on the device and the vertices sorted by level (the
 // rings of parallel_toplesets_propagation, che::compute_toplesets,
 // src/che.cpp:546-593, as sets).  A frontier BFS over all workgroups (one
-// launch per level, replayed from a captured graph, the host polling a done
+// launch per level, replayed from a graph, the host polling a done
 // flag) lowers the levels from the seeded sources, stopping where the levels
 // are already as low -- so seeding one more source into existing levels
 // updates them incrementally -- and a counting sort by level and, inside a

@@ -23,7 +23,7 @@
 //
 // Lists rotate over four slots (launch k reads list k&3, appends to
 // (k+1)&3, clears the count and key minimum of (k+2)&3, which launch k-2
-// consumed), so the launches can be replayed from captured graphs.  The
+// consumed), so the launches can be replayed from graphs.  The
 // reference has no counterpart: it runs the ring window schedule only
 // (src/cuda/geodesics_ptp.cu:198-231); results are compared with the fixed
 // point of the same relaxation.

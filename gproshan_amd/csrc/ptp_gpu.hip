@@ -12,7 +12,7 @@
 // in device memory: a sweep kernel relaxes the window it reads from there and
 // counts non-converged ring-i vertices (wave reductions, one atomic per wave
 // with a non-zero count), and a one-thread kernel advances the schedule.  The
-// pairs are captured once into a hipGraph and replayed until the device sets
+// pairs are built once into a hipGraph and replayed until the device sets
 // its done flag; the host polls that flag once per replay.  (A persistent
 // kernel with a grid barrier per sweep measured slower on MI355X: the
 // agent-scope fence each workgroup needs to publish its writes across the
@@ -233,7 +233,7 @@ struct gp_plan {
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = { nullptr, nullptr };
 	hipGraphExec_t g_big = nullptr, g_small = nullptr;
-	uint32_t big = 64, small = 8;            // sweeps per captured graph (multiples of 4)
+	uint32_t big = 64, small = 8;            // sweeps per graph replay (multiples of 4)
 	uint32_t xblk = 0;                       // log2 of the chunk blocks dealt to XCDs (0: eighths)
 	bool has_run = false;
 	bool capped = false;                     // the last run stopped at its sweep cap
@@ -1055,7 +1055,7 @@ int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const doubl
 
 // Autotuner of a plan's launch shape: the sweep kernel's workgroups (a
 // quarter to all of those that fit on the device at once) and the sweeps per
-// captured graph replay, timed on the plan itself (one warm-up run, then the
+// graph replay, timed on the plan itself (one warm-up run, then the
 // mean device time of two) and remembered per plan shape, so that a later plan
 // of the same shape takes the choice without timing.
 namespace {

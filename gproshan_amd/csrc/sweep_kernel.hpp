@@ -23,7 +23,7 @@ namespace gp {
 // for the next sweep, and relaxes its own window counting into bad[k & 3].
 // No workgroup reads a slot or counter while another workgroup of the same
 // launch writes it, so kernel boundaries are the only synchronisation.  The
-// captured graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
+// graphs hold a multiple of 4 sweeps so k & 3 stays continuous.
 //
 // A slot also carries its sweep's window in positions (ws, wc, we: start, end
 // of the convergence-tested ring, end) and, written at the end of the sweep,

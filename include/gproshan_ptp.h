@@ -87,7 +87,7 @@ enum gp_schedule {
  * ring tests and arithmetic, hence the same bits and sweep counts; other
  * plans run GP_LAUNCH_GRAPH (gp_ptp_stats.kernel tells which ran). */
 enum gp_launch_mode {
-	GP_LAUNCH_GRAPH = 0,   /* sweep + schedule kernels replayed from captured hipGraphs (default) */
+	GP_LAUNCH_GRAPH = 0,   /* sweep + schedule kernels replayed from hipGraphs (default)          */
 	GP_LAUNCH_STREAM = 1,  /* the same kernels launched one by one on the stream                 */
 	GP_LAUNCH_FLOW = 2     /* one dataflow launch for the whole schedule                          */
 };
