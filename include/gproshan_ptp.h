@@ -341,7 +341,7 @@ void gp_plan_destroy(gp_plan *plan);
 /* Launch shape of a ring or band plan's sweeps. */
 typedef struct gp_tune {
 	uint32_t grid;        /* workgroups of the sweep kernel (one wave each)            */
-	uint32_t replay;      /* sweeps per replay of the long captured graph              */
+	uint32_t replay;      /* sweeps per replay of the long graph                       */
 	double kernel_ms;     /* device time per field of the chosen shape                 */
 	uint32_t candidates;  /* shapes timed (0: the choice came from the cache)          */
 	uint32_t xcd_block;   /* log2 of the chunk blocks dealt round-robin to the eight
