@@ -266,6 +266,21 @@ class Plan:
         obj._h = h
         return obj, wout.value
 
+    @classmethod
+    def learned(cls, mesh, sources, mlp, **kw):
+        """Learned-band plan in one call (gp_plan_create_learned): the network's
+        prediction (features and forward on the device) ordered into bands."""
+        obj = cls.__new__(cls)
+        obj.mesh = mesh
+        obj._src = _u32(sources)
+        o = L.default_opts(**{k: (int(v) if k in ("coalesce", "uncapped", "clusters", "device_setup", "fp32_geometry",
+                                                  "band_check") else v) for k, v in kw.items()})
+        h = C.c_void_p()
+        check(lib().gp_plan_create_learned(mesh.handle, ptr(obj._src), len(obj._src), mlp.handle, C.byref(o),
+                                           C.byref(h)))
+        obj._h = h
+        return obj
+
     def autotune(self):
         """gp_plan_autotune: the plan's launch shape (grid, replay) timed and kept."""
         t = L.Tune()

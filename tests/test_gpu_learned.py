@@ -80,3 +80,27 @@ def test_learned_bands_reach_fixed_point(case, net):
         print(f"[learned] {name} {tag}: sweeps {st['sweeps']} check sweeps {st['check_sweeps']} relaxations "
               f"{st['relaxations']} max err {mx:.2e} kernel {st['kernel_ms']:.2f} ms", flush=True)
         assert mx <= 1e-4 and st["capped"] == 0
+
+
+@pytest.mark.parametrize("device_setup", [0, 1])
+def test_one_call_learned_plan(case, net, device_setup):
+    """gp_plan_create_learned (prediction and band plan in one C call) gives
+    the field of the two-step path (gp_mlp_predict, then gp_plan_create_bands
+    with the learned schedule), bit for bit, within 1e-4 of the fixed point."""
+    name, src, C, fp = case
+    kw = dict(band_width=0.5, tol=1e-6, device_setup=device_setup)
+    one = P.Plan.learned(C, src, net, **kw)
+    st1 = one.run()
+    d1 = one.result()
+    one.close()
+    dhat, _ = net.predict(C, src)
+    two = P.Plan(C, src, dhat=dhat, schedule=L.SCHED_LEARNED_BAND, **kw)
+    st2 = two.run()
+    d2 = two.result()
+    two.close()
+    mx, _ = rel_err(d1, fp)
+    print(f"[learned one call] {name} device_setup={device_setup}: sweeps {st1['sweeps']}+{st1['check_sweeps']} "
+          f"max err {mx:.2e}", flush=True)
+    assert st1["sweeps"] == st2["sweeps"] and st1["check_sweeps"] == st2["check_sweeps"]
+    assert np.array_equal(d1, d2)
+    assert mx <= 1e-4
