@@ -16,7 +16,7 @@
 //  * distances are (value, tag) pairs, tag = 1 + the sweep that wrote them;
 //  * cflag[c] = k + 1 once chunk c's writes of sweep k have completed, FROZEN
 //    once the window start has passed the chunk;
-//  * hist[k] = the window start i'_k of sweep k (| DONE, CAPPED), published in
+//  * hist[k] = the window start i'_k of sweep k (| DONE), published in
 //    order (pub = states published) by the last chunk of sweep k-1's tested
 //    ring to finish, which also adds that sweep's statistics.
 //
@@ -47,8 +47,7 @@
 namespace gp {
 
 constexpr uint32_t FLOW_DONE = 0x80000000u;     // hist: the loop ended with this state
-constexpr uint32_t FLOW_CAPPED = 0x40000000u;   // hist: ... at its sweep cap, ring i still below j
-constexpr uint32_t FLOW_IMASK = 0x3FFFFFFFu;
+constexpr uint32_t FLOW_IMASK = 0x7FFFFFFFu;
 constexpr uint32_t FLOW_FROZEN = 0xFFFFFFFFu;   // cflag of a chunk below the window start
 constexpr uint32_t FLOW_NEVER = 0xFFFFFFFFu;
 constexpr uint32_t FLOW_MAX_OWN = 32;           // chunks per wave
@@ -256,7 +255,7 @@ __device__ __forceinline__ void flow_publish(const FlowArgs &f, uint32_t k, uint
 		st->final_d = iter & 1u;
 		st->done = 1;
 	}
-	st_agent(f.hist + iter, ipn | (done ? FLOW_DONE : 0u) | (done && in < jn ? FLOW_CAPPED : 0u));
+	st_agent(f.hist + iter, ipn | (done ? FLOW_DONE : 0u));
 	flow_wait_stores();
 	st_agent(&f.fs->pub, iter + 1u);
 }

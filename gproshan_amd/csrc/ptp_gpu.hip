@@ -767,7 +767,7 @@ static int flow_prepare(gp_plan *p, bool &ok)
 	GP_HIP(hipMemcpy(lim.data(), p->d_limits, 4ull * p->nl, hipMemcpyDeviceToHost));
 	for (uint32_t q = 1; q < p->nl; q++)
 		if (lim[q] <= lim[q - 1]) return GP_OK;
-	if (lim[p->nl - 1] != p->nr || p->max_iter >= FLOW_IMASK) return GP_OK;
+	if (lim[p->nl - 1] != p->nr || p->nl >= FLOW_IMASK) return GP_OK;
 	int dev = 0, ncu = 0, occ = 0;
 	GP_HIP(hipGetDevice(&dev));
 	GP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
