@@ -268,6 +268,26 @@ def main():
             e2e[tag] = {"ms_min": round(walls[k], 2), "predict_ms": round(parts[k][0], 2),
                         "plan_ms": round(parts[k][1], 2), "kernel_ms": round(parts[k][2], 4)}
             log(f"[bench] end to end, {tag}: {e2e[tag]}")
+        # The same in one call (gp_plan_create_learned with device_setup): one
+        # BFS feeds the features and the plan, the prediction stays on the GPU.
+        if net is not None:
+            walls, parts = [], []
+            for _ in range(3):
+                C = P.che(xyz, faces)
+                t0 = time.perf_counter()
+                plan = P.Plan.learned(C, src, net, device_setup=True, **band_kw)
+                plan.autotune()
+                t1 = time.perf_counter()
+                stx = plan.run()
+                plan.result()
+                plan.close()
+                walls.append((time.perf_counter() - t0) * 1e3)
+                parts.append(((t1 - t0) * 1e3, stx["kernel_ms"]))
+                del C
+            k = int(np.argmin(walls))
+            e2e["band_learned_one_call"] = {"ms_min": round(walls[k], 2), "predict_and_plan_ms": round(parts[k][0], 2),
+                                            "kernel_ms": round(parts[k][1], 4)}
+            log(f"[bench] end to end, band_learned_one_call: {e2e['band_learned_one_call']}")
     # The 16-source field of the same mesh (one multi-source field).
     src16 = W.c2(args.seed, 16, args.n)[2]
     multi = {}

@@ -555,12 +555,25 @@ int gp_plan_create_learned(gp_che *mesh, const uint32_t *sources, uint32_t n_sou
 {
 	GP_GUARD(
 		if (!mlp || !mesh || !sources || !n_sources || !out) return gp::fail(GP_ERR_INVALID, "null argument");
-		std::vector<double> dhat(mesh->m.nv);
-		int rc = gp_mlp_predict(mlp, mesh, sources, n_sources, dhat.data(), nullptr);
-		if (rc) return rc;
 		gp_ptp_opts o;
 		if (opts) o = *opts; else gp_ptp_opts_default(&o);
 		o.schedule = GP_SCHED_LEARNED_BAND;
+		if (o.fp32_geometry != GP_GEO_AUTO && o.fp32_geometry != GP_GEO_FP32 && o.fp32_geometry != GP_GEO_FP64)
+			return gp::fail(GP_ERR_INVALID, "unknown fp32_geometry %d", o.fp32_geometry);
+		if (o.device_setup && !o.clusters && mesh->m.manifold) {
+			// Everything on the device: one BFS serves the features' ring
+			// index and the plan's reached set; dhat never leaves the GPU.
+			gp_ptp_opts ob = o;
+			ob.coalesce = 1;
+			ob.uncapped = 1;
+			const double width = (o.band_width > 0 ? o.band_width : 1.0) * mesh->m.mean_edge;
+			const int rd = gp::plan_create_device(&mesh->m, sources, n_sources, ob, out, nullptr, width, mlp);
+			if (rd != GP_ERR_CAPACITY) return rd;
+			gp::clear_error();   // a fan too long or a band index too large: build on the host
+		}
+		std::vector<double> dhat(mesh->m.nv);
+		int rc = gp_mlp_predict(mlp, mesh, sources, n_sources, dhat.data(), nullptr);
+		if (rc) return rc;
 		return plan_create_any(mesh, sources, n_sources, dhat.data(), o, out);
 	)
 }

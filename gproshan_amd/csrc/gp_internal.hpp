@@ -166,8 +166,10 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
                       const uint32_t *sorted, const gp_ptp_opts &opts, gp_plan **out);
 // Ring plan built on the device; with dhat, the rings are the nonempty bands
 // of width `width` of dhat (the sources first), as the host band builder.
+// With mlp (and no dhat), dhat is the network's prediction, computed on the
+// device from the plan's own BFS levels (gp_plan_create_learned).
 int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out,
-                       const double *dhat = nullptr, double width = 0);
+                       const double *dhat = nullptr, double width = 0, gp_mlp *mlp = nullptr);
 // BFS levels of the sources computed on the device (NIL where unreached).
 int levels_gpu(Che &m, const uint32_t *src, uint32_t ns, uint32_t *levels);
 int plan_run(gp_plan *p, gp_ptp_stats *stats);
@@ -217,5 +219,7 @@ int mlp_train(gp_mlp *m, const float *F, const float *t, uint32_t n, uint32_t st
               uint32_t seed, float *grad_out, float *loss_out);
 int mlp_predict(gp_mlp *m, Che &mesh, const uint32_t *src, uint32_t ns, const uint32_t *levels, double *dhat,
                 float *ms_out);
+int mlp_predict_device(gp_mlp *m, Che &mesh, const uint32_t *d_src, uint32_t ns, const uint32_t *d_levels,
+                       uint32_t lmax, double *d_dhat, void *stream);
 
 }  // namespace gp

@@ -585,6 +585,31 @@ int mlp_predict(gp_mlp *m, Che &mesh, const uint32_t *src, uint32_t ns, const ui
 	return GP_OK;
 }
 
+// The same prediction on device-resident inputs: the sources and their BFS
+// levels in device memory (lmax: the largest level), dhat written to device
+// memory, every kernel on `stream` -- the same kernels and inputs as
+// mlp_predict, hence the same dhat, without the host round trips.
+int mlp_predict_device(gp_mlp *m, Che &mesh, const uint32_t *d_src, uint32_t ns, const uint32_t *d_levels,
+                       uint32_t lmax, double *d_dhat, void *stream_)
+{
+	const hipStream_t stream = (hipStream_t) stream_;
+	int rc = upload_mesh(mesh);
+	if (rc) return rc;
+	DevBuf X, e, y;
+	const uint32_t nv = mesh.nv;
+	if ((rc = X.alloc(2ull * KP * nv)) || (rc = e.alloc(4ull * nv)) || (rc = y.alloc(4ull * nv))) return rc;
+	FeatArgs a{ mesh.dev->fan_off, mesh.dev->fan, mesh.dev->closed, mesh.dev->pos, d_src, ns, d_levels, lmax, nv,
+	            mesh.mean_edge, (__bf16 *) X.p, nullptr, (float *) e.p };
+	k_features<<<(nv + 255) / 256, 256, 0, stream>>>(a);
+	const uint32_t ntile = (nv + 15) / 16;
+	const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(2048, (ntile + 3) / 4));
+	k_mlp_forward<<<grid, 256, 0, stream>>>((const __bf16 *) X.p, nv, m->d.w, m->d.w1, m->d.w2, (float *) y.p);
+	k_dhat<<<(nv + 255) / 256, 256, 0, stream>>>((const float *) y.p, (const float *) e.p, nv, d_dhat);
+	GP_HIP(hipGetLastError());
+	GP_HIP(hipStreamSynchronize(stream));   // before the scratch is freed
+	return GP_OK;
+}
+
 // Forward on caller features F[n][10]: y[n].
 int mlp_forward(gp_mlp *m, const float *F, uint32_t n, float *y)
 {

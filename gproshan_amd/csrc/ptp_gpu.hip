@@ -552,8 +552,18 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 // count.  GP_ERR_CAPACITY when a fan needs
 // more register-held slots than the sweep kernels hold (the caller then takes
 // the host-built path).
+// Largest BFS level (NIL excluded) into *out (zeroed beforehand).
+__global__ void k_max_level(const uint32_t *level, uint32_t n, uint32_t *out)
+{
+	uint32_t m = 0;
+	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+		if (level[v] != GP_NIL) m = max(m, level[v]);
+	for (uint32_t o = 32; o > 0; o >>= 1) m = max(m, (uint32_t) __shfl_xor((int) m, (int) o));
+	if ((threadIdx.x & 63u) == 0 && m) atomicMax(out, m);
+}
+
 int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp_opts &opts, gp_plan **out,
-                       const double *dhat, double width)
+                       const double *dhat, double width, gp_mlp *mlp)
 {
 	*out = nullptr;
 	if (!mesh || !src) return fail(GP_ERR_INVALID, "plan: null argument");
@@ -597,17 +607,31 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	lv.clear();
 	lv.seed(d_srcv, 0, ns);
 	if ((rc = lv.expand())) return rc;
-	if (dhat) {
-		// Bands of dhat in place of the BFS levels of the reached vertices.
+	if (dhat || mlp) {
+		// Bands of dhat in place of the BFS levels of the reached vertices:
+		// the caller's dhat, or the network's prediction from these levels
+		// (the features' ring index), kept on the device.
 		if (!(width > 0)) return fail(GP_ERR_INVALID, "band width must be positive");
-		std::vector<double> dh(dhat, dhat + mesh->nv);
 		double *d_dhat = nullptr;
 		uint32_t *d_bmax = nullptr;
-		if ((rc = upload(d_dhat, dh))) return rc;
+		if (dhat) {
+			std::vector<double> dh(dhat, dhat + mesh->nv);
+			if ((rc = upload(d_dhat, dh))) return rc;
+		} else if (hipMalloc(&d_dhat, sizeof(double) * mesh->nv) != hipSuccess) {
+			return fail(GP_ERR_NOMEM, "plan: device allocation failed");
+		}
 		std::unique_ptr<double, hipError_t (*)(void *)> dhat_guard(d_dhat, hipFree);
 		if (hipMalloc(&d_bmax, 4) != hipSuccess) return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 		std::unique_ptr<uint32_t, hipError_t (*)(void *)> bmax_guard(d_bmax, hipFree);
 		const uint32_t gb = std::max<uint32_t>(1, std::min<uint32_t>(2048, (mesh->nv + 255) / 256));
+		if (!dhat) {
+			k_fill_u32<<<1, 64, 0, p->stream>>>(d_bmax, 1, 0);
+			k_max_level<<<gb, 256, 0, p->stream>>>(lv.level, mesh->nv, d_bmax);
+			uint32_t lmax = 0;
+			GP_HIP(hipMemcpyAsync(&lmax, d_bmax, 4, hipMemcpyDeviceToHost, p->stream));
+			GP_HIP(hipStreamSynchronize(p->stream));
+			if ((rc = mlp_predict_device(mlp, *mesh, d_srcv, ns, lv.level, lmax, d_dhat, p->stream))) return rc;
+		}
 		k_fill_u32<<<1, 64, 0, p->stream>>>(d_bmax, 1, 0);
 		k_band_levels<<<gb, 256, 0, p->stream>>>(lv.level, d_dhat, mesh->nv, width, d_bmax);
 		GP_HIP(hipGetLastError());
