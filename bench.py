@@ -108,6 +108,8 @@ def main():
                     help="only run a device-to-device copy of 256 MiB five times (to scale FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-c3c5", action="store_true",
                     help="leave out the batched (C3) and precision-stress (C5) results of the default line")
+    ap.add_argument("--no-next", action="store_true", help="leave the rows around the path (FPS, clusters, "
+                    "iteration-error loop, readers) out of the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -421,10 +423,12 @@ def main():
     if not args.no_c3c5 and not only:
         out["c3"] = batch(args, L, P, G, steps=3, warmup=1, emit=False)
         out["c5"] = stress(args, L, P, G, steps=3, warmup=1, emit=False)
+    if not args.no_next and not only:
+        out["next"] = next_rows(args, L, P, G, emit=False)
     print(json.dumps(out), flush=True)
 
 
-def next_rows(args, L, P, G):
+def next_rows(args, L, P, G, emit=True):
     """The rows around the hot path on the default height field: farthest-point
     sampling (32 samples from one seed, FP32 and FP64; the FP64 oracle
     restatement's first rounds timed on the host cores and compared sample for
@@ -521,7 +525,9 @@ def next_rows(args, L, P, G):
            "config": {"workload": f"height field {args.n}x{args.n} ({nv} vertices), seed {args.seed}; FPS from "
                                   f"{src.tolist()}, 32 samples; clusters with 16 sources"},
            "rows": rows}
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def stress(args, L, P, G, steps=None, warmup=None, emit=True):
