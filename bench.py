@@ -49,10 +49,11 @@ ERR_BAR = 1e-4
 # (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
 # profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources):
 # (bytes per dispatch, how they were measured).
-MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((13049.30 * 2.0 + 766.30 * 1.0) * 1024,
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((8300.34 * 2.0 + 766.29 * 1.0) * 1024,
                                                 "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per dispatch, "
-                                                "mean of 6,640 dispatches of band_learned (0.5-edge bands, contiguous "
-                                                "XCD eighths), kernels launched one by one under the profiler")}
+                                                "mean of 6,640 dispatches of band_learned (0.5-edge bands, 8-byte "
+                                                "FP32 slot geometry, contiguous XCD eighths), kernels launched one "
+                                                "by one under the profiler")}
 
 
 def log(*a):

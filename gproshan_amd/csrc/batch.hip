@@ -37,7 +37,7 @@ struct BMesh {
 	const uint32_t *fan_off, *fan;
 	const uint8_t *closed;
 	const uint32_t *coff, *cid;
-	const float4 *cgeo;
+	const float2 *cgeo;
 	const double4 *cgeo64;
 	const Edge64 *cx64;
 	const uint32_t *perm;   // Morton position -> original vertex id
@@ -63,14 +63,14 @@ struct BArgs {
 	// Per-workgroup field tables (fslots slots each): ids in positions, FP32
 	// geometry, the slot's index in the mesh's tables; inv: vertex -> position.
 	uint32_t *fcid, *fmidx, *finv;
-	float4 *fgeo;
+	float2 *fgeo;
 	size_t fslots;
 };
 
 // The field's tables as relax_lane reads them.
 struct FieldTables {
 	const uint32_t *cid;
-	const float4 *cgeo;
+	const float2 *cgeo;
 	GeoRemap64 cgeo64;
 	const Edge64 *cx64;
 	float h, inv_h;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 	unsigned long long *key = a.key + wg * mv;
 	float *d0 = a.dist + 2 * wg * mv, *d1 = d0 + mv;
 	uint32_t *fcid = a.fcid + wg * a.fslots, *fmidx = a.fmidx + wg * a.fslots, *finv = a.finv + wg * mv;
-	float4 *fgeo = a.fgeo + wg * a.fslots;
+	float2 *fgeo = a.fgeo + wg * a.fslots;
 	const uint32_t lane = tid & 63u, wave = tid >> 6, nwave = NT >> 6;
 	for (;;) {
 		if (tid == 0) s_job = atomicAdd(a.queue, 1u);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 					fmidx[at] = mb + k;
 				} else {
 					fcid[at] = q | SLOT_EMPTY;
-					fgeo[at] = make_float4(0.f, 0.f, 0.f, 0.f);
+					fgeo[at] = make_float2(0.f, 0.f);
 					fmidx[at] = 0;
 				}
 			}
@@ -192,7 +192,7 @@ struct gp_batch {
 		std::unique_ptr<gp::Che> ren;   // Morton-numbered copy of the mesh
 		std::vector<uint32_t> inv;      // original -> Morton numbering
 		uint32_t *coff = nullptr, *cid = nullptr, *perm = nullptr;
-		float4 *cgeo = nullptr;
+		float2 *cgeo = nullptr;
 		double4 *cgeo64 = nullptr;
 	};
 	std::vector<MeshTables> tabs;
@@ -208,7 +208,7 @@ struct gp_batch {
 	unsigned long long *d_key = nullptr;
 	float *d_dist = nullptr;
 	uint32_t *d_fcid = nullptr, *d_fmidx = nullptr, *d_finv = nullptr;
-	float4 *d_fgeo = nullptr;
+	float2 *d_fgeo = nullptr;
 	uint32_t njobs = 0, max_nv = 0, max_slots = 0, grid = 0, block = 0;
 	unsigned long long total_out = 0;
 	double tol = GP_PTP_TOL;
@@ -285,7 +285,7 @@ static int batch_configure(gp_batch *b, uint32_t block)
 	const size_t mv = b->max_nv;
 	// Field tables: the largest field's chunks at the longest fan's width.
 	const size_t fslots = ((mv + 63) / 64) * 64 * (size_t) b->max_slots;
-	const size_t per_wg = 4 * mv * 3 + 4 * (mv + 2) + 8 * mv + 8 * mv + fslots * (4 + 4 + 16);
+	const size_t per_wg = 4 * mv * 3 + 4 * (mv + 2) + 8 * mv + 8 * mv + fslots * (4 + 4 + sizeof(float2));
 	// At most 32 GiB of scratch: a wider grid than that would take no more
 	// fields at once than the workgroups that fit.
 	const size_t cap = std::max<size_t>(1, (32ull << 30) / std::max<size_t>(per_wg, 1));
@@ -295,7 +295,7 @@ static int batch_configure(gp_batch *b, uint32_t block)
 	    hipMalloc(&b->d_limits, 4 * (mv + 2) * g) != hipSuccess || hipMalloc(&b->d_nlim, 4 * g) != hipSuccess ||
 	    hipMalloc(&b->d_key, 8 * mv * g) != hipSuccess || hipMalloc(&b->d_dist, 8 * mv * g) != hipSuccess ||
 	    hipMalloc(&b->d_finv, 4 * mv * g) != hipSuccess || hipMalloc(&b->d_fcid, 4 * fslots * g) != hipSuccess ||
-	    hipMalloc(&b->d_fmidx, 4 * fslots * g) != hipSuccess || hipMalloc(&b->d_fgeo, 16 * fslots * g) != hipSuccess) {
+	    hipMalloc(&b->d_fmidx, 4 * fslots * g) != hipSuccess || hipMalloc(&b->d_fgeo, sizeof(float2) * fslots * g) != hipSuccess) {
 		b->free_scratch();
 		return fail(GP_ERR_NOMEM, "batch: scratch allocation failed (%u workgroups x %u vertices)", grid, b->max_nv);
 	}
@@ -370,7 +370,7 @@ int batch_create(Che *const *meshes, uint32_t nmeshes, const uint32_t *job_mesh,
 		Layout L;
 		if ((rc = build_layout(m, nullptr, m.nv, false, L, 0, 1, 0, true))) return rc;
 		if ((rc = up(t.coff, L.coff.data(), L.coff.size())) || (rc = up(t.cid, L.cid.data(), L.cid.size())) ||
-		    (rc = up(t.cgeo, reinterpret_cast<const float4 *>(L.cgeo.data()), L.cgeo.size())) ||
+		    (rc = up(t.cgeo, reinterpret_cast<const float2 *>(L.cgeo.data()), L.cgeo.size())) ||
 		    (rc = up(t.cgeo64, reinterpret_cast<const double4 *>(L.cgeo64.data()), L.cgeo64.size())) ||
 		    (rc = up(t.perm, order.data(), order.size())))
 			return rc;

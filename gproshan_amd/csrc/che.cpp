@@ -253,7 +253,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 		L.cx64.assign(std::max<size_t>(lanes, 1), Edge64{0, 0, 0});
 		L.cgeo.clear(); L.cgeo64.clear();
 	} else {
-		L.cgeo.assign(std::max<size_t>(lanes, 1), Geo32{0.f, 0.f, 0.f, 0.f});
+		L.cgeo.assign(std::max<size_t>(lanes, 1), Geo32{0.f, 0.f});
 		L.cgeo64.assign(std::max<size_t>(lanes, 1), Geo64{0, 0, 0, 0});
 		L.cx64.clear();
 	}
@@ -308,7 +308,7 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 						nflag++;
 					}
 				}
-				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w };
+				L.cgeo[at] = Geo32{ (float) gd.x, (float) gd.y };
 				L.cgeo64[at] = gd;
 			}
 			if (k == 0 && ns < n) id |= SLOT_WRAP;

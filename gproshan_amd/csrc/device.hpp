@@ -16,7 +16,7 @@ namespace gp {
 			return fail(GP_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
 	} while (0)
 
-static_assert(sizeof(Geo32) == sizeof(float4), "Geo32 must match float4");
+static_assert(sizeof(Geo32) == sizeof(float2), "Geo32 must match float2");
 
 struct MasterLayout;  // ptp_gpu.hip
 

@@ -54,7 +54,7 @@ struct FState {
 // ring-ordered chunks (positions), read by relax_lane.
 struct FTables {
 	const uint32_t *cid;
-	const float4 *cgeo;
+	const float2 *cgeo;
 	GeoRemap64 cgeo64;
 	const Edge64 *cx64;
 	float h, inv_h;
@@ -238,7 +238,7 @@ __global__ void k_fps_append(BfsJob *job, uint32_t *samples, FState *st)
 
 struct FpsRun {
 	uint32_t *cid = nullptr, *midx = nullptr, *inv = nullptr, *samples = nullptr;
-	float4 *cgeo = nullptr;
+	float2 *cgeo = nullptr;
 	Edge64 *cx64 = nullptr;
 	void *d0 = nullptr, *d1 = nullptr;
 	FState *st = nullptr, *h_st = nullptr;   // h_st: the round's snapshot, then two sweep snapshots
@@ -296,7 +296,7 @@ static int fps_impl(Che &m, uint32_t *samples, uint32_t n_init, uint32_t cap, ui
 	const size_t lanes = (size_t) nchunks * S * 64;
 	if (hipMalloc(&r.cid, 4 * lanes) != hipSuccess || hipMalloc(&r.inv, 4ull * nv) != hipSuccess ||
 	    (F64 ? hipMalloc(&r.cx64, sizeof(Edge64) * lanes)
-	         : (hipMalloc(&r.cgeo, sizeof(float4) * lanes) == hipSuccess ? hipMalloc(&r.midx, 4 * lanes)
+	         : (hipMalloc(&r.cgeo, sizeof(float2) * lanes) == hipSuccess ? hipMalloc(&r.midx, 4 * lanes)
 	                                                                      : hipErrorOutOfMemory)) != hipSuccess)
 		return fail(GP_ERR_NOMEM, "fps: table allocation failed");
 	if ((rc = fup(r.samples, samples, cap))) return rc;

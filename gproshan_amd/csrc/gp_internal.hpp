@@ -58,12 +58,14 @@ constexpr uint32_t ID_MASK = 0x7FFFFFFFu;
 constexpr uint32_t F64_FLAG = 0x80000000u;
 
 // FP32 geometry of fan entry k of a vertex, in units of the mean edge length h:
-// x = |X_k|^2, y = X_k.X_{k+1}, z = |X_k x X_{k+1}|^2, w = |X_{k+1} - X_k|^2
-// (y, z, w describe triangle (k, k+1) and are 0 on the last entry of an open
-// fan; entry k + 1 is entry 0 after the last entry of a closed fan stored
-// without its repeat).
-struct Geo32 { float x, y, z, w; };
-struct Geo64 { double x, y, z, w; };   // the same in FP64 (FP64-fallback lanes)
+// x = |X_k|^2, y = X_k.X_{k+1} (y describes triangle (k, k+1) and is 0 on the
+// last entry of an open fan; entry k + 1 is entry 0 after the last entry of a
+// closed fan stored without its repeat).  The FP32 update forms the
+// triangle's det and S from x, y and the next entry's x (update_stable_q).
+struct Geo32 { float x, y; };
+// FP64 geometry of the same entry: x, y as above, z = |X_k x X_{k+1}|^2,
+// w = |X_{k+1} - X_k|^2 (FP64-fallback triangles).
+struct Geo64 { double x, y, z, w; };
 struct Edge64 { double x, y, z; };     // FP64 edge vector a - v of a fan entry
 
 // FP64 update for triangle (k, k+1) of a fan when bit 31 of entry k is set:

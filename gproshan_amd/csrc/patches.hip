@@ -52,7 +52,7 @@ struct PatchState {
 struct PArgs {
 	const uint32_t *coff, *cid;
 	const uint16_t *lidx;
-	const float4 *cgeo;
+	const float2 *cgeo;
 	const double4 *cgeo64;
 	const Edge64 *cx64;    // unused (FP32 only); keeps relax helpers' interface
 	const float *pkey;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 		const float v0 = live ? a.dist[pos] : (float) INFINITY;
 		const bool fast = S <= MAXS;
 		uint32_t pk[MAXS];
-		float4 g[MAXS];
+		float2 g[MAXS];
 		float tout[MAXS];
 		uint32_t fks = 0xFFFu, nf = 0;       // cached FP64 slots, 4 bits each (0xF: none)
 #pragma unroll
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 						const uint32_t lb = pk[k + 1] & 0xFFFFu;
 						const float tb = lb != LOCAL_OUT ? sd[b][lb] : tout[k + 1];
 						if (!(pk[k] & PK_F64)) {
-							const float q = update_stable<float>(g[k].x, g[k + 1].x, g[k].y, g[k].z, g[k].w, ta, tb, h, inv_h);
+							const float q = update_stable_q(g[k].x, g[k + 1].x, g[k].y, ta, tb, h, inv_h);
 							if (q < best) best = q;
 						} else if ((fks & 0xFu) == k) {
 							fta[0][tid] = ta; ftb[0][tid] = tb;
@@ -246,16 +246,16 @@ __global__ __launch_bounds__(P) void k_patch_sweep(PArgs a, uint32_t k4)
 				// Fans longer than MAXS: stream the slots.
 				uint32_t ida = a.cid[base];
 				uint32_t lia = a.lidx[base];
-				float4 ga = a.cgeo[base];
+				float2 ga = a.cgeo[base];
 				float ta = lia != LOCAL_OUT ? sd[b][lia] : a.dist[ida & SLOT_ID_MASK];
 				for (uint32_t k = 1; k < S; k++) {
 					const uint32_t idb = a.cid[base + 64 * k];
 					if (idb & SLOT_EMPTY) break;
 					const uint32_t lib = a.lidx[base + 64 * k];
-					const float4 gb = a.cgeo[base + 64 * k];
+					const float2 gb = a.cgeo[base + 64 * k];
 					const float tb = lib != LOCAL_OUT ? sd[b][lib] : a.dist[idb & SLOT_ID_MASK];
 					const float q = (ida & F64_FLAG) ? tri_flagged(a, base + 64 * (k - 1), base + 64 * k, ta, tb)
-					                                 : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, h, inv_h);
+					                                 : update_stable_q(ga.x, gb.x, ga.y, ta, tb, h, inv_h);
 					if (q < best) best = q;
 					ida = idb; ga = gb; ta = tb;
 				}
@@ -344,7 +344,7 @@ struct PatchPlan {
 	         *d_src = nullptr;
 	uint16_t *d_lidx = nullptr;
 	uint8_t *d_defer = nullptr;
-	float4 *d_cgeo = nullptr;
+	float2 *d_cgeo = nullptr;
 	double4 *d_cgeo64 = nullptr;
 	float *d_pkey = nullptr, *d_dist = nullptr;
 	double *d_out = nullptr;
@@ -441,8 +441,8 @@ int patch_create(Che *mesh, const uint32_t *src, uint32_t ns, const double *dhat
 	if (rc) return rc;
 	pp->grid = o.grid ? o.grid : (uint32_t) ncu * (uint32_t) std::max(1, occ);
 	pp->max_launch = o.max_sweeps ? o.max_sweeps : 64 * pp->npatch + 1024;
-	std::vector<float4> cg(L.cgeo.size());
-	std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+	std::vector<float2> cg(L.cgeo.size());
+	std::memcpy(cg.data(), L.cgeo.data(), sizeof(float2) * cg.size());
 	std::vector<double4> cg64(L.cgeo64.size());
 	std::memcpy(cg64.data(), L.cgeo64.data(), sizeof(double4) * cg64.size());
 	if ((rc = pupload(pp->d_coff, L.coff.data(), L.coff.size())) || (rc = pupload(pp->d_cid, L.cid.data(), L.cid.size())) ||

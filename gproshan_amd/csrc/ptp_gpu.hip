@@ -221,7 +221,7 @@ struct gp_plan {
 	uint32_t *d_order = nullptr, *d_src = nullptr, *d_trace = nullptr;
 	uint32_t *d_limits = nullptr, *d_coff = nullptr, *d_cid = nullptr;
 	unsigned long long *d_fcum = nullptr;
-	float4 *d_cgeo = nullptr;
+	float2 *d_cgeo = nullptr;
 	double4 *d_cgeo64 = nullptr;
 	gp::Edge64 *d_cx64 = nullptr;
 	void *d_dist0 = nullptr, *d_dist1 = nullptr;
@@ -433,8 +433,8 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 	if (f64) {
 		if ((rc = upload(p->d_cx64, L.cx64))) return rc;
 	} else {
-		std::vector<float4> cg(L.cgeo.size());
-		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float4) * cg.size());
+		std::vector<float2> cg(L.cgeo.size());
+		std::memcpy(cg.data(), L.cgeo.data(), sizeof(float2) * cg.size());
 		if ((rc = upload(p->d_cgeo, cg)) || (rc = upload(p->d_cgeo64, as_d4(L.cgeo64))) ||
 		    (rc = upload(p->d_fcum, L.fcum)))
 			return rc;
@@ -456,7 +456,7 @@ int plan_create_rings(Che *mesh, const uint32_t *src, uint32_t ns, const uint32_
 // mean edge; FP64 solves: neighbour id and the FP64 edge vector.
 template <bool F64>
 __global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, const uint8_t *closed, const double *gt,
-                                uint32_t nv, double ih2, uint32_t *cid, float4 *cgeo, double4 *cgeo64, Edge64 *cx64)
+                                uint32_t nv, double ih2, uint32_t *cid, float2 *cgeo, double4 *cgeo64, Edge64 *cx64)
 {
 #pragma clang fp contract(off)
 	for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += gridDim.x * blockDim.x) {
@@ -495,7 +495,7 @@ __global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, co
 					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq))
 						id |= F64_FLAG;
 				}
-				cgeo[at] = make_float4((float) gd.x, (float) gd.y, (float) gd.z, (float) gd.w);
+				cgeo[at] = make_float2((float) gd.x, (float) gd.y);
 				cgeo64[at] = gd;
 			}
 			if (k == 0 && ns < n) id |= SLOT_WRAP;
@@ -523,7 +523,7 @@ int master_tables(Che &mesh, bool f64, VertexTables &out, uint32_t &max_slots)
 		const size_t nf = std::max<size_t>(dm->nfan, 1);
 		if (hipMalloc(&m->cid, 4 * nf) != hipSuccess ||
 		    (f64 ? hipMalloc(&m->cx64, sizeof(Edge64) * nf)
-		         : (hipMalloc(&m->cgeo, sizeof(float4) * nf) == hipSuccess ? hipMalloc(&m->cgeo64, sizeof(double4) * nf)
+		         : (hipMalloc(&m->cgeo, sizeof(float2) * nf) == hipSuccess ? hipMalloc(&m->cgeo64, sizeof(double4) * nf)
 		                                                                  : hipErrorOutOfMemory)) != hipSuccess)
 			return fail(GP_ERR_NOMEM, "layout: device allocation failed");
 		const double ih = 1.0 / mesh.mean_edge, ih2 = ih * ih;
@@ -674,7 +674,7 @@ int plan_create_device(Che *mesh, const uint32_t *src, uint32_t ns, const gp_ptp
 	if (f64) {
 		if (hipMalloc(&p->d_cx64, sizeof(Edge64) * std::max<size_t>(lanes, 1)) != hipSuccess)
 			return fail(GP_ERR_NOMEM, "plan: device allocation failed");
-	} else if (hipMalloc(&p->d_cgeo, sizeof(float4) * std::max<size_t>(lanes, 1)) != hipSuccess ||
+	} else if (hipMalloc(&p->d_cgeo, sizeof(float2) * std::max<size_t>(lanes, 1)) != hipSuccess ||
 	           hipMalloc(&p->d_cgeo64, sizeof(double4) * std::max<size_t>(lanes, 1)) != hipSuccess) {
 		return fail(GP_ERR_NOMEM, "plan: device allocation failed");
 	}

@@ -68,7 +68,7 @@ __device__ __forceinline__ void gather_slots(const T *od, const uint32_t (&id)[M
 // register path of relax_lane for such a lane, with each slot's edge length
 // computed once and no per-slot selects.
 template <uint32_t SC, uint32_t MS, typename A>
-__device__ __forceinline__ float relax_closed(const A &a, const float4 (&g)[MS], const float (&t)[MS], float best)
+__device__ __forceinline__ float relax_closed(const A &a, const float2 (&g)[MS], const float (&t)[MS], float best)
 {
 	float s[SC];
 #pragma unroll
@@ -76,8 +76,7 @@ __device__ __forceinline__ float relax_closed(const A &a, const float4 (&g)[MS],
 #pragma unroll
 	for (uint32_t k = 0; k < SC; k++) {
 		const uint32_t kn = k + 1 < SC ? k + 1 : 0;
-		const float p = update_stable_s<float>(s[k], s[kn], g[k].x, g[kn].x, g[k].y, g[k].z, g[k].w, t[k], t[kn], a.h,
-		                                       a.inv_h);
+		const float p = update_stable_qs(s[k], s[kn], g[k].x, g[kn].x, g[k].y, t[k], t[kn], a.h, a.inv_h);
 		if (p < best) best = p;
 	}
 	return best;
@@ -90,7 +89,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
            uint32_t *bk_out = nullptr)
 {
 	using T = typename std::conditional<F64, double, float>::type;
-	using G = typename std::conditional<F64, Edge64, float4>::type;
+	using G = typename std::conditional<F64, Edge64, float2>::type;
 	const T old = best;
 	uint32_t bk = NO_TRI;
 	// Strictly lower wins; an equal value from an earlier triangle (evaluated
@@ -132,8 +131,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 						p = (float) update_stable<double>(g[k].x, q1, g[k].y, g[k].z, g[k].w, (double) t[k], (double) tb,
 						                                  (double) a.h, (double) a.inv_h);
 					if (tri && !fl)
-						p = update_stable<float>((float) g[k].x, (float) q1, (float) g[k].y, (float) g[k].z,
-						                         (float) g[k].w, t[k], tb, a.h, a.inv_h);
+						p = update_stable_q((float) g[k].x, (float) q1, (float) g[k].y, t[k], tb, a.h, a.inv_h);
 					take(p, k);
 				}
 			}
@@ -210,7 +208,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 					// selects.
 					const bool fl = (id[k] & F64_FLAG) != 0u;
 					const float q1 = nr ? g[(k + 1) % MS].x : g[0].x;
-					const float p = update_stable<float>(g[k].x, q1, g[k].y, g[k].z, g[k].w, t[k], tb, a.h, a.inv_h);
+					const float p = update_stable_q(g[k].x, q1, g[k].y, t[k], tb, a.h, a.inv_h);
 					take((tri && !fl) ? p : (T) INFINITY, k);
 					const bool first = tri && fl && k == k0;
 					fa = first ? t[k] : fa;
@@ -238,7 +236,7 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 				take(tri_f64(ga, gb, ta, tb), kt);
 			} else {
 				const T p = (ida & F64_FLAG) ? tri_flagged(a, base + ST * (size_t) kt, at_n, ta, tb)
-				                             : update_stable<float>(ga.x, gb.x, ga.y, ga.z, ga.w, ta, tb, a.h, a.inv_h);
+				                             : update_stable_q(ga.x, gb.x, ga.y, ta, tb, a.h, a.inv_h);
 				take(p, kt);
 			}
 		};
@@ -276,7 +274,7 @@ struct VertexTables {
 	const uint32_t *coff;
 	const uint8_t *closed;
 	const uint32_t *cid;
-	const float4 *cgeo;
+	const float2 *cgeo;
 	const double4 *cgeo64;
 	const Edge64 *cx64;
 };
@@ -288,7 +286,7 @@ struct VertexTables {
 // device once per mesh and precision.
 struct MasterLayout {
 	uint32_t *cid = nullptr;
-	float4 *cgeo = nullptr;
+	float2 *cgeo = nullptr;
 	double4 *cgeo64 = nullptr;
 	Edge64 *cx64 = nullptr;
 	uint32_t max_slots = 0;
@@ -322,7 +320,7 @@ struct GeoRemap64 {
 // index in the vertex-major tables (midx).
 template <bool F64, bool MIDX>
 __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, const uint32_t *nr_at, uint32_t npos,
-                                uint32_t S, const uint32_t *coff, VertexTables m, uint32_t *cid, float4 *cgeo,
+                                uint32_t S, const uint32_t *coff, VertexTables m, uint32_t *cid, float2 *cgeo,
                                 double4 *cgeo64, uint32_t *midx, Edge64 *cx64)
 {
 	const uint32_t nr = *nr_at;
@@ -346,7 +344,7 @@ __global__ void k_gather_tables(const uint32_t *sorted, const uint32_t *inv, con
 			if constexpr (F64) {
 				cx64[at] = real ? m.cx64[mb + k] : Edge64{ 0, 0, 0 };
 			} else {
-				cgeo[at] = real ? m.cgeo[mb + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+				cgeo[at] = real ? m.cgeo[mb + k] : make_float2(0.f, 0.f);
 				if constexpr (MIDX) midx[at] = real ? mb + k : 0u;
 				else cgeo64[at] = real ? m.cgeo64[mb + k] : make_double4(0, 0, 0, 0);
 			}

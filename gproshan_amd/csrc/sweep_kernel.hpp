@@ -46,7 +46,7 @@ struct KArgs {
 	const uint32_t *coff;      // per chunk: first slot row
 	uint32_t sb;               // > 0: every chunk has sb slots (chunk c at row c * sb; coff unused)
 	const uint32_t *cid;
-	const float4 *cgeo;
+	const float2 *cgeo;        // FP32 geometry: q0, g per slot
 	const double4 *cgeo64;
 	const Edge64 *cx64;
 	const unsigned long long *fcum;  // FP64-update triangles of positions [0, pos)

@@ -145,4 +145,24 @@ GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 	return update_stable_s<T>(stable_sqrt(q0), stable_sqrt(q1), q0, q1, g, det, S, t0, t1, h, inv_h);
 }
 
+// The FP32 bulk path from the compact slot table (q0 and g per slot, q1 from
+// the next slot): det = q0 q1 - g^2 and S = q0 + q1 - 2 g are formed here in
+// FP32, one fused multiply-add each, so a slot carries 8 bytes of geometry
+// instead of 16.  Their rounding error relative to the exact values is
+// about 2.4e-7 / sin^2(theta) for det and 2.4e-7 / (4 sin^2(theta / 2)) for S
+// (theta: the angle at the updated vertex, q0 ~ q1); the corners where that
+// grows (sin^2 < 0.01) are flagged and take the FP64 update on the FP64
+// table, whose det and S come from the cross product and the third edge.
+GP_HD float update_stable_qs(float s0, float s1, float q0, float q1, float g, float t0, float t1, float h, float inv_h)
+{
+	const float det = __builtin_fmaf(q0, q1, -(g * g));
+	const float S = __builtin_fmaf(-2.f, g, q0 + q1);
+	return update_stable_s<float>(s0, s1, q0, q1, g, det, S, t0, t1, h, inv_h);
+}
+
+GP_HD float update_stable_q(float q0, float q1, float g, float t0, float t1, float h, float inv_h)
+{
+	return update_stable_qs(stable_sqrt(q0), stable_sqrt(q1), q0, q1, g, t0, t1, h, inv_h);
+}
+
 }  // namespace gp

@@ -24,5 +24,9 @@ def test_iter_error_f64_matches_oracle(oracle, n, seed, src):
     assert np.array_equal(band, band0)
     assert np.array_equal(it, it0)
     assert np.allclose(err, err0, rtol=1e-10, atol=0)
+    # err is a mean relative error in percent: an FP32 field whose mean
+    # relative deviation from the FP64 one is 1e-7 (about two FP32 ulps of
+    # accumulated rounding) moves it by at most 1e-5.
     it32, err32, _ = P.iter_error_parallel_toplesets_propagation_coalescence_gpu(C, src, exact, L.PREC_F32)
-    assert abs(err32[-1] - err0[-1]) <= 1e-3 * err0[-1] + 1e-6
+    print(f"  f32 last {err32[-1]:.6e} % (f64 {err0[-1]:.6e} %)", flush=True)
+    assert abs(err32[-1] - err0[-1]) <= 1e-5
