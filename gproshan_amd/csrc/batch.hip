@@ -92,11 +92,15 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 	float *d0 = a.dist + 2 * wg * mv, *d1 = d0 + mv;
 	uint32_t *fcid = a.fcid + wg * a.fslots, *fmidx = a.fmidx + wg * a.fslots, *finv = a.finv + wg * mv;
 	float2 *fgeo = a.fgeo + wg * a.fslots;
-	const uint32_t lane = tid & 63u, wave = tid >> 6, nwave = NT >> 6;
+	// Values every thread of the workgroup holds alike (the wave index, the
+	// job, the ring-test count) are taken from the first lane, so the
+	// compiler keeps them and what follows from them (the mesh, the window,
+	// the chunk index, the slot count) in scalar registers.
+	const uint32_t lane = tid & 63u, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwave = NT >> 6;
 	for (;;) {
 		if (tid == 0) s_job = atomicAdd(a.queue, 1u);
 		__syncthreads();
-		const uint32_t job = s_job;
+		const uint32_t job = __builtin_amdgcn_readfirstlane(s_job);
 		__syncthreads();
 		if (job >= a.njobs) break;
 		const uint32_t src = a.jobs[job].source;
@@ -166,7 +170,7 @@ __global__ __launch_bounds__(NT) void k_batch(BArgs a)
 			}
 			if (lane == 0 && bad) atomicAdd(&s_bad, bad);
 			__syncthreads();
-			const uint32_t nbad = s_bad;
+			const uint32_t nbad = __builtin_amdgcn_readfirstlane(s_bad);
 			relax += end - start;
 			i = ii + (nbad == 0 ? 1u : 0u);
 			if (j < nl - 1) j++;

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_fps_sweep(FArgs a, uint32_t k4)
 	const uint32_t x = blockIdx.x & 7u;
 	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
 	const uint32_t wpb = blockDim.x >> 6;
-	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
+	const uint32_t lw = (blockIdx.x >> 3) * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
 	const uint32_t part = (c1 - c0 + ngrp - 1) / ngrp;
 	const uint32_t lo = c0 + x * part, hi = min(c1, lo + part);
 	uint32_t bad = 0;

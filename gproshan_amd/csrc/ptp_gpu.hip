@@ -1083,12 +1083,14 @@ int plan_create_patches(Che *mesh, const uint32_t *src, uint32_t ns, const doubl
 // mean device time of two) and remembered per plan shape, so that a later plan
 // of the same shape takes the choice without timing.
 namespace {
+// The schedule is part of the key: learned and Euclidean bands of one mesh
+// have different window sequences, and their best XCD deal differs.
 struct TuneKey {
-	uint32_t nv_log2, max_slots, f64, band, coalesce, uncapped;
+	uint32_t nv_log2, max_slots, f64, band, coalesce, uncapped, schedule;
 	bool operator<(const TuneKey &o) const
 	{
-		return std::tie(nv_log2, max_slots, f64, band, coalesce, uncapped) <
-		       std::tie(o.nv_log2, o.max_slots, o.f64, o.band, o.coalesce, o.uncapped);
+		return std::tie(nv_log2, max_slots, f64, band, coalesce, uncapped, schedule) <
+		       std::tie(o.nv_log2, o.max_slots, o.f64, o.band, o.coalesce, o.uncapped, o.schedule);
 	}
 };
 struct WidthKey {
@@ -1105,7 +1107,9 @@ bool g_tune_loaded = false;
 
 // The choices persist across processes when GPROSHAN_AMD_TUNE_FILE names a
 // file: it is read on first use and every new choice is appended, one line
-// each ("shape <key> <grid> <replay> <ms>", "width <key> <width>").
+// each ("shape <key> <grid> <replay> <xcd block> <ms>", "width <key> <width>");
+// a line that does not parse whole (another key layout, a damaged entry) is
+// skipped.
 const char *tune_file() { return std::getenv("GPROSHAN_AMD_TUNE_FILE"); }
 
 void tune_load_locked()
@@ -1116,26 +1120,25 @@ void tune_load_locked()
 	if (!path || !*path) return;
 	FILE *f = std::fopen(path, "r");
 	if (!f) return;
-	char kind[16];
-	while (std::fscanf(f, "%15s", kind) == 1) {
-		if (!std::strcmp(kind, "shape")) {
-			TuneKey k{};
-			gp_tune t{};
-			if (std::fscanf(f, "%u %u %u %u %u %u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.f64, &k.band,
-			                &k.coalesce, &k.uncapped, &t.grid, &t.replay, &t.xcd_block, &t.kernel_ms) != 10)
-				break;
+	char line[512];
+	while (std::fgets(line, sizeof(line), f)) {
+		TuneKey k{};
+		gp_tune t{};
+		WidthKey wk{};
+		double w = 0;
+		int used = 0;
+		const size_t len = std::strlen(line);
+		if (std::sscanf(line, "shape %u %u %u %u %u %u %u %u %u %u %lf %n", &k.nv_log2, &k.max_slots, &k.f64,
+		                &k.band, &k.coalesce, &k.uncapped, &k.schedule, &t.grid, &t.replay, &t.xcd_block,
+		                &t.kernel_ms, &used) == 11 && (size_t) used == len) {
 			// Only shapes the tuner itself can choose (a damaged file must not
 			// make a plan capture an oversized graph).
 			if (t.grid && t.grid <= (1u << 20) && t.replay >= 4 && t.replay <= 1024 && t.replay % 4 == 0 &&
 			    t.xcd_block < 16)
 				g_tune_cache[k] = t;
-		} else if (!std::strcmp(kind, "width")) {
-			WidthKey k{};
-			double w = 0;
-			if (std::fscanf(f, "%u %u %u %u %lf", &k.nv_log2, &k.max_slots, &k.schedule, &k.f64, &w) != 5) break;
-			if (w > 0 && w <= 64) g_width_cache[k] = w;
-		} else {
-			break;
+		} else if (std::sscanf(line, "width %u %u %u %u %lf %n", &wk.nv_log2, &wk.max_slots, &wk.schedule, &wk.f64,
+		                       &w, &used) == 5 && (size_t) used == len) {
+			if (w > 0 && w <= 64) g_width_cache[wk] = w;
 		}
 	}
 	std::fclose(f);
@@ -1202,7 +1205,8 @@ int plan_autotune(gp_plan *p, gp_tune *out)
 	if (!p) return fail(GP_ERR_INVALID, "autotune: plan is null");
 	if (p->pp) return fail(GP_ERR_INVALID, "autotune: ring and band plans only");
 	const TuneKey key{ log2_floor(p->n), p->max_slots, p->opts.precision == GP_PREC_F64 ? 1u : 0u, p->band ? 1u : 0u,
-	                   (uint32_t) (p->opts.coalesce != 0), (uint32_t) (p->opts.uncapped != 0) };
+	                   (uint32_t) (p->opts.coalesce != 0), (uint32_t) (p->opts.uncapped != 0),
+	                   (uint32_t) p->opts.schedule };
 	{
 		std::lock_guard<std::mutex> lock(g_tune_mu);
 		tune_load_locked();
@@ -1260,9 +1264,9 @@ int plan_autotune(gp_plan *p, gp_tune *out)
 		std::lock_guard<std::mutex> lock(g_tune_mu);
 		g_tune_cache[key] = best;
 		char line[200];
-		std::snprintf(line, sizeof(line), "shape %u %u %u %u %u %u %u %u %u %.6f\n", key.nv_log2, key.max_slots,
-		              key.f64, key.band, key.coalesce, key.uncapped, best.grid, best.replay, best.xcd_block,
-		              best.kernel_ms);
+		std::snprintf(line, sizeof(line), "shape %u %u %u %u %u %u %u %u %u %u %.6f\n", key.nv_log2, key.max_slots,
+		              key.f64, key.band, key.coalesce, key.uncapped, key.schedule, best.grid, best.replay,
+		              best.xcd_block, best.kernel_ms);
 		tune_append(line);
 	}
 	if (out) *out = best;

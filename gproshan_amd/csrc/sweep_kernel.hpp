@@ -89,7 +89,11 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 	const uint32_t x = blockIdx.x & 7u;
 	const uint32_t nbx = (gridDim.x - x + 7u) >> 3;
 	const uint32_t wpb = blockDim.x >> 6;
-	const uint32_t lw = (blockIdx.x >> 3) * wpb + (threadIdx.x >> 6);
+	// The wave's index, taken from the first lane: the same for the whole
+	// wave, and known to the compiler as such, so the chunk index, its slot
+	// rows and the slot count live in scalar registers and the per-slot tests
+	// are scalar branches instead of per-lane compares and exec masks.
+	const uint32_t lw = (blockIdx.x >> 3) * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const uint32_t nw_x = nbx * wpb;
 	uint32_t bad = 0;
 	static_assert(!(VM && CL), "vertex-major sweeps carry no labels");

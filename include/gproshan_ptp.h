@@ -357,7 +357,7 @@ typedef struct gp_tune {
  * the mean device time of two runs each.  The fastest shape is kept in the plan and
  * remembered for plans of the same shape (vertex count rounded down to a power
  * of two, longest fan, precision, band or ring schedule, coalescing, uncapped
- * loop), which then take it without timing.  The plan's last result is that of
+ * loop, schedule kind), which then take it without timing.  The plan's last result is that of
  * the last timed run.  Patch plans: GP_ERR_INVALID. */
 int gp_plan_autotune(gp_plan *plan, gp_tune *out);
 

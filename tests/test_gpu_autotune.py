@@ -92,7 +92,7 @@ def test_tune_file_persists_across_processes(tmp_path):
     assert second["candidates"] == 0
     assert (second["grid"], second["replay"], second["xcd_block"]) == (first["grid"], first["replay"], first["xcd_block"])
     # A file whose only entry for this shape is out of range: tuned again.
-    key = tf.read_text().split()[1:7]
+    key = tf.read_text().split()[1:8]
     bad = tmp_path / "bad.txt"
     bad.write_text("shape " + " ".join(key) + " 4096 1000000 0 1.0\n")
     third = _tune_in_child(tmp_path, bad)
