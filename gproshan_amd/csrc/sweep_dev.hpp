@@ -77,7 +77,7 @@ __device__ __forceinline__ float relax_closed(const A &a, const float2 (&g)[MS],
 	for (uint32_t k = 0; k < SC; k++) {
 		const uint32_t kn = k + 1 < SC ? k + 1 : 0;
 		const float p = update_stable_qs(s[k], s[kn], g[k].x, g[kn].x, g[k].y, t[k], t[kn], a.h, a.inv_h);
-		if (p < best) best = p;
+		best = fminf(best, p);   // a NaN p (degenerate triangle) keeps best, as p < best would
 	}
 	return best;
 }

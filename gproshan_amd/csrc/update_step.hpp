@@ -125,8 +125,10 @@ GP_HD T update_stable_s(T s0, T s1, T q0, T q1, T g, T det, T S, T t0, T t1, T h
 	const T inf = (T) INFINITY;
 	const T dp0 = t0 + s0 * h;
 	const T dp1 = t1 + s1 * h;
-	const T edge = (dp1 < dp0) ? dp1 : dp0;
-	const T m = t0 < t1 ? t0 : t1;
+	// fmin: one min instruction; the inputs are never NaN (a finite value or
+	// INF plus a finite length) nor -0, where it could differ from a compare.
+	const T edge = fmin(dp0, dp1);
+	const T m = fmin(t0, t1);
 	const T u0 = (t0 - m) * inv_h;
 	const T u1 = (t1 - m) * inv_h;
 	const T D = u0 * (q1 - g) + u1 * (q0 - g);
