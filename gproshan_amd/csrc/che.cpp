@@ -205,7 +205,8 @@ bool prefer_f64(const Che &m)
 				const double cz = X0[0] * X1[1] - X0[1] * X1[0];
 				const double det = cx * cx + cy * cy + cz * cz;
 				const double qq = q0 * q1;
-				if (q0 * q1 - g * g == 0 || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq))
+				if (q0 * q1 - g * g == 0 || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq) ||
+				    q0 > FLAG_Q_RATIO * q1 || q1 > FLAG_Q_RATIO * q0)
 					nflag++;
 			}
 		}
@@ -303,7 +304,8 @@ int build_layout(const Che &m, const uint32_t *order, uint32_t nr, bool f64, Lay
 					// apply when an input is INF, as in the reference).
 					const bool ref_degenerate = q0 * q1 - g * g == 0;
 					if (ref_degenerate) gd.z = std::nan("");
-					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq)) {
+					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * std::sqrt(qq) ||
+					    q0 > FLAG_Q_RATIO * q1 || q1 > FLAG_Q_RATIO * q0) {
 						id |= F64_FLAG;
 						nflag++;
 					}

@@ -75,9 +75,14 @@ struct Edge64 { double x, y, z; };     // FP64 edge vector a - v of a fan entry
 // the FP64 distance field as inputs, the FP32 update of update_stable stays
 // within 8e-7 of its FP64 evaluation (no corner of those meshes is flagged;
 // the corners in 0.01 <= sin^2 < 0.03 or -0.97 <= cos < -0.9, which a wider
-// test flags, are within 2.4e-7); on a 1:100 anisotropic grid 89% of the
-// corners are flagged.
-constexpr double FLAG_SIN2_MIN = 0.01, FLAG_COS_MIN = -0.97;
+// test flags, are within 2.4e-7).  Also flagged: a corner whose two edges
+// differ in length by more than 10x (squared lengths by more than
+// FLAG_Q_RATIO), where the FP32 form in units of the mean edge loses
+// precision -- on a 512^2 grid stretched 1:100 the unflagged corners' FP32
+// updates were up to 8.6e-4 off their FP64 evaluation without this test and
+// are within 3e-5 with it.  Regular meshes have no such corner; on a 1:100
+// anisotropic grid 99% of the corners are flagged (89% by the angle tests).
+constexpr double FLAG_SIN2_MIN = 0.01, FLAG_COS_MIN = -0.97, FLAG_Q_RATIO = 100.0;
 
 // FP32 solves of a mesh whose flagged triangles exceed a quarter of its fan
 // slots run FP64 sweeps instead (gp_ptp_opts.fp32_geometry == GP_GEO_AUTO):
@@ -90,7 +95,7 @@ bool prefer_f64(const Che &m);
 // when the mesh is numbered by rings) are grouped 64 to a chunk, one lane per
 // vertex.  Chunk c has S_c = coff[c+1] - coff[c] slots, the longest fan in it;
 // slot k of lane l lives at (coff[c] + k) * 64 + l, so every slot load of a
-// wave is one contiguous 256-byte (or 1 KiB) access.  Slot k holds fan entry
+// wave is one contiguous 256-byte (id) or 512-byte (FP32 geometry) access.  Slot k holds fan entry
 // k: cid = its vertex id, bit 31 = FP64 update for the triangle of slot k,
 // bit 30 = empty slot (id = the vertex itself, never a triangle), bit 29 on
 // slot 0 = the fan is closed and stored without its repeated first entry: the

@@ -492,7 +492,8 @@ __global__ void k_master_tables(const uint32_t *fan_off, const uint32_t *fan, co
 					// (its update is NaN there; see build_layout).
 					const bool ref_degenerate = q0 * q1 - g * g == 0;
 					if (ref_degenerate) gd.z = __longlong_as_double(0x7FF8000000000000ll);
-					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq))
+					if (ref_degenerate || !(qq > 0) || det < FLAG_SIN2_MIN * qq || g < FLAG_COS_MIN * sqrt(qq) ||
+					    q0 > FLAG_Q_RATIO * q1 || q1 > FLAG_Q_RATIO * q0)
 						id |= F64_FLAG;
 				}
 				cgeo[at] = make_float2((float) gd.x, (float) gd.y);

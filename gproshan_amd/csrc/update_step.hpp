@@ -108,6 +108,8 @@ GP_HD float stable_rcp(float x)
 #endif
 }
 GP_HD double stable_rcp(double x) { return 1.0 / x; }
+GP_HD float stable_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+GP_HD double stable_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Branch-free: both the planar solution and the edge paths are evaluated and
 // the result is selected, so the unrolled fan loops of the kernels run as
@@ -119,26 +121,41 @@ GP_HD double stable_rcp(double x) { return 1.0 / x; }
 //
 // s0 = sqrt(q0) and s1 = sqrt(q1) (edge lengths in units of h) are taken as
 // arguments, so that a caller holding a whole fan computes each slot's once.
+//
+// Contraction is off and the fused multiply-adds are written out, so every
+// inlined copy rounds alike (the FP32-table and FP64-table register paths must
+// give the same bits).
 template <typename T>
 GP_HD T update_stable_s(T s0, T s1, T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
 	const T inf = (T) INFINITY;
-	const T dp0 = t0 + s0 * h;
-	const T dp1 = t1 + s1 * h;
+	const T dp0 = stable_fma(s0, h, t0);
+	const T dp1 = stable_fma(s1, h, t1);
 	// fmin: one min instruction; the inputs are never NaN (a finite value or
 	// INF plus a finite length) nor -0, where it could differ from a compare.
 	const T edge = fmin(dp0, dp1);
 	const T m = fmin(t0, t1);
-	const T u0 = (t0 - m) * inv_h;
-	const T u1 = (t1 - m) * inv_h;
-	const T D = u0 * (q1 - g) + u1 * (q0 - g);
-	const T E = u0 * u0 * q1 - (T) 2 * u0 * u1 * g + u1 * u1 * q0 - det;
-	const T disc = D * D - S * E;
+	// One of u0, u1 is 0 (the earlier arrival): with du the other one, qa the
+	// earlier vertex's q and qb the later one's, D = du (qa - g),
+	// E = du^2 qa - det, and the two conditions become
+	// qa du - p' (qa - g) (the later vertex's) and p' (g - qb) - g du (the
+	// earlier one's) -- the terms above with the zero products dropped.
+	const bool first = t0 <= t1;
+	const T du = fabs(t1 - t0) * inv_h;
+	const T qa = first ? q0 : q1;
+	const T qb = first ? q1 : q0;
+	const T ag = qa - g;
+	const T D = du * ag;
+	const T E = stable_fma(du * du, qa, -det);
+	const T disc = stable_fma(D, D, -(S * E));
 	const T pp = (D + stable_sqrt(disc)) * stable_rcp(S);
-	const T c0 = q1 * (u0 - pp) - g * (u1 - pp);
-	const T c1 = q0 * (u1 - pp) - g * (u0 - pp);
-	const bool planar = (t0 != inf && t1 != inf) && !(disc < (T) 0 || c0 >= (T) 0 || c1 >= (T) 0);
-	return planar ? m + pp * h : edge;
+	const T cl = stable_fma(qa, du, -(pp * ag));
+	const T ce = stable_fma(pp, g - qb, -(g * du));
+	const bool planar = (t0 != inf && t1 != inf) && !(disc < (T) 0 || cl >= (T) 0 || ce >= (T) 0);
+	return planar ? stable_fma(pp, h, m) : edge;
 }
 
 template <typename T>
@@ -157,6 +174,9 @@ GP_HD T update_stable(T q0, T q1, T g, T det, T S, T t0, T t1, T h, T inv_h)
 // table, whose det and S come from the cross product and the third edge.
 GP_HD float update_stable_qs(float s0, float s1, float q0, float q1, float g, float t0, float t1, float h, float inv_h)
 {
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
 	const float det = __builtin_fmaf(q0, q1, -(g * g));
 	const float S = __builtin_fmaf(-2.f, g, q0 + q1);
 	return update_stable_s<float>(s0, s1, q0, q1, g, det, S, t0, t1, h, inv_h);
