@@ -94,7 +94,8 @@ enum gp_launch_mode {
 
 /* Geometry table the FP32 ring sweeps read.  Both forms give the same bits:
  * the FP32 triangle updates use the FP32 rounding of the FP64 geometry and the
- * flagged (near-degenerate or obtuse) corners the FP64 geometry itself.
+ * flagged corners (near-degenerate or obtuse, or with edges more than 10x
+ * apart in length) the FP64 geometry itself.
  * GP_GEO_FP32 reads 8 bytes per slot (|X0|^2 and X0.X1; the FP32 update forms
  * the determinant and third edge from them) and loads the FP64 geometry of flagged
  * corners on the side (the first per vertex with the neighbour distances, the
