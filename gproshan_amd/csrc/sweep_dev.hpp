@@ -140,12 +140,14 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 		}
 	}
 	if (S <= MS) {
+		// Slots past S hold id 0 (no flag bits); every use of id[k] below is
+		// guarded by k < S.
 		uint32_t id[MS];
 		G g[MS];
 		T t[MS];
 #pragma unroll
 		for (uint32_t k = 0; k < MS; k++) {
-			id[k] = SLOT_EMPTY;
+			id[k] = 0;
 			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
 		}
 		const bool wrap = (id[0] & SLOT_WRAP) != 0u;
@@ -153,10 +155,10 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 		// closed fan without empty slots or flagged triangles (the interior of
 		// a mesh), the closed-fan path for the chunk's slot count.
 		if constexpr (!F64 && !CL && ST == 64) {
-			bool plain = wrap;
+			uint32_t flags = 0;
 #pragma unroll
-			for (uint32_t k = 0; k < MS; k++)
-				if (k < S) plain = plain && !(id[k] & (SLOT_EMPTY | F64_FLAG));
+			for (uint32_t k = 0; k < MS; k++) flags |= id[k];
+			const bool plain = wrap && !(flags & (SLOT_EMPTY | F64_FLAG));
 			if (__ballot(!plain) == 0 && S >= 3 && S <= 8 && S <= MS) {
 				gather_slots(od, id, S, t);
 				switch (S) {
