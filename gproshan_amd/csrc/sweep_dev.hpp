@@ -31,6 +31,21 @@ __device__ __forceinline__ double tri_f64(const Edge64 &xa, const Edge64 &xb, do
 	return update_ref_f64_x(X0, X1, ta, tb);
 }
 
+// The same with each fan entry's |X|^2 and |X| formed once by the caller.
+__device__ __forceinline__ double tri_f64_q(const Edge64 &xa, const Edge64 &xb, double qa, double qb, double sa,
+                                            double sb, double ta, double tb)
+{
+	const double X0[3] = { xa.x, xa.y, xa.z };
+	const double X1[3] = { xb.x, xb.y, xb.z };
+	return update_ref_f64_q(X0, X1, qa, qb, sa, sb, ta, tb);
+}
+
+__device__ __forceinline__ double edge_sq(const Edge64 &x)
+{
+	const double X[3] = { x.x, x.y, x.z };
+	return ref_sq(X);
+}
+
 // New value of the vertex at lane slot base (chunk with S slots; fans of up to
 // MS slots are held in registers, longer ones streamed; ST is the
 // distance between a lane's consecutive slots: 64 in the chunked layout, 1 in
@@ -194,6 +209,15 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 		}
 		gather_slots(od, id, S, t);
 		T fa = (T) INFINITY, fb = (T) INFINITY;
+		// FP64: each entry's |X|^2 and |X| once per fan (rolled from one
+		// triangle to the next; entry 0's kept for the closing triangle).
+		double q_0 = 0, s_0 = 0, q_a = 0, s_a = 0;
+		if constexpr (F64) {
+			q_0 = edge_sq(g[0]);
+			s_0 = sqrt(q_0);
+			q_a = q_0;
+			s_a = s_0;
+		}
 #pragma unroll
 		for (uint32_t k = 0; k < MS; k++) {
 			if (k < S) {
@@ -202,7 +226,15 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 				const T tb = nr ? t[(k + 1) % MS] : t[0];
 				if constexpr (F64) {
 					const G gb = nr ? g[(k + 1) % MS] : g[0];
-					if (tri) take(tri_f64(g[k], gb, t[k], tb), k);
+					double q_n = q_0, s_n = s_0;
+					if (k + 1 < MS && k + 1 < S) {
+						q_n = edge_sq(g[(k + 1) % MS]);
+						s_n = sqrt(q_n);
+					}
+					const double q_b = nr ? q_n : q_0, s_b = nr ? s_n : s_0;
+					if (tri) take(tri_f64_q(g[k], gb, q_a, q_b, s_a, s_b, t[k], tb), k);
+					q_a = q_n;
+					s_a = s_n;
 				} else {
 					// Straight-line per triangle: every lane evaluates the
 					// FP32 update and keeps it only for a real, unflagged

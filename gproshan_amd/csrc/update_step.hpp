@@ -21,15 +21,28 @@ namespace gp {
 // cu_update_step, src/cuda/geodesics_ptp.cu:301-369).  Same operations in the
 // same order; NaN (det == 0) is returned as NaN and loses every '<' in the
 // caller, as in the reference.
-GP_HD double update_ref_f64_x(const double X0[3], const double X1[3], double t0, double t1)
+//
+// |X|^2 as the reference forms q00 and q11 (and the square under the edge
+// paths' sqrt): the same expression, so a caller holding a whole fan can form
+// it, and its square root, once per fan entry and pass them in
+// (update_ref_f64_q) with the same bits as a result.
+GP_HD double ref_sq(const double X[3])
 {
 #ifdef __clang__
 #pragma clang fp contract(off)
 #endif
-	const double q00 = X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2];
+	return X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+}
+
+template <bool PRE>
+GP_HD double update_ref_core(const double X0[3], const double X1[3], double q00, double q11, double s0, double s1,
+                             double t0, double t1)
+{
+#ifdef __clang__
+#pragma clang fp contract(off)
+#endif
 	const double q01 = X0[0] * X1[0] + X0[1] * X1[1] + X0[2] * X1[2];
 	const double q10 = X1[0] * X0[0] + X1[1] * X0[1] + X1[2] * X0[2];
-	const double q11 = X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2];
 
 	const double det = q00 * q11 - q01 * q10;
 	const double Q00 = q11 / det;
@@ -53,11 +66,24 @@ GP_HD double update_ref_f64_x(const double X0[3], const double X1[3], double t0,
 	const double c1 = cond0 * Q10 + cond1 * Q11;
 
 	if (t0 == INFINITY || t1 == INFINITY || dis < 0 || c0 >= 0 || c1 >= 0) {
-		const double dp0 = t0 + sqrt(X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]);
-		const double dp1 = t1 + sqrt(X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2]);
+		const double dp0 = t0 + (PRE ? s0 : sqrt(ref_sq(X0)));
+		const double dp1 = t1 + (PRE ? s1 : sqrt(ref_sq(X1)));
 		p = (dp1 < dp0) ? dp1 : dp0;
 	}
 	return p;
+}
+
+GP_HD double update_ref_f64_x(const double X0[3], const double X1[3], double t0, double t1)
+{
+	return update_ref_core<false>(X0, X1, ref_sq(X0), ref_sq(X1), 0.0, 0.0, t0, t1);
+}
+
+// The same with q00 = ref_sq(X0), q11 = ref_sq(X1), s0 = sqrt(q00) and
+// s1 = sqrt(q11) supplied by the caller.
+GP_HD double update_ref_f64_q(const double X0[3], const double X1[3], double q00, double q11, double s0, double s1,
+                              double t0, double t1)
+{
+	return update_ref_core<true>(X0, X1, q00, q11, s0, s1, t0, t1);
 }
 
 // The same from the three positions: X0 = ga - gv, X1 = gb - gv, as the
