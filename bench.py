@@ -14,17 +14,21 @@ Rows:
     schedule; it stops ~1e-2 short of the relaxation's fixed point;
   * plain_uncapped_f32: the reference's uncapped loop at tol 1e-6, the plain
     ring schedule that reaches the fixed point within 1e-4 ("plain at bar");
-  * band_euclid / band_learned: bands (0.6 mean edge lengths wide) of the
-    Euclidean or the learned distance prediction in place of the rings in
-    the same uncapped loop at tol 1e-6 (band_learned when a trained model
+  * band_euclid / band_learned: bands of the Euclidean or the learned
+    distance prediction (width picked by the library's band-width tuner from
+    0.45-1.0 mean edge lengths) in place of the rings in the same uncapped
+    loop at tol 1e-6, then check sweeps (band_learned when a trained model
     exists);
   * patches: the patch schedule (LDS-iterated patches, active list) at tol 1e-6.
 value = ms per field of band_learned when it meets the 1e-4 bar against the
 FP64 fixed point, else of the fastest row that does; the JSON carries every
 row with its errors against both FP64 references, the end-to-end time of a
 reference-schedule solve on a fresh mesh handle (plan with host- or
-device-built rings and tables, upload, run, result download) and the
-16-source field of the same mesh.
+device-built rings and tables, upload, run, result download), the 16-source
+field of the same mesh, and (unless --no-c3c5 / --no-next) the batched
+256-mesh launch (c3), the precision stress grid (c5) and the rows around the
+path (next: farthest-point sampling, cluster labels, iteration-error loop,
+mesh readers).
 """
 from __future__ import annotations
 
