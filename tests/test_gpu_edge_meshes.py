@@ -13,6 +13,9 @@
     source (no sweep runs);
   * coordinates far from the origin (3e5 offset) and at a micro scale (1e-6):
     the FP32 path works on FP64 differences in units of the mean edge;
+  * strips of thin quads (every eighth column 20x narrower): right-angle
+    corners whose two edges differ 20x in length, which only the edge-ratio
+    test sends to the FP64 update;
   * non-manifold input (a bowtie vertex, an edge with three triangles), where
     the half-edge tables follow the reference's (NIL twins and EVT) and so does
     what propagates.
@@ -65,6 +68,17 @@ def collapsed():
     return xyz, f
 
 
+def strips():
+    n = 96
+    xyz, f = G.height_field(n, 38)
+    xyz = xyz.copy()
+    gap = np.where(np.arange(n - 1) % 8 == 7, 0.05, 1.0)
+    xs = np.concatenate([[0.0], np.cumsum(gap)])
+    xyz[:, 0] = (xs / xs[-1])[np.arange(len(xyz)) % n]
+    xyz[:, 2] *= 0.02          # nearly flat, so a thin quad's short edge stays short
+    return xyz, f
+
+
 def one_triangle():
     return np.array([[0, 0, 0], [1, 0, 0], [0.2, 0.9, 0.1]], float), np.array([[0, 1, 2]], np.uint32)
 
@@ -93,6 +107,7 @@ CASES = [
     ("holes", holes, lambda nv: [nv // 2]),
     ("sources300", lambda: G.height_field(128, 34), lambda nv: G.draw_sources(np.random.default_rng(34), nv, 300)),
     ("collapsed", collapsed, lambda nv: [n for n in (64 * 32 + 32,)]),
+    ("strips", strips, lambda nv: [96 * 48 + 20]),
     ("one_triangle", one_triangle, lambda nv: [0]),
     ("quad", quad, lambda nv: [1]),
     ("all_sources", lambda: G.height_field(8, 35), lambda nv: list(range(nv))),
