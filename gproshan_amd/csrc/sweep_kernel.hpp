@@ -189,6 +189,13 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	}
 	const uint32_t kp = (k4 + 3u) & 3u;
 	const Slot prev = st->slot[kp];
+	// Read with the slot, before any store of this launch, and held in a
+	// scalar register here (the empty asm takes it as an input, so the load
+	// is not sunk into the branch below): both loads are issued together and
+	// the count does not add a second round trip to the chain every sweep
+	// starts with.
+	const uint32_t nbad = st->bad[kp];
+	asm volatile("" ::"s"(nbad));
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
 	if (prev.done) {
 		// Later launches of the replay read slot k & 3 in turn: keep it done.
@@ -199,7 +206,6 @@ __global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
 	cur.fresh = 0;
 	if (!prev.fresh) {
 		// The previous sweep's outcome: the reference loop tail.
-		const uint32_t nbad = st->bad[kp];
 		const uint32_t pi = a.k2 && prev.i < (prev.j >> 1) ? (prev.j >> 1) : prev.i;
 		const bool adv = nbad == 0;
 		cur.i = pi + (adv ? 1u : 0u);
