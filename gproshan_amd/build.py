@@ -7,6 +7,11 @@ tests, smoke() and bench.py load.
 Flags of note:
   * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: hipcc emits no packed
     FP32 (v_pk_*_f32) instructions.
+  * ``-mllvm -amdgpu-kernarg-preload-count=16``: a kernel's leading plain
+    arguments (up to 16 dwords) arrive in scalar registers at wave launch, so
+    the sweep kernels' first dependent load (the schedule slot) does not wait
+    for the kernel-argument segment.  Firmware without the feature runs the
+    compiler's compatibility prologue, which loads them as before.
   * ``-ffp-contract=off`` on the host units: the host FP64 update rounds like
     the reference's default x86-64 build.
 """
@@ -29,6 +34,7 @@ CPP_SOURCES = ["che.cpp", "ptp_cpu.cpp", "capi.cpp", "mesh_io.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-mllvm", "-amdgpu-kernarg-preload-count=16",
                "-Wno-unused-command-line-argument"]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wextra",
              "-Wno-unused-parameter", f"-I{ROCM}/include"]

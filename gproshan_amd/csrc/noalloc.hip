@@ -129,7 +129,7 @@ int ptp_gpu_noalloc(Che &mesh, const uint32_t *d_src, uint32_t ns, const uint32_
 	k_init<double><<<ig, 256, 0, stream>>>(d0, d1, n, st, nl < 3 ? 1u : 0u);
 	k_noalloc_sources<<<(ns + 255) / 256, 256, 0, stream>>>(d0, d1, d_src, ns, n);
 	for (uint32_t k = 0; k <= a.max_iter; k++)
-		k_sweep<true, false, MAXS, false, true><<<ds.grid, SWEEP_BLOCK, 0, stream>>>(a, k & 3u);
+		k_sweep<true, false, MAXS, false, true><<<ds.grid, SWEEP_BLOCK, 0, stream>>>(a.st, k & 3u, a);
 	k_noalloc_result<<<ig, 256, 0, stream>>>(d0, d1, st, n, d_dist);
 	GP_HIP(hipGetLastError());
 	return GP_OK;

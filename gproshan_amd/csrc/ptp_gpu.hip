@@ -735,7 +735,8 @@ static const void *sweep_kernel(const gp_plan *p, bool cl)
 static void launch_sweep(const gp_plan *p, const KArgs &a, uint32_t k4)
 {
 	KArgs ka = a;
-	void *args[] = { &ka, &k4 };
+	PState *st = a.st;
+	void *args[] = { &st, &k4, &ka };
 	(void) hipLaunchKernel(sweep_kernel(p, a.cl0 != nullptr), dim3(p->grid), dim3(p->block), args, 0, p->stream);
 }
 
@@ -755,11 +756,12 @@ static uint32_t kernel_bits(const gp_plan *p)
 static int capture(gp_plan *p, const KArgs &a, uint32_t count, hipGraphExec_t *out)
 {
 	KArgs ka = a;
+	PState *st = a.st;
 	const uint32_t k4s[4] = { 0, 1, 2, 3 };
 	const void *fn = sweep_kernel(p, a.cl0 != nullptr);
 	GraphChain gc;
 	for (uint32_t q = 0; q < count; q++) {
-		void *args[] = { &ka, (void *) &k4s[q & 3u] };
+		void *args[] = { &st, (void *) &k4s[q & 3u], &ka };
 		gc.add(fn, dim3(p->grid), dim3(p->block), args);
 	}
 	const hipError_t e = gc.instantiate(out);

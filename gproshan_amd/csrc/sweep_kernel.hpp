@@ -174,11 +174,14 @@ constexpr uint32_t SWEEP_BLOCK = 64;
 // over every position [0, npos) from buffer d into the other, counting into
 // check_bad the positions whose relative change is not below tol (an INF
 // value next to a finite neighbour counts).
+// The state pointer and k4 come first, as plain arguments, so that they can
+// be preloaded into scalar registers at wave launch: the schedule slot's load
+// then does not wait for the kernel-argument segment (a.st is the same
+// pointer).
 template <bool F64, bool CL, uint32_t MS, bool G64 = false, bool VM = false>
-__global__ __launch_bounds__(256) void k_sweep(KArgs a, uint32_t k4)
+__global__ __launch_bounds__(256) void k_sweep(PState *st, uint32_t k4, KArgs a)
 {
 	using T = typename std::conditional<F64, double, float>::type;
-	PState *st = a.st;
 	if (k4 >= 4u) {
 		const uint32_t d = k4 & 1u;
 		const T *cod = (const T *) (d ? a.dist1 : a.dist0);
