@@ -160,10 +160,28 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 		uint32_t id[MS];
 		G g[MS];
 		T t[MS];
+		if constexpr (F64) {
+			// Every slot id is requested before any 24-byte edge vector, all
+			// MS of each without branches (slots past S reread the lane's
+			// last row, which exists): the count of loads issued after each
+			// id is then fixed, so the neighbour gathers wait only for the
+			// ids (loads return in order) while the edge vectors are still
+			// arriving.  (FP32, with 8 bytes of geometry per slot, runs
+			// faster with the loads of a slot side by side and skipped past
+			// S, below.)
+			if (S == 0) return best;
 #pragma unroll
-		for (uint32_t k = 0; k < MS; k++) {
-			id[k] = 0;
-			if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
+			for (uint32_t k = 0; k < MS; k++) id[k] = a.cid[base + ST * (k < S ? k : S - 1)];
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) g[k] = geo[base + ST * (k < S ? k : S - 1)];
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) id[k] = k < S ? id[k] : 0u;
+		} else {
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) {
+				id[k] = 0;
+				if (k < S) { id[k] = a.cid[base + ST * k]; g[k] = geo[base + ST * k]; }
+			}
 		}
 		const bool wrap = (id[0] & SLOT_WRAP) != 0u;
 		// FP32 without labels, chunked tables: when every active lane holds a
