@@ -12,6 +12,27 @@ namespace gp {
 
 constexpr uint32_t MAXS = 10;  // register-resident slots; longer fans take the streaming loop
 
+// Raw buffer loads (the LLVM intrinsics, declared by name) through a
+// descriptor over [p, p + bytes): a load at or past `bytes` returns 0 and
+// touches no memory.  The descriptor's last word is the untyped-buffer format
+// of gfx9-family parts (32-bit data format, stride 0, raw offsets).
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ int raw_load_i32(__amdgpu_buffer_rsrc_t r, int voff, int soff, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.i32");
+__device__ f32x2_t raw_load_f32x2(__amdgpu_buffer_rsrc_t r, int voff, int soff, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v2f32");
+__device__ f64x2_t raw_load_f64x2(__amdgpu_buffer_rsrc_t r, int voff, int soff, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.v2f64");
+__device__ double raw_load_f64(__amdgpu_buffer_rsrc_t r, int voff, int soff, int aux) __asm(
+    "llvm.amdgcn.raw.ptr.buffer.load.f64");
+
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const T *p, uint32_t bytes)
+{
+	return __builtin_amdgcn_make_buffer_rsrc((void *) p, 0, (int) bytes, 0x00020000);
+}
+
 // FP64 update of a flagged triangle: slot at (its first vertex) and slot
 // at_n (its second vertex: the next slot, or the lane's slot 0 for the
 // triangle that closes a fan stored without its repeated first entry).
@@ -160,15 +181,35 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 		uint32_t id[MS];
 		G g[MS];
 		T t[MS];
-		if constexpr (F64) {
-			// Every slot id is requested before any 24-byte edge vector, all
-			// MS of each without branches (slots past S reread the lane's
-			// last row, which exists): the count of loads issued after each
-			// id is then fixed, so the neighbour gathers wait only for the
-			// ids (loads return in order) while the edge vectors are still
-			// arriving.  (FP32, with 8 bytes of geometry per slot, runs
-			// faster with the loads of a slot side by side and skipped past
-			// S, below.)
+		if constexpr (ST == 64) {
+			// Chunked tables: all MS slot ids, then all MS geometry rows,
+			// through descriptors that end at the chunk's S rows (a row past
+			// S reads as 0, so id[k] = 0 there, with no branch and no memory
+			// access).  The loads are straight-line and their count fixed, so
+			// the closed-fan test and the neighbour gathers wait only for the
+			// ids (loads return in order) while the geometry is arriving.
+			const uint32_t lane = threadIdx.x & 63u;
+			const size_t r0 = base - lane;   // the chunk's first slot, the same in every lane
+			const size_t rs = ((size_t) __builtin_amdgcn_readfirstlane((uint32_t) (r0 >> 32)) << 32) |
+			                  __builtin_amdgcn_readfirstlane((uint32_t) r0);
+			const __amdgpu_buffer_rsrc_t rid = rows_rsrc(a.cid + rs, S * 64u * 4u);
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) id[k] = (uint32_t) raw_load_i32(rid, (int) (lane * 4u + k * 256u), 0, 0);
+			const __amdgpu_buffer_rsrc_t rg = rows_rsrc(geo + rs, S * 64u * (uint32_t) sizeof(G));
+#pragma unroll
+			for (uint32_t k = 0; k < MS; k++) {
+				const int o = (int) ((lane + 64u * k) * (uint32_t) sizeof(G));
+				if constexpr (F64) {
+					const f64x2_t xy = raw_load_f64x2(rg, o, 0, 0);
+					g[k] = Edge64{ xy.x, xy.y, raw_load_f64(rg, o + 16, 0, 0) };
+				} else {
+					const f32x2_t v = raw_load_f32x2(rg, o, 0, 0);
+					g[k] = make_float2(v.x, v.y);
+				}
+			}
+		} else if constexpr (F64) {
+			// Vertex-major tables (a lane's own S): every slot id before any
+			// edge vector, slots past S rereading the lane's last row.
 			if (S == 0) return best;
 #pragma unroll
 			for (uint32_t k = 0; k < MS; k++) id[k] = a.cid[base + ST * (k < S ? k : S - 1)];
