@@ -348,7 +348,12 @@ __device__ uint32_t flow_chunk(const FlowArgs &f, uint32_t c, uint32_t k, uint32
 	bool lbad = false;
 	if (pos >= ws && pos < we && pos <= last) {
 		const float old = od[pos];
-		const uint32_t cb = a.sb ? c * a.sb : a.coff[c], S = a.sb ? a.sb : a.coff[c + 1] - cb;
+		uint32_t cb = c * a.sb, S = a.sb;
+		if (!a.sb) {   // both offsets requested together (one round trip)
+			const uint32_t e0 = a.coff[c], e1 = a.coff[c + 1];
+			cb = e0;
+			S = e1 - e0;
+		}
 		const float best = relax_lane<false, 64, true, false, MS, G64>(a, (size_t) cb * 64 + lane, S, od, old);
 		st_agent(pn + pos, ((unsigned long long) (k + 1u) << 32) | __float_as_uint(best));
 		if (pos < wc) lbad = !(fabs((double) best - (double) old) / (double) old < a.tol);

@@ -114,7 +114,15 @@ __device__ __forceinline__ uint32_t sweep_body(const KArgs &a, uint32_t start, u
 			bad += (uint32_t) __popcll(__ballot(lbad));
 			return;
 		}
-		const uint32_t cb = a.sb ? c * a.sb : a.coff[c], S = a.sb ? a.sb : a.coff[c + 1] - cb;
+		// Both chunk offsets are requested together, in one block: written as
+		// two selects on sb, the second load was issued only after the first
+		// had returned, one more round trip in the chain every sweep waits on.
+		uint32_t cb = c * a.sb, S = a.sb;
+		if (!a.sb) {
+			const uint32_t e0 = a.coff[c], e1 = a.coff[c + 1];
+			cb = e0;
+			S = e1 - e0;
+		}
 		if (pos >= start && pos < end) {
 			const uint32_t v = a.order ? a.order[pos] : pos;
 			const T old = od[v];
