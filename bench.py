@@ -53,7 +53,7 @@ ERR_BAR = 1e-4
 # (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00); see
 # profiles/k_sweep_f32_height_field_1M.md.  Keyed by (row, grid side, sources):
 # (bytes per dispatch, how they were measured).
-MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((8273.10 * 2.0 + 766.20 * 1.0) * 1024,
+MEASURED_TRAFFIC = {("band_learned", 1024, 1): ((8263.80 * 2.0 + 766.20 * 1.0) * 1024,
                                                 "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per dispatch, "
                                                 "mean of 6,640 dispatches of band_learned (0.5-edge bands, 8-byte "
                                                 "FP32 slot geometry, contiguous XCD eighths), kernels launched one "
