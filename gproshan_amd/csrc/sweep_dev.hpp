@@ -188,6 +188,8 @@ relax_lane(const A &a, size_t base, uint32_t S, const OD &od, typename std::cond
 			// access).  The loads are straight-line and their count fixed, so
 			// the closed-fan test and the neighbour gathers wait only for the
 			// ids (loads return in order) while the geometry is arriving.
+			// Every chunked caller passes base = (first slot row) * 64 + lane
+			// and one S for the whole wave, so the descriptors are uniform.
 			const uint32_t lane = threadIdx.x & 63u;
 			const size_t r0 = base - lane;   // the chunk's first slot, the same in every lane
 			const size_t rs = ((size_t) __builtin_amdgcn_readfirstlane((uint32_t) (r0 >> 32)) << 32) |
