@@ -18,14 +18,75 @@
 // face sizes and OFF/PLY faces with more than four corners are rejected.
 #include "gp_internal.hpp"
 
+#include <charconv>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace gp {
+
+// OFF and PLY files are read whole and parsed from memory: numbers with
+// std::from_chars, which rounds decimal text to the nearest double as the
+// stream extraction of the first version of these readers did (a leading '+'
+// is accepted as there; text that is not a finite number is an error, as the
+// stream's failure was).
+struct Text {
+	const char *p, *e;
+
+	static bool space(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
+	void skip()
+	{
+		while (p < e && space(*p)) p++;
+	}
+	bool word(std::string &w)
+	{
+		skip();
+		const char *b = p;
+		while (p < e && !space(*p)) p++;
+		w.assign(b, p);
+		return p > b;
+	}
+	template <typename T>
+	bool num(T &x)
+	{
+		skip();
+		const char *b = p;
+		if (e - b > 1 && *b == '+' && b[1] != '+' && b[1] != '-') b++;
+		const auto r = std::from_chars(b, e, x);
+		if constexpr (std::is_floating_point_v<T>) {
+			if (r.ec == std::errc::result_out_of_range) {
+				// Underflow reads as the nearest value (0 or a subnormal), as
+				// the stream took it; overflow stays an error.
+				const double v = std::strtod(std::string(b, r.ptr).c_str(), nullptr);
+				if (!std::isfinite(v)) return false;
+				x = (T) v;
+				p = r.ptr;
+				return true;
+			}
+		}
+		if (r.ec != std::errc()) return false;
+		if constexpr (std::is_floating_point_v<T>)
+			if (!std::isfinite(x)) return false;
+		p = r.ptr;
+		return true;
+	}
+	// The rest of the current line (without its '\n'), and the position moved
+	// past it; false at the end of the text.
+	bool line(Text &l)
+	{
+		if (p >= e) return false;
+		const char *n = static_cast<const char *>(std::memchr(p, '\n', (size_t) (e - p)));
+		l = { p, n ? n : e };
+		p = n ? n + 1 : e;
+		return true;
+	}
+};
 
 static bool ends_with(const std::string &s, const char *suf)
 {
@@ -36,38 +97,44 @@ static bool ends_with(const std::string &s, const char *suf)
 	return true;
 }
 
-static int read_off(std::istream &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
+static int read_off(Text &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
 {
 	std::string head;
 	uint64_t nv = 0, nf = 0, ne = 0;
-	if (!(is >> head) || head.find("OFF") == std::string::npos)
+	if (!is.word(head) || head.find("OFF") == std::string::npos)
 		return fail(GP_ERR_INVALID, "off: missing OFF header");
-	if (!(is >> nv >> nf >> ne)) return fail(GP_ERR_INVALID, "off: missing counts");
+	if (!(is.num(nv) && is.num(nf) && is.num(ne))) return fail(GP_ERR_INVALID, "off: missing counts");
 	if (nv >= SLOT_WRAP || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "off: counts too large");
 	const bool colour = head[0] == 'C', normal = head[0] == 'N';
 	xyz.resize(3 * nv);
 	for (uint64_t v = 0; v < nv; v++) {
-		if (!(is >> xyz[3 * v] >> xyz[3 * v + 1] >> xyz[3 * v + 2]))
+		if (!(is.num(xyz[3 * v]) && is.num(xyz[3 * v + 1]) && is.num(xyz[3 * v + 2])))
 			return fail(GP_ERR_INVALID, "off: vertex %llu unreadable", (unsigned long long) v);
 		double skip;
 		for (int q = 0; q < (colour ? 4 : normal ? 3 : 0); q++)
-			if (!(is >> skip)) return fail(GP_ERR_INVALID, "off: vertex %llu attributes unreadable", (unsigned long long) v);
+			if (!is.num(skip)) return fail(GP_ERR_INVALID, "off: vertex %llu attributes unreadable", (unsigned long long) v);
 	}
 	faces.clear();
 	faces.reserve(3 * nf);
 	uint64_t first = 0;
 	for (uint64_t f = 0; f < nf; f++) {
 		uint64_t k = 0;
-		if (!(is >> k)) return fail(GP_ERR_INVALID, "off: face %llu unreadable", (unsigned long long) f);
+		if (!is.num(k)) return fail(GP_ERR_INVALID, "off: face %llu unreadable", (unsigned long long) f);
 		if (f == 0) first = k;
 		if (k != first || (k != 3 && k != 4))
 			return fail(GP_ERR_INVALID, "off: face %llu has %llu corners (triangles or quads only, not mixed)",
 			            (unsigned long long) f, (unsigned long long) k);
 		uint32_t c[4];
 		for (uint64_t q = 0; q < k; q++)
-			if (!(is >> c[q])) return fail(GP_ERR_INVALID, "off: face %llu index unreadable", (unsigned long long) f);
-		faces.insert(faces.end(), { c[0], c[1], c[2] });
-		if (k == 4) faces.insert(faces.end(), { c[3], c[0], c[2] });
+			if (!is.num(c[q])) return fail(GP_ERR_INVALID, "off: face %llu index unreadable", (unsigned long long) f);
+		faces.push_back(c[0]);
+		faces.push_back(c[1]);
+		faces.push_back(c[2]);
+		if (k == 4) {
+			faces.push_back(c[3]);
+			faces.push_back(c[0]);
+			faces.push_back(c[2]);
+		}
 	}
 	return GP_OK;
 }
@@ -109,43 +176,59 @@ static int read_obj(std::istream &is, std::vector<double> &xyz, std::vector<uint
 	return GP_OK;
 }
 
-static size_t ply_size(const std::string &t)
+// PLY scalar types, decoded little-endian into a double (indices and counts
+// as unsigned where the type is).
+enum PlyType { PLY_NONE, PLY_I8, PLY_U8, PLY_I16, PLY_U16, PLY_I32, PLY_U32, PLY_F32, PLY_F64 };
+
+static PlyType ply_type(const std::string &t)
 {
-	if (t == "char" || t == "uchar" || t == "int8" || t == "uint8") return 1;
-	if (t == "short" || t == "ushort" || t == "int16" || t == "uint16") return 2;
-	if (t == "int" || t == "uint" || t == "float" || t == "float32" || t == "int32" || t == "uint32") return 4;
-	if (t == "double" || t == "float64") return 8;
-	return 0;
+	if (t == "char" || t == "int8") return PLY_I8;
+	if (t == "uchar" || t == "uint8") return PLY_U8;
+	if (t == "short" || t == "int16") return PLY_I16;
+	if (t == "ushort" || t == "uint16") return PLY_U16;
+	if (t == "int" || t == "int32") return PLY_I32;
+	if (t == "uint" || t == "uint32") return PLY_U32;
+	if (t == "float" || t == "float32") return PLY_F32;
+	if (t == "double" || t == "float64") return PLY_F64;
+	return PLY_NONE;
 }
 
-static bool ply_float(const std::string &t) { return t == "float" || t == "float32" || t == "double" || t == "float64"; }
-
-// Little-endian scalar of type t at p, as a double (indices and counts as
-// unsigned where the type is).
-static double ply_value(const unsigned char *p, const std::string &t)
+static size_t ply_size(PlyType t)
 {
-	if (t == "float" || t == "float32") { float x; std::memcpy(&x, p, 4); return x; }
-	if (t == "double" || t == "float64") { double x; std::memcpy(&x, p, 8); return x; }
-	if (t == "char" || t == "int8") return (double) (int8_t) p[0];
-	if (t == "uchar" || t == "uint8") return (double) p[0];
-	if (t == "short" || t == "int16") { int16_t x; std::memcpy(&x, p, 2); return x; }
-	if (t == "ushort" || t == "uint16") { uint16_t x; std::memcpy(&x, p, 2); return x; }
-	if (t == "int" || t == "int32") { int32_t x; std::memcpy(&x, p, 4); return x; }
-	uint32_t x;
-	std::memcpy(&x, p, 4);
-	return x;
+	switch (t) {
+	case PLY_I8: case PLY_U8: return 1;
+	case PLY_I16: case PLY_U16: return 2;
+	case PLY_I32: case PLY_U32: case PLY_F32: return 4;
+	case PLY_F64: return 8;
+	default: return 0;
+	}
 }
 
-static int read_ply(std::istream &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
+static double ply_value(const char *p, PlyType t)
 {
-	std::string line, format, element;
+	switch (t) {
+	case PLY_I8: return (double) (int8_t) p[0];
+	case PLY_U8: return (double) (uint8_t) p[0];
+	case PLY_I16: { int16_t x; std::memcpy(&x, p, 2); return x; }
+	case PLY_U16: { uint16_t x; std::memcpy(&x, p, 2); return x; }
+	case PLY_I32: { int32_t x; std::memcpy(&x, p, 4); return x; }
+	case PLY_U32: { uint32_t x; std::memcpy(&x, p, 4); return x; }
+	case PLY_F32: { float x; std::memcpy(&x, p, 4); return x; }
+	default: { double x; std::memcpy(&x, p, 8); return x; }
+	}
+}
+
+static int read_ply(Text &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
+{
+	std::string format, element;
 	uint64_t nv = 0, nf = 0;
-	std::vector<std::string> vtypes;          // vertex property types in order
-	std::string cnt_t, idx_t;                 // face list types
-	if (!std::getline(is, line) || line.rfind("ply", 0) != 0) return fail(GP_ERR_INVALID, "ply: missing header");
+	std::vector<PlyType> vtypes;              // vertex property types in order
+	PlyType cnt_t = PLY_NONE, idx_t = PLY_NONE;   // face list types
+	Text l;
+	if (!is.line(l) || std::string(l.p, l.e).rfind("ply", 0) != 0) return fail(GP_ERR_INVALID, "ply: missing header");
 	bool end = false;
-	while (std::getline(is, line)) {
-		std::istringstream ss(line);
+	while (is.line(l)) {
+		std::istringstream ss(std::string(l.p, l.e));
 		std::string w;
 		ss >> w;
 		if (w == "end_header") { end = true; break; }
@@ -157,63 +240,69 @@ static int read_ply(std::istream &is, std::vector<double> &xyz, std::vector<uint
 		} else if (w == "property" && element == "vertex") {
 			std::string t;
 			ss >> t;
-			if (!ply_size(t)) return fail(GP_ERR_INVALID, "ply: vertex property type '%s'", t.c_str());
-			vtypes.push_back(t);
+			if (!ply_type(t)) return fail(GP_ERR_INVALID, "ply: vertex property type '%s'", t.c_str());
+			vtypes.push_back(ply_type(t));
 		} else if (w == "property" && element == "face") {
-			std::string t;
+			std::string t, c, i;
 			ss >> t;
-			if (t == "list") ss >> cnt_t >> idx_t;
+			if (t == "list") {
+				ss >> c >> i;
+				cnt_t = ply_type(c);
+				idx_t = ply_type(i);
+			}
 		}
 	}
 	if (!end) return fail(GP_ERR_INVALID, "ply: no end_header");
-	if (vtypes.size() < 3 || !ply_float(vtypes[0])) return fail(GP_ERR_INVALID, "ply: x y z must be the first vertex properties");
+	const auto fl = [](PlyType t) { return t == PLY_F32 || t == PLY_F64; };
+	if (vtypes.size() < 3 || !fl(vtypes[0])) return fail(GP_ERR_INVALID, "ply: x y z must be the first vertex properties");
 	if (nv >= SLOT_WRAP || nf > (1ull << 31)) return fail(GP_ERR_CAPACITY, "ply: counts too large");
-	if (nf && (!ply_size(cnt_t) || !ply_size(idx_t))) return fail(GP_ERR_INVALID, "ply: face list types");
+	if (nf && (!cnt_t || !idx_t)) return fail(GP_ERR_INVALID, "ply: face list types");
 	xyz.resize(3 * nv);
 	faces.clear();
 	faces.reserve(3 * nf);
 	if (format == "ascii") {
+		// One vertex or face per line; what follows the values read is ignored.
 		for (uint64_t v = 0; v < nv; v++) {
-			if (!std::getline(is, line)) return fail(GP_ERR_INVALID, "ply: vertex %llu missing", (unsigned long long) v);
-			std::istringstream ss(line);
-			if (!(ss >> xyz[3 * v] >> xyz[3 * v + 1] >> xyz[3 * v + 2]))
+			if (!is.line(l)) return fail(GP_ERR_INVALID, "ply: vertex %llu missing", (unsigned long long) v);
+			if (!(l.num(xyz[3 * v]) && l.num(xyz[3 * v + 1]) && l.num(xyz[3 * v + 2])))
 				return fail(GP_ERR_INVALID, "ply: vertex %llu unreadable", (unsigned long long) v);
 		}
 		for (uint64_t f = 0; f < nf; f++) {
-			if (!std::getline(is, line)) return fail(GP_ERR_INVALID, "ply: face %llu missing", (unsigned long long) f);
-			std::istringstream ss(line);
+			if (!is.line(l)) return fail(GP_ERR_INVALID, "ply: face %llu missing", (unsigned long long) f);
 			uint64_t k = 0;
 			uint32_t c[3];
-			if (!(ss >> k) || k != 3 || !(ss >> c[0] >> c[1] >> c[2]))
+			if (!l.num(k) || k != 3 || !(l.num(c[0]) && l.num(c[1]) && l.num(c[2])))
 				return fail(GP_ERR_INVALID, "ply: face %llu is not a triangle", (unsigned long long) f);
-			faces.insert(faces.end(), { c[0], c[1], c[2] });
+			faces.push_back(c[0]);
+			faces.push_back(c[1]);
+			faces.push_back(c[2]);
 		}
 	} else if (format == "binary_little_endian") {
 		size_t vbytes = 0;
-		for (auto &t : vtypes) vbytes += ply_size(t);
-		std::vector<unsigned char> buf(std::max<size_t>(vbytes, 8));
+		for (PlyType t : vtypes) vbytes += ply_size(t);
 		const size_t s0 = ply_size(vtypes[0]), s1 = ply_size(vtypes[1]);
-		for (uint64_t v = 0; v < nv; v++) {
-			if (!is.read((char *) buf.data(), (std::streamsize) vbytes))
-				return fail(GP_ERR_INVALID, "ply: vertex %llu truncated", (unsigned long long) v);
-			xyz[3 * v] = ply_value(buf.data(), vtypes[0]);
-			xyz[3 * v + 1] = ply_value(buf.data() + s0, vtypes[1]);
-			xyz[3 * v + 2] = ply_value(buf.data() + s0 + s1, vtypes[2]);
+		const char *p = is.p;
+		for (uint64_t v = 0; v < nv; v++, p += vbytes) {
+			if ((size_t) (is.e - p) < vbytes) return fail(GP_ERR_INVALID, "ply: vertex %llu truncated", (unsigned long long) v);
+			xyz[3 * v] = ply_value(p, vtypes[0]);
+			xyz[3 * v + 1] = ply_value(p + s0, vtypes[1]);
+			xyz[3 * v + 2] = ply_value(p + s0 + s1, vtypes[2]);
 		}
 		const size_t cs = ply_size(cnt_t), is_ = ply_size(idx_t);
+		faces.resize(3 * nf);
 		for (uint64_t f = 0; f < nf; f++) {
-			if (!is.read((char *) buf.data(), (std::streamsize) cs))
-				return fail(GP_ERR_INVALID, "ply: face %llu truncated", (unsigned long long) f);
-			const double k = ply_value(buf.data(), cnt_t);
+			if ((size_t) (is.e - p) < cs) return fail(GP_ERR_INVALID, "ply: face %llu truncated", (unsigned long long) f);
+			const double k = ply_value(p, cnt_t);
+			p += cs;
 			if (k != 3) return fail(GP_ERR_INVALID, "ply: face %llu is not a triangle", (unsigned long long) f);
-			for (int q = 0; q < 3; q++) {
-				if (!is.read((char *) buf.data(), (std::streamsize) is_))
-					return fail(GP_ERR_INVALID, "ply: face %llu truncated", (unsigned long long) f);
-				const double i = ply_value(buf.data(), idx_t);
+			if ((size_t) (is.e - p) < 3 * is_) return fail(GP_ERR_INVALID, "ply: face %llu truncated", (unsigned long long) f);
+			for (int q = 0; q < 3; q++, p += is_) {
+				const double i = ply_value(p, idx_t);
 				if (i < 0) return fail(GP_ERR_INVALID, "ply: face %llu negative index", (unsigned long long) f);
-				faces.push_back((uint32_t) i);
+				faces[3 * f + q] = (uint32_t) i;
 			}
 		}
+		is.p = p;
 	} else {
 		return fail(GP_ERR_INVALID, "ply: format '%s' not supported", format.c_str());
 	}
@@ -228,10 +317,19 @@ int read_mesh(const char *path, std::vector<double> &xyz, std::vector<uint32_t> 
 	if (!is.good()) return fail(GP_ERR_INVALID, "mesh file: cannot open '%s'", path);
 	xyz.clear();
 	faces.clear();
-	if (ends_with(p, ".off")) return read_off(is, xyz, faces);
+	const bool off = ends_with(p, ".off"), ply = ends_with(p, ".ply");
 	if (ends_with(p, ".obj")) return read_obj(is, xyz, faces);
-	if (ends_with(p, ".ply")) return read_ply(is, xyz, faces);
-	return fail(GP_ERR_INVALID, "mesh file: unknown extension of '%s' (.off, .obj, .ply)", path);
+	if (!off && !ply) return fail(GP_ERR_INVALID, "mesh file: unknown extension of '%s' (.off, .obj, .ply)", path);
+	// The whole file in one read.
+	std::string buf;
+	is.seekg(0, std::ios::end);
+	const std::streamoff size = is.tellg();
+	if (size < 0) return fail(GP_ERR_INVALID, "mesh file: cannot size '%s'", path);
+	buf.resize((size_t) size);
+	is.seekg(0, std::ios::beg);
+	if (size && !is.read(&buf[0], size)) return fail(GP_ERR_INVALID, "mesh file: cannot read '%s'", path);
+	Text t{ buf.data(), buf.data() + buf.size() };
+	return off ? read_off(t, xyz, faces) : read_ply(t, xyz, faces);
 }
 
 }  // namespace gp

@@ -144,6 +144,41 @@ def test_read_mesh_ptp_gpu_against_oracle(tmp_path, oracle, fmt):
     assert np.max(np.abs(g32.dist[m] - ref[m]) / ref[m]) <= 1e-4
 
 
+def test_number_forms_and_line_endings(tmp_path):
+    # The OFF and ASCII PLY numbers are parsed from memory: each must read as
+    # Python's correctly rounded float() of the same text (a leading '+',
+    # '.5', '5.', exponents, -0, subnormals, an underflow to 0), CRLF line
+    # ends change nothing, and text that is not a finite number is an error.
+    words = [["+1.5", ".5", "5."], ["1e-400", "-0", "2.5E+3"],
+             ["4.9406564584124654e-324", "1e-310", "-7"], ["0.1", "0.2", "0.30000000000000004"]]
+    want = np.array([[float(w) for w in row] for row in words])
+    body = "".join(" ".join(row) + "\n" for row in words)
+    for name, eol in (("lf.off", "\n"), ("crlf.off", "\r\n")):
+        (tmp_path / name).write_bytes(("OFF\n4 2 0\n" + body + "3 0 1 2\n3 0 2 3\n").replace("\n", eol).encode())
+        C = P.che.read(tmp_path / name)
+        assert np.array_equal(C.gt().view(np.uint64), want.view(np.uint64))
+        assert _faces(C) == [[0, 1, 2], [0, 2, 3]]
+    head = "ply\nformat ascii 1.0\nelement vertex 4\nproperty double x\nproperty double y\nproperty double z\n" \
+           "property uchar red\nelement face 2\nproperty list uchar int vertex_indices\nend_header\n"
+    ply = head + "".join(" ".join(row) + " 255\n" for row in words) + "3 0 1 2 9\n3 0 2 3\n"
+    (tmp_path / "crlf.ply").write_bytes(ply.replace("\n", "\r\n").encode())
+    C = P.che.read(tmp_path / "crlf.ply")
+    assert np.array_equal(C.gt().view(np.uint64), want.view(np.uint64))
+    assert _faces(C) == [[0, 1, 2], [0, 2, 3]]
+    for bad in ("inf 0 0", "nan 0 0", "1e400 0 0", "1.5x 0 0", "0 0"):
+        (tmp_path / "bad.off").write_text("OFF\n3 1 0\n" + bad + "\n1 0 0\n0 1 0\n3 0 1 2\n")
+        with pytest.raises(P.GPError) as e:
+            P.che.read(tmp_path / "bad.off")
+        assert e.value.status == L.ERR_INVALID, bad
+    # A binary PLY cut inside its faces is an error, not a short read.
+    _ply_binary(tmp_path / "cut.ply", "double", "uchar", "int", SQUARE[:4], [[0, 1, 2], [0, 2, 3]])
+    data = (tmp_path / "cut.ply").read_bytes()
+    (tmp_path / "cut.ply").write_bytes(data[:-5])
+    with pytest.raises(P.GPError) as e:
+        P.che.read(tmp_path / "cut.ply")
+    assert e.value.status == L.ERR_INVALID
+
+
 def test_unknown_extension_and_missing_file(tmp_path):
     (tmp_path / "m.stl").write_text("solid\n")
     for p in (tmp_path / "m.stl", tmp_path / "none.off"):
