@@ -31,7 +31,7 @@
 
 namespace gp {
 
-// OFF and PLY files are read whole and parsed from memory: numbers with
+// Mesh files are read whole and parsed from memory: numbers with
 // std::from_chars, which rounds decimal text to the nearest double as the
 // stream extraction of the first version of these readers did (a leading '+'
 // is accepted as there; text that is not a finite number is an error, as the
@@ -139,30 +139,39 @@ static int read_off(Text &is, std::vector<double> &xyz, std::vector<uint32_t> &f
 	return GP_OK;
 }
 
-static int read_obj(std::istream &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
+static int read_obj(Text &is, std::vector<double> &xyz, std::vector<uint32_t> &faces)
 {
-	std::string line;
 	uint64_t ln = 0;
 	std::vector<int64_t> idx;
-	while (std::getline(is, line)) {
+	std::string key;
+	Text l;
+	while (is.line(l)) {
 		ln++;
-		std::istringstream ss(line);
-		std::string key;
-		if (!(ss >> key)) continue;
+		if (!l.word(key)) continue;
 		if (key == "v") {
 			double x, y, z;
-			if (!(ss >> x >> y >> z)) return fail(GP_ERR_INVALID, "obj: line %llu: bad vertex", (unsigned long long) ln);
-			xyz.insert(xyz.end(), { x, y, z });
+			if (!(l.num(x) && l.num(y) && l.num(z))) return fail(GP_ERR_INVALID, "obj: line %llu: bad vertex", (unsigned long long) ln);
+			xyz.push_back(x);
+			xyz.push_back(y);
+			xyz.push_back(z);
 		} else if (key == "f") {
 			idx.clear();
-			std::string tok;
-			while (ss >> tok) {
-				const size_t slash = tok.find('/');
-				const std::string head = tok.substr(0, slash);
-				char *end = nullptr;
-				const long long i = std::strtoll(head.c_str(), &end, 10);
-				if (head.empty() || *end != '\0' || i <= 0)
+			for (;;) {
+				// The vertex index: the token up to its first '/', a positive
+				// decimal integer (an optional '+' sign) and nothing else.
+				l.skip();
+				if (l.p >= l.e) break;
+				const char *t0 = l.p;
+				while (l.p < l.e && !Text::space(*l.p)) l.p++;
+				const char *slash = static_cast<const char *>(std::memchr(t0, '/', (size_t) (l.p - t0)));
+				const char *b = t0, *e = slash ? slash : l.p;
+				if (e - b > 1 && *b == '+') b++;
+				long long i = 0;
+				const auto r = std::from_chars(b, e, i);
+				if (b == e || r.ec != std::errc() || r.ptr != e || i <= 0) {
+					const std::string tok(t0, l.p);
 					return fail(GP_ERR_INVALID, "obj: line %llu: bad face index '%s'", (unsigned long long) ln, tok.c_str());
+				}
 				idx.push_back(i - 1);
 			}
 			if (idx.size() == 3) {
@@ -317,9 +326,8 @@ int read_mesh(const char *path, std::vector<double> &xyz, std::vector<uint32_t> 
 	if (!is.good()) return fail(GP_ERR_INVALID, "mesh file: cannot open '%s'", path);
 	xyz.clear();
 	faces.clear();
-	const bool off = ends_with(p, ".off"), ply = ends_with(p, ".ply");
-	if (ends_with(p, ".obj")) return read_obj(is, xyz, faces);
-	if (!off && !ply) return fail(GP_ERR_INVALID, "mesh file: unknown extension of '%s' (.off, .obj, .ply)", path);
+	const bool off = ends_with(p, ".off"), obj = ends_with(p, ".obj"), ply = ends_with(p, ".ply");
+	if (!off && !obj && !ply) return fail(GP_ERR_INVALID, "mesh file: unknown extension of '%s' (.off, .obj, .ply)", path);
 	// The whole file in one read.
 	std::string buf;
 	is.seekg(0, std::ios::end);
@@ -329,7 +337,7 @@ int read_mesh(const char *path, std::vector<double> &xyz, std::vector<uint32_t> 
 	is.seekg(0, std::ios::beg);
 	if (size && !is.read(&buf[0], size)) return fail(GP_ERR_INVALID, "mesh file: cannot read '%s'", path);
 	Text t{ buf.data(), buf.data() + buf.size() };
-	return off ? read_off(t, xyz, faces) : read_ply(t, xyz, faces);
+	return off ? read_off(t, xyz, faces) : obj ? read_obj(t, xyz, faces) : read_ply(t, xyz, faces);
 }
 
 }  // namespace gp
