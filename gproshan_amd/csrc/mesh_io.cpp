@@ -19,11 +19,13 @@
 #include "gp_internal.hpp"
 
 #include <charconv>
+#include <clocale>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <locale.h>
 #include <sstream>
 #include <string>
 #include <type_traits>
@@ -61,9 +63,12 @@ struct Text {
 		const auto r = std::from_chars(b, e, x);
 		if constexpr (std::is_floating_point_v<T>) {
 			if (r.ec == std::errc::result_out_of_range) {
-				// Underflow reads as the nearest value (0 or a subnormal), as
-				// the stream took it; overflow stays an error.
-				const double v = std::strtod(std::string(b, r.ptr).c_str(), nullptr);
+				// Underflow reads as the nearest value (a signed 0), as the
+				// stream took it; overflow stays an error.  strtod_l in the
+				// "C" locale, whatever locale the host program has set.
+				static const locale_t c_locale = newlocale(LC_ALL_MASK, "C", (locale_t) 0);
+				if (!c_locale) return false;
+				const double v = strtod_l(std::string(b, r.ptr).c_str(), nullptr, c_locale);
 				if (!std::isfinite(v)) return false;
 				x = (T) v;
 				p = r.ptr;
